@@ -1,0 +1,17 @@
+"""gardenia_ocl_amd -- MI355X-native pull PageRank and CSR SpMV.
+
+A drop-in for the PageRank (src/pr) and SpMV (src/spmv) solver plug-ins of
+gardenia_ocl: hand-written HIP kernels for gfx950 behind a C ABI
+(include/gardenia_hip.h, include/gardenia_graph.h), with Python mirrors of the
+reference's ``PRSolver`` / ``SpmvSolver`` and device-resident plans over torch
+tensors.
+"""
+from . import graph
+from ._native import GardeniaError, LIB_PATH
+from .pr import PRSolver, PageRankPlan, EPSILON, K_DAMP, MAX_ITER
+from .spmv import SpmvSolver, SpmvPlan
+
+__all__ = [
+    "graph", "GardeniaError", "LIB_PATH", "PRSolver", "PageRankPlan", "SpmvSolver", "SpmvPlan",
+    "EPSILON", "K_DAMP", "MAX_ITER",
+]

@@ -1,0 +1,77 @@
+// gd_common.h -- internal helpers shared by the HIP translation units.
+#ifndef GD_COMMON_H_
+#define GD_COMMON_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "gardenia_graph.h"
+
+namespace gd {
+
+// Thread-local text of the last failure, returned by gd_last_error().
+void set_error(const std::string& msg);
+
+#define GD_HIP_TRY(expr)                                                              \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess) {                                                           \
+      ::gd::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));             \
+      return GD_ERR_DEVICE;                                                           \
+    }                                                                                 \
+  } while (0)
+
+#define GD_CHECK_ARG(cond, msg)                \
+  do {                                         \
+    if (!(cond)) {                             \
+      ::gd::set_error(msg);                    \
+      return GD_ERR_INVALID;                   \
+    }                                          \
+  } while (0)
+
+// Work unit of the pull engine (one workgroup each).
+//   row tile:    x = first row, y = end row (exclusive) > x, z = first edge, w = end edge
+//   hub segment: x = row, y = -(partial slot + 1) < 0,     z = first edge, w = end edge
+struct Tile {
+  int32_t x, y, z, w;
+};
+
+// Row-block schedule over a CSR (CSR-Adaptive style, Greathouse & Daga SC'14):
+// consecutive rows are packed into tiles of at most `tile_nnz` edges and
+// `tile_rows` rows; a row longer than `tile_nnz` becomes a hub and is split
+// into ceil(deg/tile_nnz) segments whose partial sums are combined, in segment
+// order, by a finalize pass.
+struct Schedule {
+  std::vector<Tile> tiles;
+  std::vector<int32_t> hub_row, hub_slot, hub_nseg;
+  int32_t n_slots = 0;
+  int32_t max_row_len = 0;
+};
+
+// Builds the schedule from host row offsets.  Returns GD_ERR_INVALID (and sets
+// the error text) if the offsets are not a valid CSR prefix sum of length m+1.
+int build_schedule(int m, int64_t nnz, const int32_t* row_offsets, int tile_nnz, int tile_rows,
+                   Schedule* out);
+
+// Checks on the device that every column index lies in [0, ncols); returns
+// GD_OK, GD_ERR_INVALID or GD_ERR_DEVICE.
+int check_columns(const int32_t* d_col, int64_t nnz, int32_t ncols, hipStream_t stream);
+
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  ~DeviceBuffer() { release(); }
+  int alloc(size_t n);
+  void release();
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+};
+
+}  // namespace gd
+
+#endif  // GD_COMMON_H_

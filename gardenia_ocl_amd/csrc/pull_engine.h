@@ -1,0 +1,212 @@
+// pull_engine.h -- device building blocks of the CSR pull engine (wave64).
+//
+// A workgroup of kBlock threads owns one Tile.  Latency is paid once per tile:
+// every load the tile needs (edge indices, row offsets, the epilogue's
+// per-row operands) is issued unconditionally up front -- addresses of idle
+// lanes are clamped into the tile instead of branching -- and only then does
+// the tile wait, gather (IPT independent gathers in flight per thread), stage
+// the gathered values in LDS and reduce rows:
+//   * rows of <= kSeqRow edges: the owning thread, sequentially in CSR order
+//     (the reference's summation order, so the result is bit-identical);
+//   * longer rows: one wavefront, lane-strided partials + xor butterfly; the
+//     sum goes back through LDS to the owning thread for the epilogue.
+// Hub segments gather kBlock*IPT edges of one long row into a partial sum.
+//
+// Row r of the tile is owned by thread (r - r0) % kBlock, so per-row reads and
+// writes are coalesced.
+#ifndef GD_PULL_ENGINE_H_
+#define GD_PULL_ENGINE_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gd_common.h"
+
+namespace gd {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+constexpr int kWaves = kBlock / kWave;
+constexpr int kTileRows = 1024;
+constexpr int kRowsPerThread = kTileRows / kBlock;
+constexpr int kOffLoads = (kTileRows + 1 + kBlock - 1) / kBlock;
+constexpr int kSeqRow = 64;
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Sum of one value per thread over a workgroup of NWAVES waves, fixed order;
+// result valid in thread 0.  `s_tmp` holds NWAVES values.
+template <int NWAVES, typename T>
+__device__ __forceinline__ T block_sum(T v, T* s_tmp) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (lane == 0) s_tmp[wid] = v;
+  __syncthreads();
+  T t = T(0);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int w = 0; w < NWAVES; w++) t += s_tmp[w];
+  return t;
+}
+
+// Sequential sum s + v[b] + ... + v[e-1] from LDS, 4 reads in flight.
+__device__ __forceinline__ float seq_sum(float s, const float* __restrict__ v, int b, int e) {
+  int k = b;
+  for (; k + 4 <= e; k += 4) {
+    const float a0 = v[k], a1 = v[k + 1], a2 = v[k + 2], a3 = v[k + 3];
+    s += a0;
+    s += a1;
+    s += a2;
+    s += a3;
+  }
+  for (; k < e; k++) s += v[k];
+  return s;
+}
+
+// Loads the tile's edge indices col[e0 + tid + j*kBlock] (clamped into the
+// tile so all IPT loads issue back to back).  Requires ne >= 1.
+template <int IPT>
+__device__ __forceinline__ void load_indices(const int32_t* __restrict__ col, int e0, int ne,
+                                             int (&idx)[IPT]) {
+  const int last = ne - 1;
+#pragma unroll
+  for (int j = 0; j < IPT; j++) idx[j] = col[e0 + min((int)threadIdx.x + j * kBlock, last)];
+}
+
+template <int IPT, class G>
+__device__ __forceinline__ void gather_values(const G& g, int e0, int ne, const int (&idx)[IPT],
+                                              float (&v)[IPT]) {
+  const int last = ne - 1;
+#pragma unroll
+  for (int j = 0; j < IPT; j++) v[j] = g.value(e0 + min((int)threadIdx.x + j * kBlock, last), idx[j]);
+}
+
+// Sum of the gathered values of a hub segment [e0, e0+ne), ne >= 1: per-thread
+// partial in j order, wave butterfly, then waves in order.  Valid in thread 0.
+template <int IPT, class G>
+__device__ __forceinline__ float segment_sum(const int32_t* __restrict__ col, int e0, int ne,
+                                             const G& g, float* s_tmp) {
+  int idx[IPT];
+  float v[IPT];
+  load_indices<IPT>(col, e0, ne, idx);
+  gather_values<IPT>(g, e0, ne, idx, v);
+  float s = 0.0f;
+#pragma unroll
+  for (int j = 0; j < IPT; j++) s += ((int)threadIdx.x + j * kBlock < ne) ? v[j] : 0.0f;
+  return block_sum<kWaves>(s, s_tmp);
+}
+
+// LDS of one row tile.
+template <int IPT>
+struct RowTileLds {
+  float val[kBlock * IPT];    // gathered per-edge values
+  int off[kTileRows + 1];     // row offsets, rebased to the tile on use
+  float longsum[kTileRows];   // wavefront sums of rows longer than kSeqRow
+};
+
+// One row tile: rows [r0, r1) whose edges are [e0, e0+ne).  The epilogue E
+// provides
+//   typename E::Pre          per-row operands, loaded before the gather
+//   Pre pre(int row)         (clamped row: always a valid index)
+//   float init(const Pre&)   value the row sum starts from
+//   double finish(int row, float sum, const Pre&)   writes the row, returns
+//                            its contribution to the tile's double accumulator
+template <int IPT, class G, class E>
+__device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
+                                           const int32_t* __restrict__ col, int r0, int r1,
+                                           int e0, int ne, const G& g, const E& e,
+                                           RowTileLds<IPT>& lds) {
+  const int tid = threadIdx.x;
+  const int nr = r1 - r0;
+  // --- phase A: every independent load of the tile
+  int idx[IPT];
+  if (ne > 0) load_indices<IPT>(col, e0, ne, idx);
+  int offv[kOffLoads];
+#pragma unroll
+  for (int q = 0; q < kOffLoads; q++) offv[q] = off[r0 + min(tid + q * kBlock, nr)];
+  typename E::Pre pre[kRowsPerThread];
+#pragma unroll
+  for (int q = 0; q < kRowsPerThread; q++) pre[q] = e.pre(r0 + min(tid + q * kBlock, nr - 1));
+  // --- phase B: gathers (depend on the indices only)
+  float v[IPT];
+  if (ne > 0) gather_values<IPT>(g, e0, ne, idx, v);
+#pragma unroll
+  for (int q = 0; q < kOffLoads; q++)
+    if (tid + q * kBlock <= nr) lds.off[tid + q * kBlock] = offv[q] - e0;
+  if (ne > 0) {
+#pragma unroll
+    for (int j = 0; j < IPT; j++)
+      if (tid + j * kBlock < ne) lds.val[tid + j * kBlock] = v[j];
+  }
+  __syncthreads();
+  // --- phase C: short rows by their owner thread, in CSR order
+  float sum[kRowsPerThread];
+  bool have_long = false;
+#pragma unroll
+  for (int q = 0; q < kRowsPerThread; q++) {
+    const int i = tid + q * kBlock;
+    sum[q] = 0.0f;
+    if (i < nr) {
+      const int b = lds.off[i], en = lds.off[i + 1];
+      if (en - b <= kSeqRow)
+        sum[q] = seq_sum(e.init(pre[q]), lds.val, b, en);
+      else
+        have_long = true;
+    }
+  }
+  // --- phase D: long rows, one wavefront each; sums to LDS
+  if (__syncthreads_or(have_long)) {
+    const int lane = tid & (kWave - 1), wid = tid / kWave;
+    for (int c = wid * kWave; c < nr; c += kBlock) {
+      const int i = c + lane;
+      const bool lr = (i < nr) && (lds.off[i + 1] - lds.off[i] > kSeqRow);
+      uint64_t mask = __ballot(lr);
+      while (mask) {
+        const int row = c + __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int b = lds.off[row], en = lds.off[row + 1];
+        float s = 0.0f;
+        int k = b + lane;
+        for (; k + 3 * kWave < en; k += 4 * kWave) {
+          const float a0 = lds.val[k], a1 = lds.val[k + kWave], a2 = lds.val[k + 2 * kWave],
+                      a3 = lds.val[k + 3 * kWave];
+          s += a0;
+          s += a1;
+          s += a2;
+          s += a3;
+        }
+        for (; k < en; k += kWave) s += lds.val[k];
+        s = wave_sum(s);
+        if (lane == 0) lds.longsum[row] = s;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kRowsPerThread; q++) {
+      const int i = tid + q * kBlock;
+      if (i < nr && lds.off[i + 1] - lds.off[i] > kSeqRow) sum[q] = e.init(pre[q]) + lds.longsum[i];
+    }
+  }
+  // --- phase E: epilogue by the owner thread (coalesced)
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < kRowsPerThread; q++) {
+    const int i = tid + q * kBlock;
+    if (i < nr) acc += e.finish(r0 + i, sum[q], pre[q]);
+  }
+  return acc;
+}
+
+}  // namespace gd
+
+#endif  // GD_PULL_ENGINE_H_

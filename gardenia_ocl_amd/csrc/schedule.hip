@@ -1,0 +1,117 @@
+// schedule.hip -- host schedule builder, CSR validation kernel, buffers, errors.
+#include <cstdio>
+#include <string>
+
+#include "gd_common.h"
+
+namespace gd {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int DeviceBuffer::alloc(size_t n) {
+  release();
+  if (n == 0) n = 256;
+  hipError_t e = hipMalloc(&ptr, n);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    set_error(std::string("hipMalloc(") + std::to_string(n) + "): " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? GD_ERR_ALLOC : GD_ERR_DEVICE;
+  }
+  bytes = n;
+  return GD_OK;
+}
+
+void DeviceBuffer::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+int build_schedule(int m, int64_t nnz, const int32_t* off, int tile_nnz, int tile_rows,
+                   Schedule* s) {
+  s->tiles.clear();
+  s->hub_row.clear();
+  s->hub_slot.clear();
+  s->hub_nseg.clear();
+  s->n_slots = 0;
+  s->max_row_len = 0;
+  if (m < 0 || nnz < 0 || tile_nnz <= 0 || tile_rows <= 0) {
+    set_error("build_schedule: bad sizes");
+    return GD_ERR_INVALID;
+  }
+  if (off[0] != 0 || (int64_t)off[m] != nnz) {
+    set_error("row_offsets[0] must be 0 and row_offsets[m] must equal nnz");
+    return GD_ERR_INVALID;
+  }
+  s->tiles.reserve((size_t)(nnz / tile_nnz + m / tile_rows + 16));
+  int32_t cur = 0, cur_nnz = 0;
+  for (int32_t r = 0; r < m; r++) {
+    const int32_t d = off[r + 1] - off[r];
+    if (d < 0) {
+      set_error("row_offsets must be non-decreasing (row " + std::to_string(r) + ")");
+      return GD_ERR_INVALID;
+    }
+    if (d > s->max_row_len) s->max_row_len = d;
+    if (d > tile_nnz) {
+      if (r > cur) s->tiles.push_back(Tile{cur, r, off[cur], off[r]});
+      const int32_t nseg = (d + tile_nnz - 1) / tile_nnz;
+      s->hub_row.push_back(r);
+      s->hub_slot.push_back(s->n_slots);
+      s->hub_nseg.push_back(nseg);
+      for (int32_t k = 0; k < nseg; k++) {
+        const int32_t e0 = off[r] + k * tile_nnz;
+        const int32_t e1 = (k + 1 == nseg) ? off[r + 1] : e0 + tile_nnz;
+        s->tiles.push_back(Tile{r, -(s->n_slots + k + 1), e0, e1});
+      }
+      s->n_slots += nseg;
+      cur = r + 1;
+      cur_nnz = 0;
+      continue;
+    }
+    if (r > cur && (cur_nnz + d > tile_nnz || r - cur >= tile_rows)) {
+      s->tiles.push_back(Tile{cur, r, off[cur], off[r]});
+      cur = r;
+      cur_nnz = 0;
+    }
+    cur_nnz += d;
+  }
+  if (m > cur) s->tiles.push_back(Tile{cur, m, off[cur], off[m]});
+  return GD_OK;
+}
+
+__global__ void check_columns_kernel(const int32_t* __restrict__ col, int64_t nnz, int32_t ncols,
+                                     int* __restrict__ bad) {
+  int found = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c = col[i];
+    found |= (c < 0) | (c >= ncols);
+  }
+  if (__any(found) && (threadIdx.x & 63) == 0) atomicOr(bad, 1);
+}
+
+int check_columns(const int32_t* d_col, int64_t nnz, int32_t ncols, hipStream_t stream) {
+  if (nnz == 0) return GD_OK;
+  DeviceBuffer flag;
+  int rc = flag.alloc(sizeof(int));
+  if (rc) return rc;
+  GD_HIP_TRY(hipMemsetAsync(flag.ptr, 0, sizeof(int), stream));
+  int blocks = (int)std::min<int64_t>((nnz + 255) / 256, 4096);
+  hipLaunchKernelGGL(check_columns_kernel, dim3(blocks), dim3(256), 0, stream, d_col, nnz, ncols,
+                     flag.as<int>());
+  GD_HIP_TRY(hipGetLastError());
+  int h = 0;
+  GD_HIP_TRY(hipMemcpyAsync(&h, flag.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
+  GD_HIP_TRY(hipStreamSynchronize(stream));
+  if (h) {
+    set_error("column index out of range [0, ncols)");
+    return GD_ERR_INVALID;
+  }
+  return GD_OK;
+}
+
+}  // namespace gd
+
+extern "C" const char* gd_last_error(void) { return gd::g_last_error.c_str(); }

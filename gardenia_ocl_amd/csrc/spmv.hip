@@ -1,0 +1,233 @@
+// spmv.hip -- CSR SpMV y <- y + A x for gfx950 (src/spmv/omp_base.cc:17-28).
+//
+// Same tile engine as PageRank: products x[Aj[k]] * Ax[k] are formed during
+// the gather (rounded, this file is built with -ffp-contract=off) and staged
+// in LDS; rows of <= 64 nonzeros are then summed by one thread starting from
+// y[row] in CSR order, which is exactly the reference's arithmetic.  Longer
+// rows use a wavefront tree; rows longer than a tile are split into segments
+// whose partials are added to y[row] in segment order by a second kernel.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "gardenia_hip.h"
+#include "gd_common.h"
+#include "pull_engine.h"
+
+namespace gd {
+
+constexpr int kSpIPT = 8;
+constexpr int kSpTileNnz = kBlock * kSpIPT;
+
+struct SpGather {
+  const float* __restrict__ x;
+  const float* __restrict__ Ax;
+  __device__ __forceinline__ float value(int e, int j) const { return x[j] * Ax[e]; }
+};
+
+struct SpEpilogue {
+  float* __restrict__ y;
+  using Pre = float;  // y[row] before the update
+  __device__ __forceinline__ Pre pre(int r) const { return y[r]; }
+  __device__ __forceinline__ float init(const Pre& y0) const { return y0; }
+  __device__ __forceinline__ double finish(int r, float s, const Pre&) const {
+    y[r] = s;
+    return 0.0;
+  }
+};
+
+__global__ __launch_bounds__(kBlock) void spmv_pull_kernel(
+    const Tile* __restrict__ tiles, const int32_t* __restrict__ Ap,
+    const int32_t* __restrict__ Aj, const float* __restrict__ Ax, const float* __restrict__ x,
+    float* __restrict__ y, float* __restrict__ hub_part) {
+  __shared__ RowTileLds<kSpIPT> lds;
+  const Tile t = tiles[blockIdx.x];
+  const SpGather g{x, Ax};
+  if (t.y >= 0) {
+    const SpEpilogue e{y};
+    (void)row_tile<kSpIPT>(Ap, Aj, t.x, t.y, t.z, t.w - t.z, g, e, lds);
+  } else {
+    const float p = segment_sum<kSpIPT>(Aj, t.z, t.w - t.z, g, lds.val);
+    if (threadIdx.x == 0) hub_part[-t.y - 1] = p;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void spmv_hub_kernel(const int32_t* __restrict__ hub_row,
+                                                          const int32_t* __restrict__ hub_slot,
+                                                          const int32_t* __restrict__ hub_nseg,
+                                                          int n_hubs,
+                                                          const float* __restrict__ hub_part,
+                                                          float* __restrict__ y) {
+  const int h = blockIdx.x * kBlock + threadIdx.x;
+  if (h >= n_hubs) return;
+  const float* p = hub_part + hub_slot[h];
+  float s = y[hub_row[h]];
+  for (int k = 0; k < hub_nseg[h]; k++) s += p[k];
+  y[hub_row[h]] = s;
+}
+
+}  // namespace gd
+
+using namespace gd;
+
+struct gd_spmv_plan {
+  int rows = 0, cols = 0;
+  int64_t nnz = 0;
+  const int32_t* Ap = nullptr;
+  const int32_t* Aj = nullptr;
+  const float* Ax = nullptr;
+  hipStream_t stream = nullptr;
+  Schedule sched;
+  int n_tiles = 0, n_hubs = 0;
+  DeviceBuffer tiles, hub_row, hub_slot, hub_nseg, hub_part;
+};
+
+extern "C" {
+
+int gd_spmv_plan_create(int rows, int cols, int64_t nnz, const int32_t* d_Ap, const int32_t* d_Aj,
+                        const float* d_Ax, void* stream, gd_spmv_plan** out) {
+  GD_CHECK_ARG(out != nullptr, "plan output pointer is NULL");
+  *out = nullptr;
+  GD_CHECK_ARG(rows >= 0 && cols >= 0 && nnz >= 0 && nnz < (1ll << 31), "size out of range");
+  GD_CHECK_ARG(rows == 0 || d_Ap, "NULL row offsets");
+  GD_CHECK_ARG(nnz == 0 || (d_Aj && d_Ax), "NULL column indices / values");
+  if (gd_device_count() <= 0) {
+    set_error("no HIP device visible");
+    return GD_ERR_NO_DEVICE;
+  }
+  gd_spmv_plan* p = new gd_spmv_plan();
+  p->rows = rows;
+  p->cols = cols;
+  p->nnz = nnz;
+  p->Ap = d_Ap;
+  p->Aj = d_Aj;
+  p->Ax = d_Ax;
+  p->stream = (hipStream_t)stream;
+  int rc = GD_OK;
+  if (rows > 0) {
+    std::vector<int32_t> h_off(rows + 1);
+    hipError_t e = hipMemcpyAsync(h_off.data(), d_Ap, sizeof(int32_t) * (rows + 1),
+                                  hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) {
+      set_error(std::string("copying row offsets: ") + hipGetErrorString(e));
+      rc = GD_ERR_DEVICE;
+    }
+    if (!rc) rc = build_schedule(rows, nnz, h_off.data(), kSpTileNnz, kTileRows, &p->sched);
+    if (!rc) rc = check_columns(d_Aj, nnz, cols, p->stream);
+    if (!rc) {
+      p->n_tiles = (int)p->sched.tiles.size();
+      p->n_hubs = (int)p->sched.hub_row.size();
+      rc = p->tiles.alloc(sizeof(Tile) * p->n_tiles);
+      if (!rc) rc = p->hub_row.alloc(4 * std::max(1, p->n_hubs));
+      if (!rc) rc = p->hub_slot.alloc(4 * std::max(1, p->n_hubs));
+      if (!rc) rc = p->hub_nseg.alloc(4 * std::max(1, p->n_hubs));
+      if (!rc) rc = p->hub_part.alloc(sizeof(float) * std::max(1, p->sched.n_slots));
+    }
+    if (!rc) {
+      hipError_t e2 = hipMemcpyAsync(p->tiles.ptr, p->sched.tiles.data(),
+                                     sizeof(Tile) * p->n_tiles, hipMemcpyHostToDevice, p->stream);
+      if (e2 == hipSuccess && p->n_hubs) {
+        e2 = hipMemcpyAsync(p->hub_row.ptr, p->sched.hub_row.data(), 4 * p->n_hubs,
+                            hipMemcpyHostToDevice, p->stream);
+        if (e2 == hipSuccess)
+          e2 = hipMemcpyAsync(p->hub_slot.ptr, p->sched.hub_slot.data(), 4 * p->n_hubs,
+                              hipMemcpyHostToDevice, p->stream);
+        if (e2 == hipSuccess)
+          e2 = hipMemcpyAsync(p->hub_nseg.ptr, p->sched.hub_nseg.data(), 4 * p->n_hubs,
+                              hipMemcpyHostToDevice, p->stream);
+      }
+      if (e2 == hipSuccess) e2 = hipStreamSynchronize(p->stream);
+      if (e2 != hipSuccess) {
+        set_error(std::string("uploading schedule: ") + hipGetErrorString(e2));
+        rc = GD_ERR_DEVICE;
+      }
+    }
+  }
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return GD_OK;
+}
+
+int gd_spmv_plan_run(gd_spmv_plan* p, const float* d_x, float* d_y) {
+  GD_CHECK_ARG(p != nullptr, "NULL plan");
+  if (p->rows == 0) return GD_OK;
+  GD_CHECK_ARG(d_y != nullptr && (p->nnz == 0 || d_x != nullptr), "NULL x / y");
+  hipLaunchKernelGGL(spmv_pull_kernel, dim3(p->n_tiles), dim3(kBlock), 0, p->stream,
+                     p->tiles.as<Tile>(), p->Ap, p->Aj, p->Ax, d_x, d_y, p->hub_part.as<float>());
+  if (p->n_hubs)
+    hipLaunchKernelGGL(spmv_hub_kernel, dim3((p->n_hubs + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       p->stream, p->hub_row.as<int32_t>(), p->hub_slot.as<int32_t>(),
+                       p->hub_nseg.as<int32_t>(), p->n_hubs, p->hub_part.as<float>(), d_y);
+  GD_HIP_TRY(hipGetLastError());
+  return GD_OK;
+}
+
+int gd_spmv_plan_info(const gd_spmv_plan* p, int64_t* n_tiles, int64_t* n_hubs,
+                      int32_t* max_row_len) {
+  GD_CHECK_ARG(p != nullptr, "NULL plan");
+  if (n_tiles) *n_tiles = p->n_tiles;
+  if (n_hubs) *n_hubs = p->n_hubs;
+  if (max_row_len) *max_row_len = p->sched.max_row_len;
+  return GD_OK;
+}
+
+int gd_spmv_plan_destroy(gd_spmv_plan* p) {
+  delete p;
+  return GD_OK;
+}
+
+int gd_spmv_solve(int m, int nnz, const int32_t* Ap, const int32_t* Aj, const float* Ax,
+                  const float* x, float* y, float* kernel_ms) {
+  GD_CHECK_ARG(m >= 0 && nnz >= 0, "negative size");
+  GD_CHECK_ARG(m == 0 || (Ap && x && y), "NULL host array");
+  GD_CHECK_ARG(nnz == 0 || (Aj && Ax), "NULL column indices / values");
+  if (gd_device_count() <= 0) {
+    set_error("no HIP device visible");
+    return GD_ERR_NO_DEVICE;
+  }
+  DeviceBuffer dAp, dAj, dAx, dx, dy;
+  int rc = dAp.alloc(4 * ((size_t)m + 1));
+  if (!rc) rc = dAj.alloc(4 * (size_t)nnz);
+  if (!rc) rc = dAx.alloc(4 * (size_t)nnz);
+  if (!rc) rc = dx.alloc(4 * (size_t)m);
+  if (!rc) rc = dy.alloc(4 * (size_t)m);
+  if (rc) return rc;
+  if (m > 0) {
+    GD_HIP_TRY(hipMemcpy(dAp.ptr, Ap, 4 * ((size_t)m + 1), hipMemcpyHostToDevice));
+    GD_HIP_TRY(hipMemcpy(dx.ptr, x, 4 * (size_t)m, hipMemcpyHostToDevice));
+    GD_HIP_TRY(hipMemcpy(dy.ptr, y, 4 * (size_t)m, hipMemcpyHostToDevice));
+  }
+  if (nnz > 0) {
+    GD_HIP_TRY(hipMemcpy(dAj.ptr, Aj, 4 * (size_t)nnz, hipMemcpyHostToDevice));
+    GD_HIP_TRY(hipMemcpy(dAx.ptr, Ax, 4 * (size_t)nnz, hipMemcpyHostToDevice));
+  }
+  gd_spmv_plan* plan = nullptr;
+  rc = gd_spmv_plan_create(m, m, nnz, dAp.as<int32_t>(), dAj.as<int32_t>(), dAx.as<float>(),
+                           nullptr, &plan);
+  if (rc) return rc;
+  hipEvent_t t0, t1;
+  GD_HIP_TRY(hipEventCreate(&t0));
+  GD_HIP_TRY(hipEventCreate(&t1));
+  GD_HIP_TRY(hipEventRecord(t0, nullptr));
+  rc = gd_spmv_plan_run(plan, dx.as<float>(), dy.as<float>());
+  GD_HIP_TRY(hipEventRecord(t1, nullptr));
+  GD_HIP_TRY(hipEventSynchronize(t1));
+  float ms = 0.0f;
+  GD_HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+  if (kernel_ms) *kernel_ms = ms;
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  gd_spmv_plan_destroy(plan);
+  if (rc) return rc;
+  if (m > 0) GD_HIP_TRY(hipMemcpy(y, dy.ptr, 4 * (size_t)m, hipMemcpyDeviceToHost));
+  return GD_OK;
+}
+
+}  // extern "C"

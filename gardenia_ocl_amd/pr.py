@@ -1,0 +1,138 @@
+"""Pull PageRank: the reference's PRSolver interface over the HIP library.
+
+``PRSolver`` keeps the reference signature (src/pr/pr.h:23) with host numpy
+arrays, scores in/out, and prints the reference's stdout lines when asked
+(src/pr/omp_base.cc:15,42,46-47).  ``PageRankPlan`` is the device-resident
+form: torch tensors in HBM, a schedule built once, many runs.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from . import _native as N
+
+K_DAMP = 0.85        # include/common.h:18
+EPSILON = 1e-3       # src/pr/pr.h:4
+MAX_ITER = 30        # src/pr/pr.h:5
+
+
+def make_options(damping=K_DAMP, tol=EPSILON, max_iter=MAX_ITER, seed_vertex=-1, chunk=0,
+                 profile=False) -> N.PrOptions:
+    o = N.PrOptions()
+    N.lib().gd_pr_default_options(ctypes.byref(o))
+    o.damping = float(damping)
+    o.tol = float(tol)
+    o.max_iter = int(max_iter)
+    o.seed_vertex = int(seed_vertex)
+    o.chunk = int(chunk)
+    o.profile = int(bool(profile))
+    return o
+
+
+def _np_ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _as_i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def PRSolver(m: int, nnz: int, in_row_offsets, in_column_indices, out_row_offsets,
+             out_column_indices, degree, scores: np.ndarray, *, damping=K_DAMP, tol=EPSILON,
+             max_iter=MAX_ITER, seed_vertex=-1, verbose=False, variant="hip_pull"):
+    """PRSolver(m, nnz, in_off, in_col, out_off, out_col, degree, scores) on the GPU.
+
+    ``scores`` (float32, length m) is updated in place, as in the reference.
+    Returns (stats dict, per-iteration L1 errors).  Raises GardeniaError on any
+    failure (the reference exit()s instead, include/cutil_subset.h:4-12).
+    """
+    if not (isinstance(scores, np.ndarray) and scores.dtype == np.float32
+            and scores.flags.c_contiguous):
+        raise TypeError("scores must be a contiguous float32 numpy array (updated in place)")
+    if scores.shape[0] != m:
+        raise ValueError("scores must have length m")
+    in_off, in_col, deg = _as_i32(in_row_offsets), _as_i32(in_column_indices), _as_i32(degree)
+    out_off = None if out_row_offsets is None else _as_i32(out_row_offsets)
+    out_col = None if out_column_indices is None else _as_i32(out_column_indices)
+    if in_off.shape[0] != m + 1 or deg.shape[0] != m or in_col.shape[0] < nnz:
+        raise ValueError("CSR array lengths do not match m / nnz")
+    opts = make_options(damping, tol, max_iter, seed_vertex)
+    stats = N.PrStats()
+    errors = np.zeros(max(1, max_iter), np.float64)
+    if verbose:
+        print("Launching HIP PR solver (gfx950 pull, fused contrib/L1) ...")
+    N.call("gd_pr_solve", m, nnz, _np_ptr(in_off), _np_ptr(in_col), _np_ptr(out_off),
+           _np_ptr(out_col), _np_ptr(deg), _np_ptr(scores), ctypes.byref(opts),
+           ctypes.byref(stats), _np_ptr(errors))
+    errs = errors[: stats.iterations].copy()
+    if verbose:
+        for i, e in enumerate(errs):
+            print(f" {i + 1:2d}    {e:f}")
+        printed = stats.iterations + (0 if stats.converged or max_iter == 0 else 1)
+        print(f"\titerations = {printed}.")
+        print(f"\truntime [{variant}] = {stats.solve_ms:f} ms.")
+    return stats.as_dict(), errs
+
+
+class PageRankPlan:
+    """Device-resident PR plan over torch int32 CUDA tensors (in-CSR + out-degree).
+
+    The tensors are borrowed: keep them alive and unchanged while the plan lives.
+    Work is issued on ``stream`` (default: torch's current stream).
+    """
+
+    def __init__(self, in_row_offsets, in_column_indices, out_degree, stream=None):
+        import torch
+
+        for t in (in_row_offsets, in_column_indices, out_degree):
+            if not (t.is_cuda and t.dtype == torch.int32 and t.is_contiguous()):
+                raise TypeError("plan inputs must be contiguous int32 CUDA tensors")
+        self.m = int(out_degree.shape[0])
+        self.nnz = int(in_column_indices.shape[0])
+        if int(in_row_offsets.shape[0]) != self.m + 1:
+            raise ValueError("in_row_offsets must have m+1 entries")
+        self._keep = (in_row_offsets, in_column_indices, out_degree)
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        h = ctypes.c_void_p()
+        N.call("gd_pr_plan_create", self.m, self.nnz, in_row_offsets.data_ptr(),
+               in_column_indices.data_ptr(), out_degree.data_ptr(),
+               ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h))
+        self._h = h
+
+    def info(self) -> dict:
+        t, hb, sl = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        mx = ctypes.c_int32()
+        N.call("gd_pr_plan_info", self._h, ctypes.byref(t), ctypes.byref(hb), ctypes.byref(sl),
+               ctypes.byref(mx))
+        return {"tiles": t.value, "hubs": hb.value, "hub_segments": sl.value,
+                "max_row_len": mx.value}
+
+    def run(self, scores, *, damping=K_DAMP, tol=EPSILON, max_iter=MAX_ITER, seed_vertex=-1,
+            chunk=0, profile=False):
+        """Iterate on ``scores`` (float32 CUDA tensor, in/out).  Returns
+        (stats dict, per-iteration L1 errors as numpy)."""
+        import torch
+
+        if not (scores.is_cuda and scores.dtype == torch.float32 and scores.is_contiguous()
+                and scores.shape[0] == self.m):
+            raise TypeError("scores must be a contiguous float32 CUDA tensor of length m")
+        opts = make_options(damping, tol, max_iter, seed_vertex, chunk, profile)
+        stats = N.PrStats()
+        errors = np.zeros(max(1, max_iter), np.float64)
+        N.call("gd_pr_plan_run", self._h, scores.data_ptr(), ctypes.byref(opts),
+               ctypes.byref(stats), _np_ptr(errors))
+        return stats.as_dict(), errors[: stats.iterations].copy()
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            N.lib().gd_pr_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,115 @@
+/* gardenia_hip.h -- C ABI of libgardenia_hip.so (MI355X / gfx950).
+ *
+ * Drop-in replacement for the device side of the reference's two solver
+ * plug-ins:
+ *   void PRSolver(int m, int nnz, int *in_row_offsets, int *in_column_indices,
+ *                 int *out_row_offsets, int *out_column_indices, int *degree,
+ *                 ScoreT *scores);                               src/pr/pr.h:23
+ *   void SpmvSolver(int m, int nnz, int *Ap, int *Aj, ValueType *Ax,
+ *                   ValueType *x, ValueType *y);                 src/spmv/spmv.h:29
+ * Plain pointers and sizes only.  Every entry point returns a GD_* status
+ * (gardenia_graph.h); gd_last_error() gives the text of the last failure on
+ * the calling thread.  "d_" pointers are device (HBM) pointers, others host.
+ * `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ */
+#ifndef GARDENIA_HIP_H_
+#define GARDENIA_HIP_H_
+
+#include <stdint.h>
+
+#include "gardenia_graph.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *gd_last_error(void);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int gd_device_count(void);
+
+/* ------------------------------------------------------------ PageRank */
+
+/* Run parameters.  Defaults (gd_pr_default_options) are the reference's
+ * compile-time constants: kDamp 0.85f (include/common.h:18), EPSILON 1e-3 and
+ * MAX_ITER 30 (src/pr/pr.h:4-5). */
+typedef struct gd_pr_options {
+  float damping;
+  double tol;      /* stop after the first iteration whose L1 change < tol */
+  int max_iter;
+  int seed_vertex; /* -1: uniform teleport (1-d)/m as in the reference;
+                      v >= 0: personalised, teleport mass (1-d) to v only */
+  int chunk;       /* iterations launched between host convergence checks (0: auto) */
+  int profile;     /* nonzero: time every pull-kernel launch with HIP events */
+} gd_pr_options;
+
+typedef struct gd_pr_stats {
+  int iterations;       /* iterations executed (== reference's count when it converges) */
+  int converged;        /* 1 if an iteration's L1 change fell below tol */
+  double final_error;   /* L1 change of the last iteration */
+  float solve_ms;       /* HIP-event time of the whole iteration loop */
+  float pull_ms;        /* summed pull-kernel time (profile != 0), else 0 */
+  int pull_launches;    /* pull kernels timed */
+  int near_tie;         /* 1 if one of the last two L1 changes lies within
+                           1e-5*tol + 2.4e-7 of tol (the FP32 noise band) */
+} gd_pr_stats;
+
+void gd_pr_default_options(gd_pr_options *opts);
+
+typedef struct gd_pr_plan gd_pr_plan;
+
+/* Plan over a device-resident in-CSR (rows = destinations, columns = sources)
+ * and the out-degree array, all borrowed (must outlive the plan).  Validates
+ * the CSR (offsets, column range) and builds the degree-binned tile schedule
+ * plus scratch (2 contrib buffers, partials). */
+int gd_pr_plan_create(int m, int64_t nnz, const int32_t *d_in_row_offsets,
+                      const int32_t *d_in_column_indices, const int32_t *d_out_degree, void *stream,
+                      gd_pr_plan **plan);
+
+/* Pull PageRank on d_scores (in: initial scores, out: result), the loop of
+ * src/pr/omp_base.cc:23-44 with one fused pull/damping/contrib/L1 kernel per
+ * iteration.  errors (host, may be NULL) receives max_iter L1 values. */
+int gd_pr_plan_run(gd_pr_plan *plan, float *d_scores, const gd_pr_options *opts,
+                   gd_pr_stats *stats, double *errors);
+
+/* Schedule facts: tiles, hub rows, hub segments, longest row. */
+int gd_pr_plan_info(const gd_pr_plan *plan, int64_t *n_tiles, int64_t *n_hubs, int64_t *n_slots,
+                    int32_t *max_row_len);
+
+int gd_pr_plan_destroy(gd_pr_plan *plan);
+
+/* One-shot form of PRSolver with host arrays (src/pr/pr.h:23): copies the
+ * in-CSR, degree and scores to HBM, runs, copies scores back, frees.  The
+ * out-CSR arguments are accepted and unused, as in every pull variant of the
+ * reference. */
+int gd_pr_solve(int m, int nnz, const int32_t *in_row_offsets, const int32_t *in_column_indices,
+                const int32_t *out_row_offsets, const int32_t *out_column_indices,
+                const int32_t *degree, float *scores, const gd_pr_options *opts,
+                gd_pr_stats *stats, double *errors);
+
+/* ------------------------------------------------------------ SpMV */
+
+typedef struct gd_spmv_plan gd_spmv_plan;
+
+/* Plan over a device-resident CSR matrix (num_rows x num_cols), borrowed. */
+int gd_spmv_plan_create(int num_rows, int num_cols, int64_t nnz, const int32_t *d_Ap,
+                        const int32_t *d_Aj, const float *d_Ax, void *stream, gd_spmv_plan **plan);
+
+/* y <- y + A x (src/spmv/omp_base.cc:17-28).  Rows up to 64 nonzeros are
+ * summed sequentially in CSR order (bit-identical to the reference); longer
+ * rows use a wavefront tree. */
+int gd_spmv_plan_run(gd_spmv_plan *plan, const float *d_x, float *d_y);
+
+int gd_spmv_plan_info(const gd_spmv_plan *plan, int64_t *n_tiles, int64_t *n_hubs,
+                      int32_t *max_row_len);
+
+int gd_spmv_plan_destroy(gd_spmv_plan *plan);
+
+/* One-shot form of SpmvSolver with host arrays (src/spmv/spmv.h:29). */
+int gd_spmv_solve(int m, int nnz, const int32_t *Ap, const int32_t *Aj, const float *Ax,
+                  const float *x, float *y, float *kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GARDENIA_HIP_H_ */

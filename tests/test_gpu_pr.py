@@ -1,0 +1,228 @@
+"""GPU parity of the HIP pull PageRank against the reference (golden vectors)
+and the oracle.  All calls go through the C ABI (ctypes)."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import dataset
+from gardenia_ocl_amd import PRSolver, PageRankPlan, graph
+
+pytestmark = pytest.mark.gpu
+
+DATASETS = ["test_pr.mtx", "4.mtx", "4w.mtx", "chesapeake.mtx"]
+
+
+def _key(ds):
+    return ds.replace(".mtx", "").replace(".", "_")
+
+
+def _gpu_pr(device, in_csr, outdeg, tol, max_iter, seed_vertex=-1, init=None, chunk=0):
+    import torch
+
+    m = in_csr.m
+    off = torch.from_numpy(in_csr.row_offsets).to(device)
+    col = torch.from_numpy(in_csr.column_indices).to(device)
+    deg = torch.from_numpy(np.ascontiguousarray(outdeg, np.int32)).to(device)
+    if init is None:
+        init = np.full(m, np.float32(1.0) / np.float32(m), np.float32)
+    scores = torch.from_numpy(np.array(init, np.float32)).to(device)
+    plan = PageRankPlan(off, col, deg)
+    stats, errs = plan.run(scores, tol=tol, max_iter=max_iter, seed_vertex=seed_vertex,
+                           chunk=chunk)
+    out = scores.cpu().numpy()
+    info = plan.info()
+    plan.close()
+    return out, stats, errs, info
+
+
+@pytest.mark.parametrize("ds", DATASETS)
+@pytest.mark.parametrize("directed", [True, False])
+def test_pr_datasets_bit_exact_vs_reference(golden, device, ds, directed):
+    arrays, meta = golden
+    out_csr, in_csr = graph.read_pr_graph(dataset(ds), is_directed=directed)
+    mode = "directed" if directed else "undirected"
+    s, stats, errs, _ = _gpu_pr(device, in_csr, out_csr.degree, meta["epsilon"], meta["max_iter"])
+    ref = arrays[f"{_key(ds)}/pr_{mode}/scores"]
+    if in_csr.degree.max() <= 64:
+        # every row is summed in CSR order: bit-identical to the reference
+        np.testing.assert_array_equal(s, ref)
+    else:
+        # rows of > 64 in-edges use a wavefront tree (here: the symmetrized
+        # 4.mtx keeps duplicate edges, so some rows exceed 64)
+        np.testing.assert_allclose(s, ref, rtol=1e-6, atol=0)
+    assert stats["iterations"] == meta["datasets"][ds][f"pr_{mode}"]["iterations_printed"]
+    # printed with %lf (6 decimals); tree-summed rows add up to ~1e-6 of L1 drift
+    slack = 5.1e-7 if in_csr.degree.max() <= 64 else 1.6e-6
+    np.testing.assert_allclose(errs, arrays[f"{_key(ds)}/pr_{mode}/errors_printed"], atol=slack)
+
+
+def test_pr_host_solver_interface(golden):
+    """PRSolver(m, nnz, in_off, in_col, out_off, out_col, degree, scores) -- host arrays."""
+    arrays, meta = golden
+    out_csr, in_csr = graph.read_pr_graph(dataset("test_pr.mtx"))
+    m = in_csr.m
+    scores = np.full(m, np.float32(1.0) / np.float32(m), np.float32)
+    stats, errs = PRSolver(m, in_csr.nnz, in_csr.row_offsets, in_csr.column_indices,
+                           out_csr.row_offsets, out_csr.column_indices, out_csr.degree, scores)
+    np.testing.assert_array_equal(scores, arrays["test_pr/pr_directed/scores"])
+    assert stats["iterations"] == 11
+
+
+@pytest.mark.parametrize("scale", [12, 16])
+def test_pr_rmat_vs_reference(golden, device, scale):
+    arrays, meta = golden
+    md = meta["rmat"][str(scale)]
+    out_csr, in_csr = graph.rmat(scale, 16, seed=md["seed"])
+    s, stats, errs, info = _gpu_pr(device, in_csr, out_csr.degree, meta["epsilon"],
+                                   meta["max_iter"])
+    ref, it, _ = oracle.pr(in_csr.m, in_csr.row_offsets, in_csr.column_indices, out_csr.degree,
+                           tol=meta["epsilon"], max_iter=meta["max_iter"])
+    assert stats["iterations"] == md["iterations_printed"] == it
+    np.testing.assert_allclose(errs, md["errors_printed"], atol=5.1e-7)
+    assert np.max(np.abs(s - ref) / ref) <= 1e-5
+    if scale == 12:
+        np.testing.assert_allclose(s, arrays["rmat12/scores"], rtol=1e-5, atol=0)
+    assert info["hubs"] == int(np.sum(in_csr.degree > 2048))
+
+
+def test_pr_first_iteration_short_rows_bit_exact(device):
+    """After one iteration every row of <= 64 in-edges is bit-identical to the
+    reference arithmetic (CSR-order sum, no FMA, IEEE division)."""
+    out_csr, in_csr = graph.rmat(16, 16, seed=1)
+    s, stats, _, _ = _gpu_pr(device, in_csr, out_csr.degree, 0.0, 1)
+    ref, it, _ = oracle.pr(in_csr.m, in_csr.row_offsets, in_csr.column_indices, out_csr.degree,
+                           tol=0.0, max_iter=1)
+    short = in_csr.degree <= 64
+    np.testing.assert_array_equal(s[short], ref[short])
+    assert np.max(np.abs(s - ref) / ref) <= 1e-5
+
+
+@pytest.mark.parametrize("scale,tol", [(16, 1e-4), (18, 1e-4), (18, 1e-6), (20, 1e-4)])
+def test_pr_rmat_vs_oracle(device, scale, tol):
+    out_csr, in_csr = graph.rmat(scale, 16, seed=1)
+    s, stats, errs, info = _gpu_pr(device, in_csr, out_csr.degree, tol, 100)
+    ref, it, ref_errs = oracle.pr(in_csr.m, in_csr.row_offsets, in_csr.column_indices,
+                                  out_csr.degree, tol=tol, max_iter=100)
+    if not stats["near_tie"]:
+        assert stats["iterations"] == it
+    n = min(it, len(errs))
+    # per-iteration L1: equal up to the FP32 noise band (hub rows round differently)
+    assert np.all(np.abs(errs[:n] - ref_errs[:n]) <= 2.4e-7 + 1e-4 * ref_errs[:n])
+    rel = np.abs(s - ref) / ref
+    assert rel.max() <= 1e-5, rel.max()
+    # L1 error against the FP64 fixed point of the same operator
+    x64, _ = oracle.pr_fp64(in_csr.m, in_csr.row_offsets, in_csr.column_indices, out_csr.degree)
+    l1_gpu = np.abs(s.astype(np.float64) - x64).sum()
+    l1_cpu = np.abs(ref.astype(np.float64) - x64).sum()
+    assert l1_gpu <= 1.05 * l1_cpu + 1e-7
+    assert info["hubs"] > 0
+
+
+def test_pr_deterministic_and_chunk_invariant(device):
+    out_csr, in_csr = graph.rmat(16, 16, seed=5)
+    a, sa, ea, _ = _gpu_pr(device, in_csr, out_csr.degree, 1e-4, 100, chunk=1)
+    b, sb, eb, _ = _gpu_pr(device, in_csr, out_csr.degree, 1e-4, 100, chunk=7)
+    c, sc, ec, _ = _gpu_pr(device, in_csr, out_csr.degree, 1e-4, 100)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(a, c)
+    np.testing.assert_array_equal(ea, eb)
+    assert sa["iterations"] == sb["iterations"] == sc["iterations"]
+
+
+def test_pr_fixed_iterations_not_converged(device):
+    out_csr, in_csr = graph.rmat(14, 16, seed=2)
+    s, stats, errs, _ = _gpu_pr(device, in_csr, out_csr.degree, 0.0, 20)
+    ref, it, _ = oracle.pr(in_csr.m, in_csr.row_offsets, in_csr.column_indices, out_csr.degree,
+                           tol=0.0, max_iter=20)
+    assert stats["iterations"] == it == 20 and not stats["converged"]
+    assert np.max(np.abs(s - ref) / ref) <= 1e-5
+
+
+def test_pr_max_iter_zero_leaves_scores(device):
+    out_csr, in_csr = graph.rmat(10, 8, seed=3)
+    init = np.random.default_rng(0).random(in_csr.m).astype(np.float32)
+    s, stats, errs, _ = _gpu_pr(device, in_csr, out_csr.degree, 1e-3, 0, init=init)
+    assert stats["iterations"] == 0
+    np.testing.assert_array_equal(s, init)
+
+
+def _star(n_leaves, hub=0):
+    """Every leaf points at the hub and the hub points back at every leaf."""
+    m = n_leaves + 1
+    leaves = np.arange(1, m, dtype=np.int32)
+    src = np.concatenate([leaves, np.full(n_leaves, hub, np.int32)])
+    dst = np.concatenate([np.full(n_leaves, hub, np.int32), leaves])
+    out_csr = graph.csr_from_edges(m, src, dst, sorted=True)
+    in_csr = graph.csr_from_edges(m, src, dst, transpose=True, sorted=True)
+    return out_csr, in_csr
+
+
+@pytest.mark.parametrize("n_leaves", [1, 63, 64, 65, 2047, 2048, 2049, 100_000])
+def test_pr_star_hub_boundaries(device, n_leaves):
+    """Hub of n in-edges, all addends equal: the reference's sequential FP32 sum
+    drifts (its L1 change plateaus at ~3e-6 for n=65, ~7e-5 for n=2048 and the
+    hub score is 3.5e-5 off the FP64 fixed point), so the bar is the FP64
+    solution: the GPU must be at least as close to it as the reference."""
+    out_csr, in_csr = _star(n_leaves)
+    s, stats, errs, info = _gpu_pr(device, in_csr, out_csr.degree, 1e-6, 100)
+    ref, it, ref_errs = oracle.pr(in_csr.m, in_csr.row_offsets, in_csr.column_indices,
+                                  out_csr.degree, tol=1e-6, max_iter=100)
+    x64, _ = oracle.pr_fp64(in_csr.m, in_csr.row_offsets, in_csr.column_indices, out_csr.degree)
+    l1_gpu = np.abs(s.astype(np.float64) - x64).sum()
+    l1_ref = np.abs(ref.astype(np.float64) - x64).sum()
+    assert l1_gpu <= l1_ref + 2e-7
+    if ref_errs[-1] < 1e-6 and not stats["near_tie"]:
+        assert stats["iterations"] == it
+    if n_leaves <= 64:  # every row sequential: same arithmetic as the reference
+        np.testing.assert_array_equal(s, ref)
+    assert info["hubs"] == (1 if n_leaves > 2048 else 0)
+
+
+def test_pr_empty_and_edgeless(device):
+    import torch
+
+    # m = 0: nothing to do; one (empty) iteration as in the reference loop
+    z = torch.zeros(1, dtype=torch.int32, device=device)
+    e = torch.zeros(0, dtype=torch.int32, device=device)
+    plan = PageRankPlan(z, e, e)
+    stats, errs = plan.run(torch.zeros(0, dtype=torch.float32, device=device))
+    assert stats["iterations"] == 1 and stats["converged"]
+    # no edges: every score becomes (1-d)/m after one iteration
+    m = 1000
+    in_csr = graph.csr_from_edges(m, np.zeros(0, np.int32), np.zeros(0, np.int32))
+    s, stats, errs, _ = _gpu_pr(device, in_csr, np.zeros(m, np.int32), 1e-3, 30)
+    ref, it, _ = oracle.pr(m, in_csr.row_offsets, in_csr.column_indices, np.zeros(m, np.int32))
+    np.testing.assert_array_equal(s, ref)
+    assert stats["iterations"] == it
+
+
+def test_pr_personalised_seed_matches_oracle(device):
+    out_csr, in_csr = graph.rmat(14, 16, seed=4)
+    m = in_csr.m
+    seed = int(np.argmax(out_csr.degree))
+    init = np.zeros(m, np.float32)
+    init[seed] = 1.0
+    s, stats, errs, _ = _gpu_pr(device, in_csr, out_csr.degree, 1e-7, 200, seed_vertex=seed,
+                                init=init)
+    ref, it, _ = oracle.ppr(m, in_csr.row_offsets, in_csr.column_indices, out_csr.degree, seed,
+                            tol=1e-7, max_iter=200)
+    if not stats["near_tie"]:
+        assert stats["iterations"] == it
+    big = ref > 1e-6
+    assert np.max(np.abs(s[big] - ref[big]) / ref[big]) <= 1e-5
+    np.testing.assert_allclose(s, ref, rtol=1e-5, atol=1e-10)
+
+
+def test_pr_invalid_csr_rejected(device):
+    import torch
+
+    from gardenia_ocl_amd import GardeniaError
+
+    off = torch.tensor([0, 2, 3], dtype=torch.int32, device=device)
+    col = torch.tensor([1, 5, 0], dtype=torch.int32, device=device)  # 5 >= m
+    deg = torch.tensor([1, 1], dtype=torch.int32, device=device)
+    with pytest.raises(GardeniaError, match="column index out of range"):
+        PageRankPlan(off, col, deg)
+    off_bad = torch.tensor([0, 3, 2], dtype=torch.int32, device=device)
+    with pytest.raises(GardeniaError):
+        PageRankPlan(off_bad, torch.tensor([0, 1], dtype=torch.int32, device=device), deg)

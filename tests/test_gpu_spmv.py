@@ -1,0 +1,132 @@
+"""GPU parity of the HIP CSR SpMV against the reference (golden vectors) and the
+oracle.  All calls go through the C ABI (ctypes)."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import dataset
+from gardenia_ocl_amd import SpmvPlan, SpmvSolver, graph
+
+pytestmark = pytest.mark.gpu
+
+DATASETS = ["test_pr.mtx", "4.mtx", "4w.mtx", "chesapeake.mtx"]
+REL_TOL = 1e-6  # maximum_relative_error bar where the summation order is the reference's
+VERIFIER_TOL = 5 * np.sqrt(np.finfo(np.float32).eps)  # src/spmv/verifier.cc:58
+
+
+def _fp64_spmv(csr, Ax, x, y0):
+    prod = np.asarray(Ax, np.float64) * np.asarray(x, np.float64)[csr.column_indices]
+    rows = np.repeat(np.arange(csr.m), np.diff(csr.row_offsets))
+    out = np.asarray(y0, np.float64).copy()
+    np.add.at(out, rows, prod)
+    return out
+
+
+def _check_long_rows(csr, y, ref, Ax, x, y0):
+    """Rows of <= 64 nonzeros: bit-identical.  Longer rows (wavefront tree):
+    within the reference verifier's bar, and no further from the FP64 result
+    than the reference's own sequential sum (2x slack)."""
+    short = csr.degree <= 64
+    np.testing.assert_array_equal(y[short], ref[short])
+    assert oracle.max_rel_err(y, ref) <= VERIFIER_TOL
+    y64 = _fp64_spmv(csr, Ax, x, y0)
+    assert np.abs(y - y64).max() <= 2 * np.abs(ref - y64).max() + 1e-6
+
+
+def _key(ds):
+    return ds.replace(".mtx", "").replace(".", "_")
+
+
+def _gpu_spmv(device, csr, Ax, x, y0, reps=1):
+    import torch
+
+    Ap = torch.from_numpy(csr.row_offsets).to(device)
+    Aj = torch.from_numpy(csr.column_indices).to(device)
+    tAx = torch.from_numpy(np.ascontiguousarray(Ax, np.float32)).to(device)
+    plan = SpmvPlan(Ap, Aj, tAx, num_cols=len(x))
+    tx = torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(device)
+    ty = torch.from_numpy(np.array(y0, np.float32)).to(device)
+    for _ in range(reps):
+        plan.run(tx, ty)
+    torch.cuda.synchronize()
+    info = plan.info()
+    plan.close()
+    return ty.cpu().numpy(), info
+
+
+@pytest.mark.parametrize("ds", DATASETS)
+def test_spmv_datasets_bit_exact_vs_reference(golden, device, ds):
+    arrays, _ = golden
+    k = _key(ds)
+    csr = graph.read_spmv_matrix(dataset(ds))
+    y, _ = _gpu_spmv(device, csr, arrays[f"{k}/spmv/Ax"], arrays[f"{k}/spmv/x"],
+                     arrays[f"{k}/spmv/y0"])
+    np.testing.assert_array_equal(y, arrays[f"{k}/spmv/y"])
+
+
+def test_spmv_host_solver_interface(golden):
+    arrays, _ = golden
+    csr = graph.read_spmv_matrix(dataset("chesapeake.mtx"))
+    y = np.array(arrays["chesapeake/spmv/y0"], np.float32)
+    ms = SpmvSolver(csr.m, csr.nnz, csr.row_offsets, csr.column_indices,
+                    arrays["chesapeake/spmv/Ax"], arrays["chesapeake/spmv/x"], y)
+    np.testing.assert_array_equal(y, arrays["chesapeake/spmv/y"])
+    assert ms >= 0.0
+
+
+def test_spmv_er_bit_exact_vs_reference(golden, device):
+    arrays, meta = golden
+    md = meta["spmv"]["er12"]
+    er = graph.er_matrix(12, 32, seed=md["seed_matrix"])
+    sd = md["seed_values"]
+    Ax, x, y0 = (graph.uniform_pm1(n, sd + k) for k, n in ((0, er.nnz), (1, er.m), (2, er.m)))
+    y, info = _gpu_spmv(device, er, Ax, x, y0)
+    np.testing.assert_array_equal(y, arrays["er12/spmv/y"])
+    assert info["hubs"] == 0
+
+
+def test_spmv_rmat_vs_reference(golden, device):
+    arrays, meta = golden
+    _, ic = graph.rmat(12, 16, seed=1)
+    sd = meta["spmv"]["rmat12_in"]["seed"]
+    Ax, x, y0 = (graph.uniform_pm1(n, sd + k) for k, n in ((0, ic.nnz), (1, ic.m), (2, ic.m)))
+    y, _ = _gpu_spmv(device, ic, Ax, x, y0)
+    _check_long_rows(ic, y, arrays["rmat12_in/spmv/y"], Ax, x, y0)
+
+
+@pytest.mark.parametrize("scale", [16, 18])
+def test_spmv_power_law_long_rows_vs_oracle(device, scale):
+    _, ic = graph.rmat(scale, 16, seed=9)
+    Ax, x, y0 = (graph.uniform_pm1(n, 40 + k) for k, n in ((0, ic.nnz), (1, ic.m), (2, ic.m)))
+    y, info = _gpu_spmv(device, ic, Ax, x, y0)
+    ref = oracle.spmv(ic.m, ic.row_offsets, ic.column_indices, Ax, x, y0)
+    assert info["hubs"] > 0
+    _check_long_rows(ic, y, ref, Ax, x, y0)
+
+
+def test_spmv_er_2_20_bit_exact_vs_oracle(device):
+    er = graph.er_matrix(20, 32, seed=11)
+    Ax = er.weight
+    x = graph.uniform_pm1(er.m, 12)
+    y0 = np.zeros(er.m, np.float32)
+    y, _ = _gpu_spmv(device, er, Ax, x, y0, reps=3)
+    ref = y0
+    for _ in range(3):
+        ref = oracle.spmv(er.m, er.row_offsets, er.column_indices, Ax, x, ref)
+    np.testing.assert_array_equal(y, ref)
+
+
+def test_spmv_empty_rows_and_matrix(device):
+    m = 5000
+    rng = np.random.default_rng(1)
+    src = rng.integers(0, m, 3000).astype(np.int32)
+    dst = rng.integers(0, m, 3000).astype(np.int32)
+    csr = graph.csr_from_edges(m, src, dst, sorted=True, remove_selfloops=False)
+    Ax = graph.uniform_pm1(csr.nnz, 1)
+    x = graph.uniform_pm1(m, 2)
+    y0 = graph.uniform_pm1(m, 3)
+    y, _ = _gpu_spmv(device, csr, Ax, x, y0)
+    np.testing.assert_array_equal(y, oracle.spmv(m, csr.row_offsets, csr.column_indices, Ax, x, y0))
+    empty = graph.csr_from_edges(m, np.zeros(0, np.int32), np.zeros(0, np.int32))
+    y, _ = _gpu_spmv(device, empty, np.zeros(0, np.float32), x, y0)
+    np.testing.assert_array_equal(y, y0)
