@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--relabel", choices=["auto", "on", "off"], default="auto")
+    ap.add_argument("--ipt", type=int, default=0, help="edges per thread per tile (8/16, 0=auto)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -99,7 +101,8 @@ def main():
     deg = torch.from_numpy(out_deg).to(dev)
     scores = torch.empty(V, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
-    plan = PageRankPlan(off, col, deg, stream=stream)
+    relabel = {"auto": None, "on": True, "off": False}[args.relabel]
+    plan = PageRankPlan(off, col, deg, stream=stream, relabel=relabel, items_per_thread=args.ipt)
     info = plan.info()
     log(f"plan: {info}")
     init = float(np.float32(1.0) / np.float32(V))
@@ -145,7 +148,9 @@ def main():
         "config": {"workload": f"RMAT-{args.scale} pull PageRank, {args.iters} fixed iterations",
                    "scale": args.scale, "edgefactor": args.edgefactor, "seed": args.seed,
                    "V": V, "E": E, "iterations_per_step": args.iters,
-                   "abcd": [0.57, 0.19, 0.19, 0.05], "parallelism": "single-gpu"},
+                   "abcd": [0.57, 0.19, 0.19, 0.05], "parallelism": "single-gpu",
+                   "relabel": bool(info["relabel"]), "items_per_thread": info["items_per_thread"],
+                   "plan_ms": round(info["plan_ms"], 1)},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

@@ -10,8 +10,9 @@ from . import graph
 from ._native import GardeniaError, LIB_PATH
 from .pr import PRSolver, PageRankPlan, EPSILON, K_DAMP, MAX_ITER
 from .spmv import SpmvSolver, SpmvPlan
+from .ppr import PersonalizedPageRankPlan
 
 __all__ = [
-    "graph", "GardeniaError", "LIB_PATH", "PRSolver", "PageRankPlan", "SpmvSolver", "SpmvPlan",
+    "graph", "GardeniaError", "PersonalizedPageRankPlan", "LIB_PATH", "PRSolver", "PageRankPlan", "SpmvSolver", "SpmvPlan",
     "EPSILON", "K_DAMP", "MAX_ITER",
 ]

@@ -71,6 +71,44 @@ class PrStats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class PrPlanConfig(ctypes.Structure):
+    _fields_ = [("relabel", ctypes.c_int), ("items_per_thread", ctypes.c_int)]
+
+
+class PrPlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_tiles", ctypes.c_int64),
+        ("n_hubs", ctypes.c_int64),
+        ("n_hub_segments", ctypes.c_int64),
+        ("max_row_len", ctypes.c_int32),
+        ("relabel", ctypes.c_int32),
+        ("items_per_thread", ctypes.c_int32),
+        ("plan_ms", ctypes.c_float),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class PprPlanConfig(ctypes.Structure):
+    _fields_ = [("relabel", ctypes.c_int), ("k_chunk", ctypes.c_int)]
+
+
+class PprPlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_tiles", ctypes.c_int64),
+        ("n_hubs", ctypes.c_int64),
+        ("k", ctypes.c_int32),
+        ("k_chunk", ctypes.c_int32),
+        ("relabel", ctypes.c_int32),
+        ("grid", ctypes.c_int32),
+        ("plan_ms", ctypes.c_float),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _P = ctypes.c_void_p
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _F32P = ctypes.POINTER(ctypes.c_float)
@@ -96,12 +134,22 @@ SIGNATURES = {
     "gd_last_error": (ctypes.c_char_p, []),
     "gd_device_count": (_i, []),
     "gd_pr_default_options": (None, [ctypes.POINTER(PrOptions)]),
+    "gd_pr_default_plan_config": (None, [ctypes.POINTER(PrPlanConfig)]),
+    "gd_pr_plan_create_ex": (_i, [_i, _i64, _P, _P, _P, _P, ctypes.POINTER(PrPlanConfig),
+                                  ctypes.POINTER(_P)]),
     "gd_pr_plan_create": (_i, [_i, _i64, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "gd_pr_plan_run": (_i, [_P, _P, ctypes.POINTER(PrOptions), ctypes.POINTER(PrStats), _P]),
-    "gd_pr_plan_info": (_i, [_P, _I64P, _I64P, _I64P, _I32P]),
+    "gd_pr_plan_info": (_i, [_P, ctypes.POINTER(PrPlanInfo)]),
     "gd_pr_plan_destroy": (_i, [_P]),
     "gd_pr_solve": (_i, [_i, _i, _P, _P, _P, _P, _P, _P, ctypes.POINTER(PrOptions),
                          ctypes.POINTER(PrStats), _P]),
+    "gd_ppr_default_plan_config": (None, [ctypes.POINTER(PprPlanConfig)]),
+    "gd_ppr_plan_create": (_i, [_i, _i64, _P, _P, _P, _i, _P, ctypes.POINTER(PprPlanConfig),
+                                ctypes.POINTER(_P)]),
+    "gd_ppr_plan_run": (_i, [_P, _P, _P, ctypes.POINTER(PrOptions), ctypes.POINTER(PrStats), _P,
+                             _P]),
+    "gd_ppr_plan_info": (_i, [_P, ctypes.POINTER(PprPlanInfo)]),
+    "gd_ppr_plan_destroy": (_i, [_P]),
     "gd_spmv_plan_create": (_i, [_i, _i, _i64, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "gd_spmv_plan_run": (_i, [_P, _P, _P]),
     "gd_spmv_plan_info": (_i, [_P, _I64P, _I64P, _I32P]),

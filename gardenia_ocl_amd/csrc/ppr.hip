@@ -1,0 +1,625 @@
+// ppr.hip -- batched personalised PageRank (k seeds at once), gfx950.
+//
+// The same pull iteration as pr.hip, k columns wide: column j teleports its
+// (1-d) mass to seed vertex seeds[j] instead of (1-d)/m to every vertex.
+// Scores are stored vertex-major with the k seeds of a vertex contiguous
+// (X[v*k + j]), so one gathered index u feeds k additions: a wavefront owns a
+// chunk of kc = 64*VEC columns, lane l holds columns l*VEC..l*VEC+VEC-1, and
+// the gathered row segment C[u][chunk] is one coalesced 256*VEC-byte load.
+//
+// Each column is summed sequentially in CSR order (the scalar personalised
+// loop's order) except rows longer than kSegEdges, which are split into
+// segments whose partials are added in segment order.  The per-iteration
+// update is base + d*sum with two roundings (built with -ffp-contract=off).
+//
+// A persistent grid of workgroups walks the tile list; each workgroup keeps
+// per-column L1 partials for the whole sweep, and two small kernels reduce
+// them (deterministically) to per-column L1 values, their maximum and the
+// convergence flag.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "gardenia_hip.h"
+#include "gd_common.h"
+#include "relabel.h"
+
+namespace gd {
+
+namespace {
+
+constexpr int kPBlock = 256;
+constexpr int kPWaves = kPBlock / 64;
+constexpr int kSegEdges = 4096;   // rows longer than this are split into segments
+constexpr int kTileEdges = 1024;  // edges per wavefront tile
+constexpr int kTileRowsP = 256;   // rows per wavefront tile
+constexpr int kUnroll = 8;        // gathers in flight per lane
+constexpr int kSlices = 32;       // first-level L1 reduction slices
+
+struct PprState {
+  int iter;
+  int done;
+  int max_iter;
+  int pad;
+  double tol;
+};
+
+template <int VEC>
+struct Vec {
+  float v[VEC];
+};
+
+template <int VEC>
+__device__ __forceinline__ Vec<VEC> load_vec(const float* __restrict__ p) {
+  Vec<VEC> r;
+  if constexpr (VEC == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    r.v[0] = t.x; r.v[1] = t.y; r.v[2] = t.z; r.v[3] = t.w;
+  } else if constexpr (VEC == 2) {
+    const float2 t = *reinterpret_cast<const float2*>(p);
+    r.v[0] = t.x; r.v[1] = t.y;
+  } else {
+    r.v[0] = *p;
+  }
+  return r;
+}
+
+template <int VEC>
+__device__ __forceinline__ void store_vec(float* __restrict__ p, const float (&v)[VEC]) {
+  if constexpr (VEC == 4) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+  } else if constexpr (VEC == 2) {
+    *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+  } else {
+    *p = v[0];
+  }
+}
+
+// acc[j] += C[u][col0 + j] over the edges [b, e) in order, kUnroll gathers in
+// flight.  b, e, k, col0 are wavefront-uniform.
+template <int VEC>
+__device__ __forceinline__ void sum_edges(const int32_t* __restrict__ col,
+                                          const float* __restrict__ C, int64_t k, int col0,
+                                          int b, int e, float (&acc)[VEC]) {
+  int i = b;
+  for (; i + kUnroll <= e; i += kUnroll) {
+    Vec<VEC> g[kUnroll];
+#pragma unroll
+    for (int q = 0; q < kUnroll; q++) {
+      const int u = __builtin_amdgcn_readfirstlane(col[i + q]);
+      g[q] = load_vec<VEC>(C + (int64_t)u * k + col0);
+    }
+#pragma unroll
+    for (int q = 0; q < kUnroll; q++)
+#pragma unroll
+      for (int j = 0; j < VEC; j++) acc[j] += g[q].v[j];
+  }
+  for (; i < e; i++) {
+    const int u = __builtin_amdgcn_readfirstlane(col[i]);
+    const Vec<VEC> g = load_vec<VEC>(C + (int64_t)u * k + col0);
+#pragma unroll
+    for (int j = 0; j < VEC; j++) acc[j] += g.v[j];
+  }
+}
+
+template <int VEC>
+__device__ __forceinline__ void row_update(int v, const float (&acc)[VEC], const int (&seed)[VEC],
+                                           float* __restrict__ X, float* __restrict__ Cn,
+                                           const int32_t* __restrict__ outdeg, int64_t k,
+                                           int col0, float teleport, float damp,
+                                           double (&err)[VEC]) {
+  const int64_t o = (int64_t)v * k + col0;
+  const Vec<VEC> old = load_vec<VEC>(X + o);
+  const float deg = (float)outdeg[v];
+  float nx[VEC], nc[VEC];
+#pragma unroll
+  for (int j = 0; j < VEC; j++) {
+    const float b = (seed[j] == v) ? teleport : 0.0f;
+    nx[j] = b + damp * acc[j];
+    nc[j] = nx[j] / deg;
+    err[j] += (double)fabsf(nx[j] - old.v[j]);
+  }
+  store_vec<VEC>(X + o, nx);
+  store_vec<VEC>(Cn + o, nc);
+}
+
+template <int VEC>
+__global__ __launch_bounds__(kPBlock) void ppr_pull_kernel(
+    const Tile* __restrict__ tiles, int n_tiles, const int32_t* __restrict__ off,
+    const int32_t* __restrict__ col, const int32_t* __restrict__ outdeg,
+    const int32_t* __restrict__ seeds, int64_t k, float* __restrict__ X,
+    float* __restrict__ C0, float* __restrict__ C1, float* __restrict__ seg_part,
+    double* __restrict__ err_part, const PprState* __restrict__ st, float teleport,
+    float damp) {
+  if (st->done) return;
+  const int odd = st->iter & 1;
+  const float* __restrict__ C = odd ? C1 : C0;
+  float* __restrict__ Cn = odd ? C0 : C1;
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int col0 = blockIdx.y * 64 * VEC + lane * VEC;
+  int seed[VEC];
+#pragma unroll
+  for (int j = 0; j < VEC; j++) seed[j] = seeds[col0 + j];
+  double err[VEC];
+#pragma unroll
+  for (int j = 0; j < VEC; j++) err[j] = 0.0;
+  const int nw = gridDim.x * kPWaves;
+  for (int t = blockIdx.x * kPWaves + wid; t < n_tiles; t += nw) {
+    const Tile tl = tiles[t];
+    if (tl.y >= 0) {
+      for (int r = tl.x; r < tl.y; r++) {
+        float acc[VEC];
+#pragma unroll
+        for (int j = 0; j < VEC; j++) acc[j] = 0.0f;
+        sum_edges<VEC>(col, C, k, col0, off[r], off[r + 1], acc);
+        row_update<VEC>(r, acc, seed, X, Cn, outdeg, k, col0, teleport, damp, err);
+      }
+    } else {
+      float acc[VEC];
+#pragma unroll
+      for (int j = 0; j < VEC; j++) acc[j] = 0.0f;
+      sum_edges<VEC>(col, C, k, col0, tl.z, tl.w, acc);
+      store_vec<VEC>(seg_part + (int64_t)(-tl.y - 1) * k + col0, acc);
+    }
+  }
+  // workgroup partial per column: waves in order, through LDS
+  __shared__ double s_err[kPWaves][64 * VEC];
+#pragma unroll
+  for (int j = 0; j < VEC; j++) s_err[wid][lane * VEC + j] = err[j];
+  __syncthreads();
+  for (int c = threadIdx.x; c < 64 * VEC; c += kPBlock) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kPWaves; w++) s += s_err[w][c];
+    err_part[(int64_t)blockIdx.x * k + blockIdx.y * 64 * VEC + c] = s;
+  }
+}
+
+// Hub rows: segment partials added in segment order, then the row update.
+template <int VEC>
+__global__ __launch_bounds__(kPBlock) void ppr_hub_kernel(
+    const int32_t* __restrict__ hub_row, const int32_t* __restrict__ hub_slot,
+    const int32_t* __restrict__ hub_nseg, int n_hubs, const float* __restrict__ seg_part,
+    const int32_t* __restrict__ outdeg, const int32_t* __restrict__ seeds, int64_t k,
+    float* __restrict__ X, float* __restrict__ C0, float* __restrict__ C1,
+    double* __restrict__ hub_err, const PprState* __restrict__ st, float teleport, float damp) {
+  if (st->done) return;
+  const int odd = st->iter & 1;
+  float* __restrict__ Cn = odd ? C0 : C1;
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int col0 = blockIdx.y * 64 * VEC + lane * VEC;
+  int seed[VEC];
+#pragma unroll
+  for (int j = 0; j < VEC; j++) seed[j] = seeds[col0 + j];
+  double err[VEC];
+#pragma unroll
+  for (int j = 0; j < VEC; j++) err[j] = 0.0;
+  for (int h = blockIdx.x * kPWaves + wid; h < n_hubs; h += gridDim.x * kPWaves) {
+    float acc[VEC];
+#pragma unroll
+    for (int j = 0; j < VEC; j++) acc[j] = 0.0f;
+    const int s0 = hub_slot[h], ns = hub_nseg[h];
+    for (int q = 0; q < ns; q++) {
+      const Vec<VEC> p = load_vec<VEC>(seg_part + (int64_t)(s0 + q) * k + col0);
+#pragma unroll
+      for (int j = 0; j < VEC; j++) acc[j] += p.v[j];
+    }
+    row_update<VEC>(hub_row[h], acc, seed, X, Cn, outdeg, k, col0, teleport, damp, err);
+  }
+  __shared__ double s_err[kPWaves][64 * VEC];
+#pragma unroll
+  for (int j = 0; j < VEC; j++) s_err[wid][lane * VEC + j] = err[j];
+  __syncthreads();
+  for (int c = threadIdx.x; c < 64 * VEC; c += kPBlock) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kPWaves; w++) s += s_err[w][c];
+    hub_err[(int64_t)blockIdx.x * k + blockIdx.y * 64 * VEC + c] = s;
+  }
+}
+
+// First level: fin[s][c] = sum over the rows of err_part in slice s (fixed order).
+__global__ __launch_bounds__(kPBlock) void ppr_err_slices_kernel(const double* __restrict__ part,
+                                                                 int rows, int64_t k,
+                                                                 double* __restrict__ fin,
+                                                                 const PprState* __restrict__ st) {
+  if (st->done) return;
+  const int64_t c = (int64_t)blockIdx.x * kPBlock + threadIdx.x;
+  if (c >= k) return;
+  const int per = (rows + kSlices - 1) / kSlices;
+  const int lo = blockIdx.y * per, hi = min(lo + per, rows);
+  double s = 0.0;
+  for (int r = lo; r < hi; r++) s += part[(int64_t)r * k + c];
+  fin[(int64_t)blockIdx.y * k + c] = s;
+}
+
+// Second level: per-column L1 (slices in order), max over columns, state update.
+__global__ __launch_bounds__(kPBlock) void ppr_finalize_kernel(const double* __restrict__ fin,
+                                                               int64_t k,
+                                                               double* __restrict__ col_err,
+                                                               PprState* __restrict__ st,
+                                                               double* __restrict__ errors) {
+  if (st->done) return;
+  __shared__ double s_max[kPBlock];
+  double mx = 0.0;
+  for (int64_t c = threadIdx.x; c < k; c += kPBlock) {
+    double s = 0.0;
+    for (int q = 0; q < 2 * kSlices; q++) s += fin[(int64_t)q * k + c];
+    col_err[c] = s;
+    mx = fmax(mx, s);
+  }
+  s_max[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = kPBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int it = st->iter;
+    errors[it] = s_max[0];
+    st->iter = it + 1;
+    if (s_max[0] < st->tol || it + 1 >= st->max_iter) st->done = 1;
+  }
+}
+
+__global__ void ppr_init_kernel(int64_t n, int64_t k, const float* __restrict__ X,
+                                const int32_t* __restrict__ outdeg, float* __restrict__ C0,
+                                PprState* __restrict__ st, int max_iter, double tol) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    C0[i] = X[i] / (float)outdeg[i / k];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->iter = 0;
+    st->done = max_iter <= 0;
+    st->max_iter = max_iter;
+    st->tol = tol;
+  }
+}
+
+// rows of X between the user's vertex order and the plan's
+__global__ void ppr_permute_rows_kernel(int m, int64_t k, const int32_t* __restrict__ perm,
+                                        const float* __restrict__ src, float* __restrict__ dst,
+                                        int to_internal) {
+  const int lane = threadIdx.x & 63;
+  for (int i = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; i < m;
+       i += gridDim.x * (blockDim.x / 64)) {
+    const int v = perm[i];
+    const float* s = src + (int64_t)(to_internal ? v : i) * k;
+    float* d = dst + (int64_t)(to_internal ? i : v) * k;
+    for (int64_t j = lane; j < k; j += 64) d[j] = s[j];
+  }
+}
+
+__global__ void ppr_map_seeds_kernel(int k, const int32_t* __restrict__ seeds,
+                                     const int32_t* __restrict__ inv, int32_t* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < k) out[j] = inv ? inv[seeds[j]] : seeds[j];
+}
+
+__global__ void check_seeds_kernel(int k, const int32_t* __restrict__ seeds, int m,
+                                   int* __restrict__ bad) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < k && (seeds[j] < 0 || seeds[j] >= m)) atomicOr(bad, 1);
+}
+
+}  // namespace
+
+}  // namespace gd
+
+using namespace gd;
+
+struct gd_ppr_plan {
+  int m = 0, k = 0, kc = 64;
+  int64_t nnz = 0;
+  hipStream_t stream = nullptr;
+  bool relabel = false;
+  const int32_t* off = nullptr;
+  const int32_t* col = nullptr;
+  const int32_t* outdeg = nullptr;
+  Relabel rl;
+  Schedule sched;
+  int n_tiles = 0, n_hubs = 0, grid = 0, hub_grid = 0;
+  DeviceBuffer tiles, hub_row, hub_slot, hub_nseg, seg_part, err_part, hub_err, fin, col_err,
+      C0, C1, Xint, seeds, state, errors;
+  int errors_cap = 0;
+  float plan_ms = 0.0f;
+};
+
+namespace {
+
+int ppr_build(gd_ppr_plan* p, const int32_t* d_off, const int32_t* d_col,
+              const int32_t* d_outdeg) {
+  const int m = p->m;
+  int rc = check_columns(d_col, p->nnz, m, p->stream);
+  if (rc) return rc;
+  std::vector<int32_t> h_off(m + 1);
+  GD_HIP_TRY(hipMemcpyAsync(h_off.data(), d_off, 4 * ((size_t)m + 1), hipMemcpyDeviceToHost,
+                            p->stream));
+  GD_HIP_TRY(hipStreamSynchronize(p->stream));
+  {
+    Schedule tmp;  // validates the offsets before any device walk
+    rc = build_schedule(m, p->nnz, h_off.data(), kSegEdges, m, &tmp);
+    if (rc) return rc;
+  }
+  p->off = d_off;
+  p->col = d_col;
+  p->outdeg = d_outdeg;
+  if (p->relabel) {
+    rc = build_relabel(m, p->nnz, d_off, d_col, d_outdeg, p->stream, &p->rl);
+    if (rc) return rc;
+    p->off = p->rl.off.as<int32_t>();
+    p->col = p->rl.col.as<int32_t>();
+    p->outdeg = p->rl.outdeg.as<int32_t>();
+    GD_HIP_TRY(hipMemcpyAsync(h_off.data(), p->off, 4 * ((size_t)m + 1), hipMemcpyDeviceToHost,
+                              p->stream));
+    GD_HIP_TRY(hipStreamSynchronize(p->stream));
+  }
+  // wavefront tiles: <= kTileEdges edges / kTileRowsP rows; rows > kSegEdges split
+  Schedule& s = p->sched;
+  {
+    Schedule hubs;
+    rc = build_schedule(m, p->nnz, h_off.data(), kSegEdges, m, &hubs);
+    if (rc) return rc;
+    s.hub_row = hubs.hub_row;
+    s.hub_slot = hubs.hub_slot;
+    s.hub_nseg = hubs.hub_nseg;
+    s.n_slots = hubs.n_slots;
+    s.max_row_len = hubs.max_row_len;
+    // re-tile the non-hub rows with the smaller wavefront tiles
+    Schedule small;
+    rc = build_schedule(m, p->nnz, h_off.data(), kTileEdges, kTileRowsP, &small);
+    if (rc) return rc;
+    // heaviest work first: hub segments, then single rows of (kTileEdges,
+    // kSegEdges] edges, then the packed row tiles
+    for (const Tile& t : hubs.tiles)
+      if (t.y < 0) s.tiles.push_back(t);
+    for (size_t h = 0; h < small.hub_row.size(); h++) {
+      const int r = small.hub_row[h];
+      const int d = h_off[r + 1] - h_off[r];
+      if (d <= kSegEdges) s.tiles.push_back(Tile{r, r + 1, h_off[r], h_off[r + 1]});
+    }
+    for (const Tile& t : small.tiles)
+      if (t.y >= 0) s.tiles.push_back(t);
+  }
+  p->n_tiles = (int)s.tiles.size();
+  p->n_hubs = (int)s.hub_row.size();
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  p->grid = std::max(1, std::min(cus * 8, (p->n_tiles + kPWaves - 1) / kPWaves));
+  p->hub_grid = std::max(1, std::min(cus * 2, (p->n_hubs + kPWaves - 1) / kPWaves));
+  const size_t mk = (size_t)m * p->k;
+  rc = p->tiles.alloc(sizeof(Tile) * p->n_tiles);
+  if (!rc) rc = p->hub_row.alloc(4 * (size_t)std::max(1, p->n_hubs));
+  if (!rc) rc = p->hub_slot.alloc(4 * (size_t)std::max(1, p->n_hubs));
+  if (!rc) rc = p->hub_nseg.alloc(4 * (size_t)std::max(1, p->n_hubs));
+  if (!rc) rc = p->seg_part.alloc(4 * (size_t)std::max(1, s.n_slots) * p->k);
+  if (!rc) rc = p->err_part.alloc(8 * (size_t)p->grid * p->k);
+  if (!rc) rc = p->hub_err.alloc(8 * (size_t)p->hub_grid * p->k);
+  if (!rc) rc = p->fin.alloc(8 * (size_t)2 * kSlices * p->k);
+  if (!rc) rc = p->col_err.alloc(8 * (size_t)p->k);
+  if (!rc) rc = p->C0.alloc(4 * mk);
+  if (!rc) rc = p->C1.alloc(4 * mk);
+  if (!rc && p->relabel) rc = p->Xint.alloc(4 * mk);
+  if (!rc) rc = p->seeds.alloc(4 * (size_t)p->k);
+  if (!rc) rc = p->state.alloc(sizeof(PprState));
+  if (rc) return rc;
+  GD_HIP_TRY(hipMemcpyAsync(p->tiles.ptr, s.tiles.data(), sizeof(Tile) * p->n_tiles,
+                            hipMemcpyHostToDevice, p->stream));
+  if (p->n_hubs) {
+    GD_HIP_TRY(hipMemcpyAsync(p->hub_row.ptr, s.hub_row.data(), 4 * p->n_hubs,
+                              hipMemcpyHostToDevice, p->stream));
+    GD_HIP_TRY(hipMemcpyAsync(p->hub_slot.ptr, s.hub_slot.data(), 4 * p->n_hubs,
+                              hipMemcpyHostToDevice, p->stream));
+    GD_HIP_TRY(hipMemcpyAsync(p->hub_nseg.ptr, s.hub_nseg.data(), 4 * p->n_hubs,
+                              hipMemcpyHostToDevice, p->stream));
+  }
+  GD_HIP_TRY(hipMemsetAsync(p->hub_err.ptr, 0, 8 * (size_t)p->hub_grid * p->k, p->stream));
+  GD_HIP_TRY(hipStreamSynchronize(p->stream));
+  return GD_OK;
+}
+
+template <int VEC>
+void ppr_launch_iteration(gd_ppr_plan* p, float* X, PprState* st, float teleport, float damp) {
+  const int chunks = p->k / (64 * VEC);
+  float* C0 = p->C0.as<float>();
+  float* C1 = p->C1.as<float>();
+  hipLaunchKernelGGL(ppr_pull_kernel<VEC>, dim3(p->grid, chunks), dim3(kPBlock), 0, p->stream,
+                     p->tiles.as<Tile>(), p->n_tiles, p->off, p->col, p->outdeg,
+                     p->seeds.as<int32_t>(), (int64_t)p->k, X, C0, C1, p->seg_part.as<float>(),
+                     p->err_part.as<double>(), st, teleport, damp);
+  if (p->n_hubs)
+    hipLaunchKernelGGL(ppr_hub_kernel<VEC>, dim3(p->hub_grid, chunks), dim3(kPBlock), 0,
+                       p->stream, p->hub_row.as<int32_t>(), p->hub_slot.as<int32_t>(),
+                       p->hub_nseg.as<int32_t>(), p->n_hubs, p->seg_part.as<float>(), p->outdeg,
+                       p->seeds.as<int32_t>(), (int64_t)p->k, X, C0, C1,
+                       p->hub_err.as<double>(), st, teleport, damp);
+  const dim3 sg((p->k + kPBlock - 1) / kPBlock, kSlices);
+  double* fin = p->fin.as<double>();
+  hipLaunchKernelGGL(ppr_err_slices_kernel, sg, dim3(kPBlock), 0, p->stream,
+                     p->err_part.as<double>(), p->grid, (int64_t)p->k, fin, st);
+  hipLaunchKernelGGL(ppr_err_slices_kernel, sg, dim3(kPBlock), 0, p->stream,
+                     p->hub_err.as<double>(), p->hub_grid, (int64_t)p->k,
+                     fin + (size_t)kSlices * p->k, st);
+  hipLaunchKernelGGL(ppr_finalize_kernel, dim3(1), dim3(kPBlock), 0, p->stream, fin,
+                     (int64_t)p->k, p->col_err.as<double>(), st, p->errors.as<double>());
+}
+
+}  // namespace
+
+extern "C" {
+
+void gd_ppr_default_plan_config(gd_ppr_plan_config* c) {
+  c->relabel = -1;
+  c->k_chunk = 0;
+}
+
+int gd_ppr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* d_col,
+                       const int32_t* d_outdeg, int k, void* stream, const gd_ppr_plan_config* cfg,
+                       gd_ppr_plan** out) {
+  GD_CHECK_ARG(out != nullptr, "plan output pointer is NULL");
+  *out = nullptr;
+  GD_CHECK_ARG(m >= 0 && nnz >= 0 && nnz < (1ll << 31), "m/nnz out of range");
+  GD_CHECK_ARG(m == 0 || (d_off && d_outdeg), "NULL device array");
+  GD_CHECK_ARG(nnz == 0 || d_col, "NULL column indices");
+  GD_CHECK_ARG(k > 0 && k % 64 == 0, "k must be a positive multiple of 64");
+  gd_ppr_plan_config c;
+  if (cfg)
+    c = *cfg;
+  else
+    gd_ppr_default_plan_config(&c);
+  GD_CHECK_ARG(c.k_chunk == 0 || c.k_chunk == 64 || c.k_chunk == 128 || c.k_chunk == 256,
+               "k_chunk must be 0, 64, 128 or 256");
+  GD_CHECK_ARG(c.k_chunk == 0 || k % c.k_chunk == 0, "k must be a multiple of k_chunk");
+  if (gd_device_count() <= 0) {
+    set_error("no HIP device visible");
+    return GD_ERR_NO_DEVICE;
+  }
+  gd_ppr_plan* p = new gd_ppr_plan();
+  p->m = m;
+  p->nnz = nnz;
+  p->k = k;
+  p->kc = c.k_chunk ? c.k_chunk : (k % 256 == 0 ? 256 : (k % 128 == 0 ? 128 : 64));
+  p->stream = (hipStream_t)stream;
+  p->relabel = (c.relabel == 1) || (c.relabel == -1 && nnz >= (1 << 20));
+  int rc = GD_OK;
+  if (m > 0) {
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (hipEventCreate(&t0) == hipSuccess && hipEventCreate(&t1) == hipSuccess)
+      (void)hipEventRecord(t0, p->stream);
+    rc = ppr_build(p, d_off, d_col, d_outdeg);
+    if (t0 && t1) {
+      (void)hipEventRecord(t1, p->stream);
+      (void)hipEventSynchronize(t1);
+      (void)hipEventElapsedTime(&p->plan_ms, t0, t1);
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+  }
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return GD_OK;
+}
+
+int gd_ppr_plan_info(const gd_ppr_plan* p, gd_ppr_plan_info_t* info) {
+  GD_CHECK_ARG(p != nullptr && info != nullptr, "NULL plan / info");
+  info->n_tiles = p->n_tiles;
+  info->n_hubs = p->n_hubs;
+  info->k = p->k;
+  info->k_chunk = p->kc;
+  info->relabel = p->relabel ? 1 : 0;
+  info->grid = p->grid;
+  info->plan_ms = p->plan_ms;
+  return GD_OK;
+}
+
+int gd_ppr_plan_destroy(gd_ppr_plan* p) {
+  delete p;
+  return GD_OK;
+}
+
+int gd_ppr_plan_run(gd_ppr_plan* p, const int32_t* d_seeds, float* d_X,
+                    const gd_pr_options* opts, gd_pr_stats* stats, double* errors,
+                    double* col_errors) {
+  GD_CHECK_ARG(p != nullptr, "NULL plan");
+  gd_pr_options o;
+  if (opts)
+    o = *opts;
+  else
+    gd_pr_default_options(&o);
+  GD_CHECK_ARG(o.max_iter >= 0, "max_iter must be >= 0");
+  gd_pr_stats s;
+  memset(&s, 0, sizeof(s));
+  const int m = p->m;
+  if (m == 0) {
+    if (stats) *stats = s;
+    return GD_OK;
+  }
+  GD_CHECK_ARG(d_seeds != nullptr && d_X != nullptr, "NULL seeds / X");
+  {
+    DeviceBuffer bad;
+    int rc = bad.alloc(4);
+    if (rc) return rc;
+    GD_HIP_TRY(hipMemsetAsync(bad.ptr, 0, 4, p->stream));
+    hipLaunchKernelGGL(check_seeds_kernel, dim3((p->k + 255) / 256), dim3(256), 0, p->stream,
+                       p->k, d_seeds, m, bad.as<int>());
+    int h = 0;
+    GD_HIP_TRY(hipMemcpyAsync(&h, bad.ptr, 4, hipMemcpyDeviceToHost, p->stream));
+    GD_HIP_TRY(hipStreamSynchronize(p->stream));
+    GD_CHECK_ARG(h == 0, "seed vertex out of range");
+  }
+  if (p->errors_cap < std::max(1, o.max_iter)) {
+    int rc = p->errors.alloc(sizeof(double) * std::max(1, o.max_iter));
+    if (rc) return rc;
+    p->errors_cap = std::max(1, o.max_iter);
+  }
+  const float damp = o.damping, teleport = 1.0f - damp;
+  const int chunk = o.chunk > 0 ? o.chunk : (o.tol > 0 ? 4 : std::max(1, o.max_iter));
+  hipEvent_t t0, t1;
+  GD_HIP_TRY(hipEventCreate(&t0));
+  GD_HIP_TRY(hipEventCreate(&t1));
+  PprState* st = p->state.as<PprState>();
+  float* X = p->relabel ? p->Xint.as<float>() : d_X;
+  const int64_t n = (int64_t)m * p->k;
+  GD_HIP_TRY(hipEventRecord(t0, p->stream));
+  hipLaunchKernelGGL(ppr_map_seeds_kernel, dim3((p->k + 255) / 256), dim3(256), 0, p->stream,
+                     p->k, d_seeds, p->relabel ? p->rl.inv.as<int32_t>() : nullptr,
+                     p->seeds.as<int32_t>());
+  if (p->relabel)
+    hipLaunchKernelGGL(ppr_permute_rows_kernel, dim3(std::min(16384, (m + 3) / 4)), dim3(256), 0,
+                       p->stream, m, (int64_t)p->k, p->rl.perm.as<int32_t>(), d_X, X, 1);
+  hipLaunchKernelGGL(ppr_init_kernel, dim3((int)std::min<int64_t>((n + 255) / 256, 65536)),
+                     dim3(256), 0, p->stream, n, (int64_t)p->k, X, p->outdeg,
+                     p->C0.as<float>(), st, o.max_iter, o.tol);
+  GD_HIP_TRY(hipGetLastError());
+  int launched = 0;
+  PprState h{};
+  while (launched < o.max_iter) {
+    const int nl = std::min(chunk, o.max_iter - launched);
+    for (int q = 0; q < nl; q++) {
+      if (p->kc == 256)
+        ppr_launch_iteration<4>(p, X, st, teleport, damp);
+      else if (p->kc == 128)
+        ppr_launch_iteration<2>(p, X, st, teleport, damp);
+      else
+        ppr_launch_iteration<1>(p, X, st, teleport, damp);
+    }
+    GD_HIP_TRY(hipGetLastError());
+    launched += nl;
+    GD_HIP_TRY(hipMemcpyAsync(&h, st, sizeof(PprState), hipMemcpyDeviceToHost, p->stream));
+    GD_HIP_TRY(hipStreamSynchronize(p->stream));
+    if (h.done) break;
+  }
+  if (p->relabel)
+    hipLaunchKernelGGL(ppr_permute_rows_kernel, dim3(std::min(16384, (m + 3) / 4)), dim3(256), 0,
+                       p->stream, m, (int64_t)p->k, p->rl.perm.as<int32_t>(), X, d_X, 0);
+  GD_HIP_TRY(hipEventRecord(t1, p->stream));
+  GD_HIP_TRY(hipEventSynchronize(t1));
+  GD_HIP_TRY(hipEventElapsedTime(&s.solve_ms, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  s.iterations = h.iter;
+  std::vector<double> errs(std::max(1, h.iter));
+  if (h.iter > 0) {
+    GD_HIP_TRY(hipMemcpy(errs.data(), p->errors.ptr, 8 * (size_t)h.iter, hipMemcpyDeviceToHost));
+    s.final_error = errs[h.iter - 1];
+    s.converged = s.final_error < o.tol;
+    if (errors)
+      for (int i = 0; i < h.iter; i++) errors[i] = errs[i];
+    if (col_errors)
+      GD_HIP_TRY(hipMemcpy(col_errors, p->col_err.ptr, 8 * (size_t)p->k, hipMemcpyDeviceToHost));
+  }
+  if (stats) *stats = s;
+  return GD_OK;
+}
+
+}  // extern "C"

@@ -23,11 +23,10 @@
 #include "gardenia_hip.h"
 #include "gd_common.h"
 #include "pull_engine.h"
+#include "relabel.h"
 
 namespace gd {
 
-constexpr int kPrIPT = 8;
-constexpr int kPrTileNnz = kBlock * kPrIPT;
 constexpr int kFinBlocks = 128;  // finalize workgroups (<= kBlock: combined by one block)
 
 struct PrState {
@@ -83,6 +82,7 @@ __global__ __launch_bounds__(kBlock) void pr_init_kernel(int m, const float* __r
   }
 }
 
+template <int IPT>
 __global__ __launch_bounds__(kBlock) void pr_pull_kernel(
     const Tile* __restrict__ tiles, const int32_t* __restrict__ off,
     const int32_t* __restrict__ col, const int32_t* __restrict__ outdeg,
@@ -91,16 +91,16 @@ __global__ __launch_bounds__(kBlock) void pr_pull_kernel(
     float base, float damp, int seed) {
   if (st->done) return;
   const int odd = st->iter & 1;
-  __shared__ RowTileLds<kPrIPT> lds;
+  __shared__ RowTileLds<IPT> lds;
   __shared__ double s_err[kWaves];
   const Tile t = tiles[blockIdx.x];
   const PrGather g{odd ? contrib1 : contrib0};
   double err = 0.0;
   if (t.y >= 0) {
     const PrEpilogue e{scores, odd ? contrib0 : contrib1, outdeg, base, damp, seed};
-    err = row_tile<kPrIPT>(off, col, t.x, t.y, t.z, t.w - t.z, g, e, lds);
+    err = row_tile<IPT>(off, col, t.x, t.y, t.z, t.w - t.z, g, e, lds);
   } else {
-    const float p = segment_sum<kPrIPT>(col, t.z, t.w - t.z, g, lds.val);
+    const float p = segment_sum<IPT>(col, t.z, t.w - t.z, g, lds.val);
     if (threadIdx.x == 0) hub_part[-t.y - 1] = p;
   }
   const double tot = block_sum<kWaves>(err, s_err);
@@ -169,10 +169,16 @@ using namespace gd;
 struct gd_pr_plan {
   int m = 0;
   int64_t nnz = 0;
+  hipStream_t stream = nullptr;
+  gd_pr_plan_config cfg{};
+  bool relabel = false;
+  int ipt = 8;
+  // arrays the kernels read: the caller's, or the relabelled copies
   const int32_t* off = nullptr;
   const int32_t* col = nullptr;
   const int32_t* outdeg = nullptr;
-  hipStream_t stream = nullptr;
+  Relabel rl;
+  DeviceBuffer s_int;  // scores in the internal order (relabel only)
   Schedule sched;
   int n_tiles = 0, n_hubs = 0;
   DeviceBuffer tiles, hub_row, hub_slot, hub_nseg, hub_part, err_part, fin_part, contrib0,
@@ -180,7 +186,80 @@ struct gd_pr_plan {
   int fin_blocks = 1;
   int errors_cap = 0;
   std::vector<hipEvent_t> ev;
+  float plan_ms = 0.0f;
 };
+
+namespace {
+
+int upload(DeviceBuffer& b, const void* src, size_t n, hipStream_t s) {
+  if (!n) return GD_OK;
+  GD_HIP_TRY(hipMemcpyAsync(b.ptr, src, n, hipMemcpyHostToDevice, s));
+  return GD_OK;
+}
+
+int build_plan(gd_pr_plan* p, const int32_t* d_off, const int32_t* d_col,
+               const int32_t* d_outdeg) {
+  const int m = p->m;
+  int rc = check_columns(d_col, p->nnz, m, p->stream);
+  if (rc) return rc;
+  p->off = d_off;
+  p->col = d_col;
+  p->outdeg = d_outdeg;
+  if (p->relabel) {
+    // offsets must be valid before the device walks them
+    std::vector<int32_t> h_off(m + 1);
+    GD_HIP_TRY(hipMemcpyAsync(h_off.data(), d_off, 4 * ((size_t)m + 1), hipMemcpyDeviceToHost,
+                              p->stream));
+    GD_HIP_TRY(hipStreamSynchronize(p->stream));
+    Schedule tmp;
+    rc = build_schedule(m, p->nnz, h_off.data(), kBlock * p->ipt, kTileRows, &tmp);
+    if (rc) return rc;
+    rc = build_relabel(m, p->nnz, d_off, d_col, d_outdeg, p->stream, &p->rl);
+    if (rc) return rc;
+    rc = p->s_int.alloc(4 * (size_t)m);
+    if (rc) return rc;
+    p->off = p->rl.off.as<int32_t>();
+    p->col = p->rl.col.as<int32_t>();
+    p->outdeg = p->rl.outdeg.as<int32_t>();
+  }
+  std::vector<int32_t> h_off(m + 1);
+  GD_HIP_TRY(hipMemcpyAsync(h_off.data(), p->off, 4 * ((size_t)m + 1), hipMemcpyDeviceToHost,
+                            p->stream));
+  GD_HIP_TRY(hipStreamSynchronize(p->stream));
+  rc = build_schedule(m, p->nnz, h_off.data(), kBlock * p->ipt, kTileRows, &p->sched);
+  if (rc) return rc;
+  p->n_tiles = (int)p->sched.tiles.size();
+  p->n_hubs = (int)p->sched.hub_row.size();
+  rc = p->tiles.alloc(sizeof(Tile) * p->n_tiles);
+  if (!rc) rc = p->hub_row.alloc(4 * (size_t)std::max(1, p->n_hubs));
+  if (!rc) rc = p->hub_slot.alloc(4 * (size_t)std::max(1, p->n_hubs));
+  if (!rc) rc = p->hub_nseg.alloc(4 * (size_t)std::max(1, p->n_hubs));
+  if (!rc) rc = p->hub_part.alloc(4 * (size_t)std::max(1, p->sched.n_slots));
+  if (!rc) rc = p->err_part.alloc(8 * (size_t)std::max(1, p->n_tiles));
+  if (!rc) rc = p->contrib0.alloc(4 * (size_t)m);
+  if (!rc) rc = p->contrib1.alloc(4 * (size_t)m);
+  if (!rc) rc = p->state.alloc(sizeof(PrState));
+  if (!rc) rc = p->fin_part.alloc(8 * (size_t)kFinBlocks);
+  if (rc) return rc;
+  p->fin_blocks = std::max(1, std::min(kFinBlocks, std::max(p->n_tiles, p->n_hubs) / kBlock));
+  rc = upload(p->tiles, p->sched.tiles.data(), sizeof(Tile) * p->n_tiles, p->stream);
+  if (!rc) rc = upload(p->hub_row, p->sched.hub_row.data(), 4 * (size_t)p->n_hubs, p->stream);
+  if (!rc) rc = upload(p->hub_slot, p->sched.hub_slot.data(), 4 * (size_t)p->n_hubs, p->stream);
+  if (!rc) rc = upload(p->hub_nseg, p->sched.hub_nseg.data(), 4 * (size_t)p->n_hubs, p->stream);
+  if (rc) return rc;
+  GD_HIP_TRY(hipStreamSynchronize(p->stream));
+  return GD_OK;
+}
+
+template <int IPT>
+void launch_pull(gd_pr_plan* p, float* scores, float* c0, float* c1, PrState* st, float base,
+                 float damp, int seed) {
+  hipLaunchKernelGGL(pr_pull_kernel<IPT>, dim3(p->n_tiles), dim3(kBlock), 0, p->stream,
+                     p->tiles.as<Tile>(), p->off, p->col, p->outdeg, scores, c0, c1,
+                     p->hub_part.as<float>(), p->err_part.as<double>(), st, base, damp, seed);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -199,13 +278,27 @@ void gd_pr_default_options(gd_pr_options* o) {
   o->profile = 0;
 }
 
-int gd_pr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* d_col,
-                      const int32_t* d_outdeg, void* stream, gd_pr_plan** out) {
+void gd_pr_default_plan_config(gd_pr_plan_config* c) {
+  c->relabel = -1;
+  c->items_per_thread = 0;
+}
+
+int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t* d_col,
+                         const int32_t* d_outdeg, void* stream, const gd_pr_plan_config* cfg,
+                         gd_pr_plan** out) {
   GD_CHECK_ARG(out != nullptr, "plan output pointer is NULL");
   *out = nullptr;
   GD_CHECK_ARG(m >= 0 && nnz >= 0 && nnz < (1ll << 31), "m/nnz out of range");
   GD_CHECK_ARG(m == 0 || (d_off && d_outdeg), "NULL device array");
   GD_CHECK_ARG(nnz == 0 || d_col, "NULL column indices");
+  gd_pr_plan_config c;
+  if (cfg)
+    c = *cfg;
+  else
+    gd_pr_default_plan_config(&c);
+  GD_CHECK_ARG(c.items_per_thread == 0 || c.items_per_thread == 8 || c.items_per_thread == 16,
+               "items_per_thread must be 0, 8 or 16");
+  GD_CHECK_ARG(c.relabel >= -1 && c.relabel <= 1, "relabel must be -1, 0 or 1");
   if (gd_device_count() <= 0) {
     set_error("no HIP device visible");
     return GD_ERR_NO_DEVICE;
@@ -213,51 +306,23 @@ int gd_pr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* d
   gd_pr_plan* p = new gd_pr_plan();
   p->m = m;
   p->nnz = nnz;
-  p->off = d_off;
-  p->col = d_col;
-  p->outdeg = d_outdeg;
   p->stream = (hipStream_t)stream;
+  p->cfg = c;
+  p->ipt = c.items_per_thread ? c.items_per_thread : 8;
+  p->relabel = (c.relabel == 1) || (c.relabel == -1 && nnz >= (1 << 20));
   int rc = GD_OK;
   if (m > 0) {
-    std::vector<int32_t> h_off(m + 1);
-    hipError_t e = hipMemcpyAsync(h_off.data(), d_off, sizeof(int32_t) * (m + 1),
-                                  hipMemcpyDeviceToHost, p->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
-    if (e != hipSuccess) {
-      set_error(std::string("copying row offsets: ") + hipGetErrorString(e));
-      rc = GD_ERR_DEVICE;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (hipEventCreate(&t0) == hipSuccess && hipEventCreate(&t1) == hipSuccess)
+      (void)hipEventRecord(t0, p->stream);
+    rc = build_plan(p, d_off, d_col, d_outdeg);
+    if (t0 && t1) {
+      (void)hipEventRecord(t1, p->stream);
+      (void)hipEventSynchronize(t1);
+      (void)hipEventElapsedTime(&p->plan_ms, t0, t1);
     }
-    if (!rc) rc = build_schedule(m, nnz, h_off.data(), kPrTileNnz, kTileRows, &p->sched);
-    if (!rc) rc = check_columns(d_col, nnz, m, p->stream);
-    if (!rc) {
-      p->n_tiles = (int)p->sched.tiles.size();
-      p->n_hubs = (int)p->sched.hub_row.size();
-      rc = p->tiles.alloc(sizeof(Tile) * p->n_tiles);
-      if (!rc) rc = p->hub_row.alloc(sizeof(int32_t) * std::max(1, p->n_hubs));
-      if (!rc) rc = p->hub_slot.alloc(sizeof(int32_t) * std::max(1, p->n_hubs));
-      if (!rc) rc = p->hub_nseg.alloc(sizeof(int32_t) * std::max(1, p->n_hubs));
-      if (!rc) rc = p->hub_part.alloc(sizeof(float) * std::max(1, p->sched.n_slots));
-      if (!rc) rc = p->err_part.alloc(sizeof(double) * std::max(1, p->n_tiles));
-      if (!rc) rc = p->contrib0.alloc(sizeof(float) * m);
-      if (!rc) rc = p->contrib1.alloc(sizeof(float) * m);
-      if (!rc) rc = p->state.alloc(sizeof(PrState));
-      if (!rc) rc = p->fin_part.alloc(sizeof(double) * kFinBlocks);
-      p->fin_blocks = std::max(1, std::min(kFinBlocks, std::max(p->n_tiles, p->n_hubs) / kBlock));
-    }
-    if (!rc) {
-      auto cp = [&](DeviceBuffer& b, const void* src, size_t n) -> hipError_t {
-        return n ? hipMemcpyAsync(b.ptr, src, n, hipMemcpyHostToDevice, p->stream) : hipSuccess;
-      };
-      hipError_t e2 = cp(p->tiles, p->sched.tiles.data(), sizeof(Tile) * p->n_tiles);
-      if (e2 == hipSuccess) e2 = cp(p->hub_row, p->sched.hub_row.data(), 4 * p->n_hubs);
-      if (e2 == hipSuccess) e2 = cp(p->hub_slot, p->sched.hub_slot.data(), 4 * p->n_hubs);
-      if (e2 == hipSuccess) e2 = cp(p->hub_nseg, p->sched.hub_nseg.data(), 4 * p->n_hubs);
-      if (e2 == hipSuccess) e2 = hipStreamSynchronize(p->stream);
-      if (e2 != hipSuccess) {
-        set_error(std::string("uploading schedule: ") + hipGetErrorString(e2));
-        rc = GD_ERR_DEVICE;
-      }
-    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
   }
   if (rc) {
     delete p;
@@ -267,13 +332,20 @@ int gd_pr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* d
   return GD_OK;
 }
 
-int gd_pr_plan_info(const gd_pr_plan* p, int64_t* n_tiles, int64_t* n_hubs, int64_t* n_slots,
-                    int32_t* max_row_len) {
-  GD_CHECK_ARG(p != nullptr, "NULL plan");
-  if (n_tiles) *n_tiles = p->n_tiles;
-  if (n_hubs) *n_hubs = p->n_hubs;
-  if (n_slots) *n_slots = p->sched.n_slots;
-  if (max_row_len) *max_row_len = p->sched.max_row_len;
+int gd_pr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* d_col,
+                      const int32_t* d_outdeg, void* stream, gd_pr_plan** out) {
+  return gd_pr_plan_create_ex(m, nnz, d_off, d_col, d_outdeg, stream, nullptr, out);
+}
+
+int gd_pr_plan_info(const gd_pr_plan* p, gd_pr_plan_info_t* info) {
+  GD_CHECK_ARG(p != nullptr && info != nullptr, "NULL plan / info");
+  info->n_tiles = p->n_tiles;
+  info->n_hubs = p->n_hubs;
+  info->n_hub_segments = p->sched.n_slots;
+  info->max_row_len = p->sched.max_row_len;
+  info->relabel = p->relabel ? 1 : 0;
+  info->items_per_thread = p->ipt;
+  info->plan_ms = p->plan_ms;
   return GD_OK;
 }
 
@@ -293,7 +365,7 @@ int gd_pr_plan_run(gd_pr_plan* p, float* d_scores, const gd_pr_options* opts, gd
   else
     gd_pr_default_options(&o);
   GD_CHECK_ARG(o.max_iter >= 0, "max_iter must be >= 0");
-  GD_CHECK_ARG(o.seed_vertex < p->m, "seed_vertex out of range");
+  GD_CHECK_ARG(o.seed_vertex >= -1 && o.seed_vertex < p->m, "seed_vertex out of range");
   GD_CHECK_ARG(p->m == 0 || d_scores != nullptr, "NULL scores");
   gd_pr_stats s;
   memset(&s, 0, sizeof(s));
@@ -315,9 +387,15 @@ int gd_pr_plan_run(gd_pr_plan* p, float* d_scores, const gd_pr_options* opts, gd
   }
   const float damp = o.damping;
   const float base = (o.seed_vertex < 0) ? (1.0f - damp) / (float)m : (1.0f - damp);
+  int seed = o.seed_vertex;
+  if (p->relabel && seed >= 0) {
+    GD_HIP_TRY(hipMemcpyAsync(&seed, p->rl.inv.as<int32_t>() + o.seed_vertex, 4,
+                              hipMemcpyDeviceToHost, p->stream));
+    GD_HIP_TRY(hipStreamSynchronize(p->stream));
+  }
   const int chunk = o.chunk > 0 ? o.chunk : (o.tol > 0 ? 8 : std::max(1, o.max_iter));
-  if (o.profile && (int)p->ev.size() < 2 * chunk + 2) {
-    while ((int)p->ev.size() < 2 * chunk + 2) {
+  if (o.profile) {
+    while ((int)p->ev.size() < 2 * chunk) {
       hipEvent_t e;
       GD_HIP_TRY(hipEventCreate(&e));
       p->ev.push_back(e);
@@ -329,9 +407,14 @@ int gd_pr_plan_run(gd_pr_plan* p, float* d_scores, const gd_pr_options* opts, gd
   PrState* st = p->state.as<PrState>();
   float* c0 = p->contrib0.as<float>();
   float* c1 = p->contrib1.as<float>();
+  float* work = p->relabel ? p->s_int.as<float>() : d_scores;
   const int init_blocks = std::min((m + kBlock - 1) / kBlock, 8192);
   GD_HIP_TRY(hipEventRecord(t0, p->stream));
-  hipLaunchKernelGGL(pr_init_kernel, dim3(init_blocks), dim3(kBlock), 0, p->stream, m, d_scores,
+  if (p->relabel) {
+    int rc = relabel_to_internal(p->rl, m, d_scores, work, p->stream);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(pr_init_kernel, dim3(init_blocks), dim3(kBlock), 0, p->stream, m, work,
                      p->outdeg, c0, st, o.max_iter, o.tol);
   GD_HIP_TRY(hipGetLastError());
   int launched = 0;
@@ -342,17 +425,16 @@ int gd_pr_plan_run(gd_pr_plan* p, float* d_scores, const gd_pr_options* opts, gd
     const int n = std::min(chunk, o.max_iter - launched);
     for (int k = 0; k < n; k++) {
       if (o.profile) GD_HIP_TRY(hipEventRecord(p->ev[2 * k], p->stream));
-      hipLaunchKernelGGL(pr_pull_kernel, dim3(p->n_tiles), dim3(kBlock), 0, p->stream,
-                         p->tiles.as<Tile>(), p->off, p->col, p->outdeg, d_scores, c0, c1,
-                         p->hub_part.as<float>(), p->err_part.as<double>(), st, base, damp,
-                         o.seed_vertex);
+      if (p->ipt == 16)
+        launch_pull<16>(p, work, c0, c1, st, base, damp, seed);
+      else
+        launch_pull<8>(p, work, c0, c1, st, base, damp, seed);
       if (o.profile) GD_HIP_TRY(hipEventRecord(p->ev[2 * k + 1], p->stream));
       hipLaunchKernelGGL(pr_finalize_kernel, dim3(p->fin_blocks), dim3(kBlock), 0, p->stream,
                          p->hub_row.as<int32_t>(), p->hub_slot.as<int32_t>(),
                          p->hub_nseg.as<int32_t>(), p->n_hubs, p->hub_part.as<float>(),
-                         p->err_part.as<double>(), p->n_tiles, p->outdeg, d_scores, c0, c1, st,
-                         p->errors.as<double>(), p->fin_part.as<double>(), base, damp,
-                         o.seed_vertex);
+                         p->err_part.as<double>(), p->n_tiles, p->outdeg, work, c0, c1, st,
+                         p->errors.as<double>(), p->fin_part.as<double>(), base, damp, seed);
     }
     GD_HIP_TRY(hipGetLastError());
     launched += n;
@@ -369,6 +451,10 @@ int gd_pr_plan_run(gd_pr_plan* p, float* d_scores, const gd_pr_options* opts, gd
       }
     }
     if (h.done) break;
+  }
+  if (p->relabel) {
+    int rc = relabel_to_user(p->rl, m, work, d_scores, p->stream);
+    if (rc) return rc;
   }
   GD_HIP_TRY(hipEventRecord(t1, p->stream));
   GD_HIP_TRY(hipEventSynchronize(t1));

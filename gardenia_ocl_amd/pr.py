@@ -84,7 +84,9 @@ class PageRankPlan:
     Work is issued on ``stream`` (default: torch's current stream).
     """
 
-    def __init__(self, in_row_offsets, in_column_indices, out_degree, stream=None):
+    def __init__(self, in_row_offsets, in_column_indices, out_degree, stream=None, *,
+                 relabel=None, items_per_thread=0):
+        """relabel: None = library default (auto), True / False force it."""
         import torch
 
         for t in (in_row_offsets, in_column_indices, out_degree):
@@ -96,19 +98,23 @@ class PageRankPlan:
             raise ValueError("in_row_offsets must have m+1 entries")
         self._keep = (in_row_offsets, in_column_indices, out_degree)
         self.stream = stream if stream is not None else torch.cuda.current_stream()
+        cfg = N.PrPlanConfig()
+        N.lib().gd_pr_default_plan_config(ctypes.byref(cfg))
+        if relabel is not None:
+            cfg.relabel = 1 if relabel else 0
+        cfg.items_per_thread = int(items_per_thread)
         h = ctypes.c_void_p()
-        N.call("gd_pr_plan_create", self.m, self.nnz, in_row_offsets.data_ptr(),
+        N.call("gd_pr_plan_create_ex", self.m, self.nnz, in_row_offsets.data_ptr(),
                in_column_indices.data_ptr(), out_degree.data_ptr(),
-               ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h))
+               ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(cfg), ctypes.byref(h))
         self._h = h
 
     def info(self) -> dict:
-        t, hb, sl = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
-        mx = ctypes.c_int32()
-        N.call("gd_pr_plan_info", self._h, ctypes.byref(t), ctypes.byref(hb), ctypes.byref(sl),
-               ctypes.byref(mx))
-        return {"tiles": t.value, "hubs": hb.value, "hub_segments": sl.value,
-                "max_row_len": mx.value}
+        inf = N.PrPlanInfo()
+        N.call("gd_pr_plan_info", self._h, ctypes.byref(inf))
+        d = inf.as_dict()
+        d["tiles"], d["hubs"], d["hub_segments"] = d["n_tiles"], d["n_hubs"], d["n_hub_segments"]
+        return d
 
     def run(self, scores, *, damping=K_DAMP, tol=EPSILON, max_iter=MAX_ITER, seed_vertex=-1,
             chunk=0, profile=False):

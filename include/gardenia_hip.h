@@ -58,10 +58,36 @@ void gd_pr_default_options(gd_pr_options *opts);
 
 typedef struct gd_pr_plan gd_pr_plan;
 
+/* Plan-time choices (the autotuner's knobs). */
+typedef struct gd_pr_plan_config {
+  int relabel;          /* 1: relabel vertices by decreasing out-degree inside the plan
+                           (hot sources become an L2-resident prefix of contrib; rows keep
+                           their edge order; scores permuted in/out per run), 0: never,
+                           -1: auto (when nnz >= 2^20) */
+  int items_per_thread; /* edges per thread per tile: 8 (2048-edge tiles) or 16; 0: auto */
+} gd_pr_plan_config;
+
+void gd_pr_default_plan_config(gd_pr_plan_config *cfg);
+
+typedef struct gd_pr_plan_info_t {
+  int64_t n_tiles;        /* workgroups per pull launch */
+  int64_t n_hubs;         /* rows longer than a tile */
+  int64_t n_hub_segments; /* tiles spent on hub rows */
+  int32_t max_row_len;
+  int32_t relabel;        /* 1 if the plan relabelled the vertices */
+  int32_t items_per_thread;
+  float plan_ms;          /* plan construction time (validation, relabel, schedule) */
+} gd_pr_plan_info_t;
+
 /* Plan over a device-resident in-CSR (rows = destinations, columns = sources)
  * and the out-degree array, all borrowed (must outlive the plan).  Validates
- * the CSR (offsets, column range) and builds the degree-binned tile schedule
- * plus scratch (2 contrib buffers, partials). */
+ * the CSR (offsets, column range) and builds the tile schedule plus scratch
+ * (2 contrib buffers, partials).  cfg may be NULL (defaults). */
+int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t *d_in_row_offsets,
+                         const int32_t *d_in_column_indices, const int32_t *d_out_degree,
+                         void *stream, const gd_pr_plan_config *cfg, gd_pr_plan **plan);
+
+/* gd_pr_plan_create_ex with the default configuration. */
 int gd_pr_plan_create(int m, int64_t nnz, const int32_t *d_in_row_offsets,
                       const int32_t *d_in_column_indices, const int32_t *d_out_degree, void *stream,
                       gd_pr_plan **plan);
@@ -72,9 +98,7 @@ int gd_pr_plan_create(int m, int64_t nnz, const int32_t *d_in_row_offsets,
 int gd_pr_plan_run(gd_pr_plan *plan, float *d_scores, const gd_pr_options *opts,
                    gd_pr_stats *stats, double *errors);
 
-/* Schedule facts: tiles, hub rows, hub segments, longest row. */
-int gd_pr_plan_info(const gd_pr_plan *plan, int64_t *n_tiles, int64_t *n_hubs, int64_t *n_slots,
-                    int32_t *max_row_len);
+int gd_pr_plan_info(const gd_pr_plan *plan, gd_pr_plan_info_t *info);
 
 int gd_pr_plan_destroy(gd_pr_plan *plan);
 
@@ -86,6 +110,51 @@ int gd_pr_solve(int m, int nnz, const int32_t *in_row_offsets, const int32_t *in
                 const int32_t *out_row_offsets, const int32_t *out_column_indices,
                 const int32_t *degree, float *scores, const gd_pr_options *opts,
                 gd_pr_stats *stats, double *errors);
+
+/* ------------------------------------------------------------ batched personalised PageRank */
+
+/* k personalised PageRank vectors at once (no counterpart in the reference,
+ * which has no PPR): column j teleports its (1-d) mass to seeds[j].  Scores
+ * are vertex-major, X[v*k + j], so each gathered source index feeds k
+ * additions (an SpMM-style pull).  Same iteration as gd_pr_plan_run with a
+ * per-column teleport; a run stops when every column's L1 change is < tol. */
+typedef struct gd_ppr_plan gd_ppr_plan;
+
+typedef struct gd_ppr_plan_config {
+  int relabel; /* as gd_pr_plan_config.relabel */
+  int k_chunk; /* columns per wavefront pass: 64, 128 or 256 (one 256*k_chunk/64-byte
+                  gather per edge); 0: the largest that divides k */
+} gd_ppr_plan_config;
+
+void gd_ppr_default_plan_config(gd_ppr_plan_config *cfg);
+
+typedef struct gd_ppr_plan_info_t {
+  int64_t n_tiles;
+  int64_t n_hubs;
+  int32_t k;
+  int32_t k_chunk;
+  int32_t relabel;
+  int32_t grid; /* persistent workgroups per column chunk */
+  float plan_ms;
+} gd_ppr_plan_info_t;
+
+/* k must be a positive multiple of 64.  Arrays are borrowed, as for gd_pr_plan_create. */
+int gd_ppr_plan_create(int m, int64_t nnz, const int32_t *d_in_row_offsets,
+                       const int32_t *d_in_column_indices, const int32_t *d_out_degree, int k,
+                       void *stream, const gd_ppr_plan_config *cfg, gd_ppr_plan **plan);
+
+/* d_seeds: k device int32 vertex ids; d_X: m*k device floats, in: initial
+ * scores (typically one-hot at the seeds), out: result.  errors (host, may be
+ * NULL) gets max over columns of the L1 change per iteration; col_errors
+ * (host, may be NULL) the last iteration's per-column L1 (k values).
+ * opts->seed_vertex is ignored. */
+int gd_ppr_plan_run(gd_ppr_plan *plan, const int32_t *d_seeds, float *d_X,
+                    const gd_pr_options *opts, gd_pr_stats *stats, double *errors,
+                    double *col_errors);
+
+int gd_ppr_plan_info(const gd_ppr_plan *plan, gd_ppr_plan_info_t *info);
+
+int gd_ppr_plan_destroy(gd_ppr_plan *plan);
 
 /* ------------------------------------------------------------ SpMV */
 

@@ -16,7 +16,8 @@ def _key(ds):
     return ds.replace(".mtx", "").replace(".", "_")
 
 
-def _gpu_pr(device, in_csr, outdeg, tol, max_iter, seed_vertex=-1, init=None, chunk=0):
+def _gpu_pr(device, in_csr, outdeg, tol, max_iter, seed_vertex=-1, init=None, chunk=0,
+            relabel=None, ipt=0):
     import torch
 
     m = in_csr.m
@@ -26,7 +27,7 @@ def _gpu_pr(device, in_csr, outdeg, tol, max_iter, seed_vertex=-1, init=None, ch
     if init is None:
         init = np.full(m, np.float32(1.0) / np.float32(m), np.float32)
     scores = torch.from_numpy(np.array(init, np.float32)).to(device)
-    plan = PageRankPlan(off, col, deg)
+    plan = PageRankPlan(off, col, deg, relabel=relabel, items_per_thread=ipt)
     stats, errs = plan.run(scores, tol=tol, max_iter=max_iter, seed_vertex=seed_vertex,
                            chunk=chunk)
     out = scores.cpu().numpy()
@@ -97,10 +98,14 @@ def test_pr_first_iteration_short_rows_bit_exact(device):
     assert np.max(np.abs(s - ref) / ref) <= 1e-5
 
 
-@pytest.mark.parametrize("scale,tol", [(16, 1e-4), (18, 1e-4), (18, 1e-6), (20, 1e-4)])
-def test_pr_rmat_vs_oracle(device, scale, tol):
+@pytest.mark.parametrize("scale,tol,relabel,ipt", [
+    (16, 1e-4, False, 8), (16, 1e-4, True, 16), (18, 1e-4, True, 8), (18, 1e-6, False, 16),
+    (20, 1e-4, True, 8), (20, 1e-4, False, 16)])
+def test_pr_rmat_vs_oracle(device, scale, tol, relabel, ipt):
     out_csr, in_csr = graph.rmat(scale, 16, seed=1)
-    s, stats, errs, info = _gpu_pr(device, in_csr, out_csr.degree, tol, 100)
+    s, stats, errs, info = _gpu_pr(device, in_csr, out_csr.degree, tol, 100, relabel=relabel,
+                                   ipt=ipt)
+    assert info["relabel"] == int(relabel) and info["items_per_thread"] == ipt
     ref, it, ref_errs = oracle.pr(in_csr.m, in_csr.row_offsets, in_csr.column_indices,
                                   out_csr.degree, tol=tol, max_iter=100)
     if not stats["near_tie"]:
@@ -226,3 +231,29 @@ def test_pr_invalid_csr_rejected(device):
     off_bad = torch.tensor([0, 3, 2], dtype=torch.int32, device=device)
     with pytest.raises(GardeniaError):
         PageRankPlan(off_bad, torch.tensor([0, 1], dtype=torch.int32, device=device), deg)
+
+
+@pytest.mark.parametrize("ipt", [8, 16])
+def test_pr_relabel_is_bit_identical(device, ipt):
+    """Relabelling keeps every row's edges in order and the reductions depend
+    only on positions within a row, so scores are bit-identical."""
+    out_csr, in_csr = graph.rmat(17, 16, seed=6)
+    a, sa, ea, ia = _gpu_pr(device, in_csr, out_csr.degree, 1e-5, 60, relabel=False, ipt=ipt)
+    b, sb, eb, ib = _gpu_pr(device, in_csr, out_csr.degree, 1e-5, 60, relabel=True, ipt=ipt)
+    assert ia["relabel"] == 0 and ib["relabel"] == 1
+    np.testing.assert_array_equal(a, b)
+    assert sa["iterations"] == sb["iterations"]
+    np.testing.assert_allclose(ea, eb, rtol=1e-12)
+
+
+def test_pr_relabel_personalised_seed(device):
+    out_csr, in_csr = graph.rmat(14, 16, seed=8)
+    m = in_csr.m
+    seed = 12345
+    init = np.zeros(m, np.float32)
+    init[seed] = 1.0
+    a, sa, _, _ = _gpu_pr(device, in_csr, out_csr.degree, 1e-7, 200, seed_vertex=seed, init=init,
+                          relabel=False)
+    b, sb, _, _ = _gpu_pr(device, in_csr, out_csr.degree, 1e-7, 200, seed_vertex=seed, init=init,
+                          relabel=True)
+    np.testing.assert_array_equal(a, b)
