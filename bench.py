@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--relabel", choices=["auto", "on", "off"], default="auto")
     ap.add_argument("--ipt", type=int, default=0, help="edges per thread per tile (8/16, 0=auto)")
+    ap.add_argument("--nt", type=int, default=-1, help="non-temporal level 0/1/2 (-1 auto)")
+    ap.add_argument("--persistent", choices=["auto", "on", "off"], default="auto")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -102,7 +104,9 @@ def main():
     scores = torch.empty(V, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
     relabel = {"auto": None, "on": True, "off": False}[args.relabel]
-    plan = PageRankPlan(off, col, deg, stream=stream, relabel=relabel, items_per_thread=args.ipt)
+    plan = PageRankPlan(off, col, deg, stream=stream, relabel=relabel, items_per_thread=args.ipt,
+                        nontemporal=None if args.nt < 0 else args.nt,
+                        persistent={"auto": None, "on": True, "off": False}[args.persistent])
     info = plan.info()
     log(f"plan: {info}")
     init = float(np.float32(1.0) / np.float32(V))
@@ -150,6 +154,8 @@ def main():
                    "V": V, "E": E, "iterations_per_step": args.iters,
                    "abcd": [0.57, 0.19, 0.19, 0.05], "parallelism": "single-gpu",
                    "relabel": bool(info["relabel"]), "items_per_thread": info["items_per_thread"],
+                   "nontemporal": info["nontemporal"], "persistent": bool(info["persistent"]),
+                   "pull_grid": info["pull_grid"],
                    "plan_ms": round(info["plan_ms"], 1)},
         "roofline": roof, "cpu_baseline": cpu,
     }

@@ -72,7 +72,8 @@ class PrStats(ctypes.Structure):
 
 
 class PrPlanConfig(ctypes.Structure):
-    _fields_ = [("relabel", ctypes.c_int), ("items_per_thread", ctypes.c_int)]
+    _fields_ = [("relabel", ctypes.c_int), ("items_per_thread", ctypes.c_int),
+                ("nontemporal", ctypes.c_int), ("persistent", ctypes.c_int)]
 
 
 class PrPlanInfo(ctypes.Structure):
@@ -83,6 +84,9 @@ class PrPlanInfo(ctypes.Structure):
         ("max_row_len", ctypes.c_int32),
         ("relabel", ctypes.c_int32),
         ("items_per_thread", ctypes.c_int32),
+        ("nontemporal", ctypes.c_int32),
+        ("persistent", ctypes.c_int32),
+        ("pull_grid", ctypes.c_int32),
         ("plan_ms", ctypes.c_float),
     ]
 

@@ -1,0 +1,212 @@
+// iter_engine.h -- host side of the fixed-point pull iteration (pull_iter.h):
+// tile schedule, scratch, persistent grid size, and the chunked launch loop.
+#ifndef GD_ITER_ENGINE_H_
+#define GD_ITER_ENGINE_H_
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "gardenia_hip.h"
+#include "gd_common.h"
+#include "pull_iter.h"
+
+namespace gd {
+
+struct IterEngine {
+  int m = 0;
+  int64_t nnz = 0;
+  hipStream_t stream = nullptr;
+  int ipt = 8;               // edges per thread per tile (8 or 16)
+  int nt = 1;                // cache-hint level (see gd_pr_plan_config.nontemporal)
+  bool persistent = true;
+  int pull_grid = 0;
+  const int32_t* off = nullptr;
+  const int32_t* col = nullptr;
+  int e_max = 0;
+  DeviceBuffer dummy_col;
+  Schedule sched;
+  int n_tiles = 0, n_hubs = 0, fin_blocks = 1;
+  DeviceBuffer tiles, hub_row, hub_slot, hub_nseg, hub_part, err_part, fin_part, state, errors;
+  int errors_cap = 0;
+  std::vector<hipEvent_t> ev;
+
+  ~IterEngine() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+
+  // Schedule and scratch over a validated CSR already in HBM (offsets are
+  // copied to the host for the schedule).
+  int build(int m_, int64_t nnz_, const int32_t* d_off, const int32_t* d_col, hipStream_t s) {
+    m = m_;
+    nnz = nnz_;
+    stream = s;
+    off = d_off;
+    col = d_col;
+    int rc = GD_OK;
+    if (nnz == 0) {
+      rc = dummy_col.alloc(4);
+      if (rc) return rc;
+      GD_HIP_TRY(hipMemsetAsync(dummy_col.ptr, 0, 4, stream));
+      col = dummy_col.as<int32_t>();
+    }
+    e_max = (int)std::max<int64_t>(nnz - 1, 0);
+    std::vector<int32_t> h_off(m + 1);
+    GD_HIP_TRY(hipMemcpyAsync(h_off.data(), off, 4 * ((size_t)m + 1), hipMemcpyDeviceToHost,
+                              stream));
+    GD_HIP_TRY(hipStreamSynchronize(stream));
+    rc = build_schedule(m, nnz, h_off.data(), kBlock * ipt, kTileRows, &sched);
+    if (rc) return rc;
+    n_tiles = (int)sched.tiles.size();
+    n_hubs = (int)sched.hub_row.size();
+    rc = tiles.alloc(sizeof(Tile) * n_tiles);
+    if (!rc) rc = hub_row.alloc(4 * (size_t)std::max(1, n_hubs));
+    if (!rc) rc = hub_slot.alloc(4 * (size_t)std::max(1, n_hubs));
+    if (!rc) rc = hub_nseg.alloc(4 * (size_t)std::max(1, n_hubs));
+    if (!rc) rc = hub_part.alloc(4 * (size_t)std::max(1, sched.n_slots));
+    if (!rc) rc = err_part.alloc(8 * (size_t)std::max(1, n_tiles));
+    if (!rc) rc = fin_part.alloc(8 * (size_t)kFinBlocks);
+    if (!rc) rc = state.alloc(sizeof(IterState));
+    if (rc) return rc;
+    auto up = [&](DeviceBuffer& b, const void* src, size_t n) -> hipError_t {
+      return n ? hipMemcpyAsync(b.ptr, src, n, hipMemcpyHostToDevice, stream) : hipSuccess;
+    };
+    GD_HIP_TRY(up(tiles, sched.tiles.data(), sizeof(Tile) * n_tiles));
+    GD_HIP_TRY(up(hub_row, sched.hub_row.data(), 4 * (size_t)n_hubs));
+    GD_HIP_TRY(up(hub_slot, sched.hub_slot.data(), 4 * (size_t)n_hubs));
+    GD_HIP_TRY(up(hub_nseg, sched.hub_nseg.data(), 4 * (size_t)n_hubs));
+    GD_HIP_TRY(hipStreamSynchronize(stream));
+    return GD_OK;
+  }
+
+  template <class P>
+  int setup_grid() {
+    if (persistent) {
+      int dev = 0, cus = 0, per_cu = 0;
+      GD_HIP_TRY(hipGetDevice(&dev));
+      GD_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      const void* fn = (ipt == 16)
+                           ? reinterpret_cast<const void*>(&pull_persistent_kernel<P, 16, 1>)
+                           : reinterpret_cast<const void*>(&pull_persistent_kernel<P, 8, 1>);
+      GD_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0));
+      pull_grid = std::max(1, std::min(n_tiles, cus * std::max(1, per_cu)));
+    } else {
+      pull_grid = n_tiles;
+    }
+    fin_blocks = std::max(1, std::min(kFinBlocks, std::max(pull_grid, n_hubs) / kBlock));
+    return GD_OK;
+  }
+
+  IterArgs iter_args() {
+    return IterArgs{tiles.as<Tile>(), n_tiles, off, col, e_max, hub_part.as<float>(),
+                    err_part.as<double>(), state.as<IterState>()};
+  }
+
+  FinArgs fin_args() {
+    return FinArgs{hub_row.as<int32_t>(), hub_slot.as<int32_t>(), hub_nseg.as<int32_t>(), n_hubs,
+                   pull_grid, errors.as<double>(), fin_part.as<double>()};
+  }
+
+  template <class P, int IPT, int NT>
+  void launch_pull_t(const typename P::Args& a, const IterArgs& it) {
+    if (persistent)
+      hipLaunchKernelGGL((pull_persistent_kernel<P, IPT, NT>), dim3(pull_grid), dim3(kBlock), 0,
+                         stream, a, it);
+    else
+      hipLaunchKernelGGL((pull_tile_kernel<P, IPT, NT>), dim3(n_tiles), dim3(kBlock), 0, stream,
+                         a, it);
+  }
+
+  template <class P>
+  void launch_pull(const typename P::Args& a, const IterArgs& it) {
+    switch (ipt * 4 + nt) {
+      case 8 * 4 + 0: launch_pull_t<P, 8, 0>(a, it); break;
+      case 8 * 4 + 1: launch_pull_t<P, 8, 1>(a, it); break;
+      case 8 * 4 + 2: launch_pull_t<P, 8, 2>(a, it); break;
+      case 16 * 4 + 0: launch_pull_t<P, 16, 0>(a, it); break;
+      case 16 * 4 + 1: launch_pull_t<P, 16, 1>(a, it); break;
+      default: launch_pull_t<P, 16, 2>(a, it); break;
+    }
+  }
+
+  // Runs up to max_iter iterations (stopping after the first with L1 < tol).
+  // `make_args(odd)` is not needed: the kernels pick buffers by the device
+  // iteration parity.  Fills iterations / converged / final_error / pull
+  // timing / near_tie of *s (solve_ms is the caller's).
+  template <class P>
+  int run(const typename P::Args& a, int max_iter, double tol, int chunk_opt, bool profile,
+          gd_pr_stats* s, double* errors_out) {
+    if (errors_cap < std::max(1, max_iter)) {
+      int rc = errors.alloc(sizeof(double) * std::max(1, max_iter));
+      if (rc) return rc;
+      errors_cap = std::max(1, max_iter);
+    }
+    const int chunk = chunk_opt > 0 ? chunk_opt : (tol > 0 ? 8 : std::max(1, max_iter));
+    if (profile) {
+      while ((int)ev.size() < 2 * chunk) {
+        hipEvent_t e;
+        GD_HIP_TRY(hipEventCreate(&e));
+        ev.push_back(e);
+      }
+    }
+    const IterArgs it = iter_args();
+    const FinArgs f = fin_args();
+    hipLaunchKernelGGL(iter_state_init_kernel, dim3(1), dim3(1), 0, stream, it.st, max_iter, tol);
+    GD_HIP_TRY(hipGetLastError());
+    int launched = 0;
+    IterState h{};
+    float pull_ms = 0.0f;
+    int pull_n = 0;
+    while (launched < max_iter) {
+      const int n = std::min(chunk, max_iter - launched);
+      for (int k = 0; k < n; k++) {
+        if (profile) GD_HIP_TRY(hipEventRecord(ev[2 * k], stream));
+        launch_pull<P>(a, it);
+        if (profile) GD_HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
+        hipLaunchKernelGGL(finalize_kernel<P>, dim3(fin_blocks), dim3(kBlock), 0, stream, a, it,
+                           f);
+      }
+      GD_HIP_TRY(hipGetLastError());
+      launched += n;
+      GD_HIP_TRY(hipMemcpyAsync(&h, it.st, sizeof(IterState), hipMemcpyDeviceToHost, stream));
+      GD_HIP_TRY(hipStreamSynchronize(stream));
+      if (profile) {
+        // only launches that did work count; later ones in the chunk exit at once
+        const int worked = std::min(n, h.iter - (launched - n));
+        for (int k = 0; k < worked; k++) {
+          float ms = 0.0f;
+          GD_HIP_TRY(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+          pull_ms += ms;
+          pull_n++;
+        }
+      }
+      if (h.done) break;
+    }
+    s->iterations = h.iter;
+    s->pull_ms = pull_ms;
+    s->pull_launches = pull_n;
+    std::vector<double> errs(std::max(1, h.iter));
+    if (h.iter > 0) {
+      GD_HIP_TRY(hipMemcpy(errs.data(), errors.ptr, sizeof(double) * h.iter,
+                           hipMemcpyDeviceToHost));
+      s->final_error = errs[h.iter - 1];
+      s->converged = s->final_error < tol;
+      // Two correct FP32 evaluations of an iteration differ in L1 by up to about
+      // one ulp per vertex, ~2^-22 * sum(scores) (<= 1 for scores started from a
+      // distribution); flag a tie within that band.
+      const double band = 1e-5 * tol + 2.4e-7;
+      for (int i = std::max(0, h.iter - 2); i < h.iter; i++)
+        if (tol > 0 && std::fabs(errs[i] - tol) <= band) s->near_tie = 1;
+      if (errors_out)
+        for (int i = 0; i < h.iter; i++) errors_out[i] = errs[i];
+    }
+    return GD_OK;
+  }
+};
+
+}  // namespace gd
+
+#endif  // GD_ITER_ENGINE_H_

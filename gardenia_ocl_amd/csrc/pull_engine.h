@@ -32,6 +32,25 @@ constexpr int kRowsPerThread = kTileRows / kBlock;
 constexpr int kOffLoads = (kTileRows + 1 + kBlock - 1) / kBlock;
 constexpr int kSeqRow = 64;
 
+// Loads / stores of arrays streamed once per launch.  With NT the accesses
+// carry the non-temporal hint, so the 1 GB-per-iteration index stream does not
+// evict the cache-resident part of the gathered vector.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
+template <bool NT, typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
@@ -75,12 +94,13 @@ __device__ __forceinline__ float seq_sum(float s, const float* __restrict__ v, i
 
 // Loads the tile's edge indices col[e0 + tid + j*kBlock] (clamped into the
 // tile so all IPT loads issue back to back).  Requires ne >= 1.
-template <int IPT>
+template <int IPT, bool NT>
 __device__ __forceinline__ void load_indices(const int32_t* __restrict__ col, int e0, int ne,
                                              int (&idx)[IPT]) {
   const int last = ne - 1;
 #pragma unroll
-  for (int j = 0; j < IPT; j++) idx[j] = col[e0 + min((int)threadIdx.x + j * kBlock, last)];
+  for (int j = 0; j < IPT; j++)
+    idx[j] = ld_stream<NT>(col + e0 + min((int)threadIdx.x + j * kBlock, last));
 }
 
 template <int IPT, class G>
@@ -93,12 +113,12 @@ __device__ __forceinline__ void gather_values(const G& g, int e0, int ne, const 
 
 // Sum of the gathered values of a hub segment [e0, e0+ne), ne >= 1: per-thread
 // partial in j order, wave butterfly, then waves in order.  Valid in thread 0.
-template <int IPT, class G>
+template <int IPT, bool NT, class G>
 __device__ __forceinline__ float segment_sum(const int32_t* __restrict__ col, int e0, int ne,
                                              const G& g, float* s_tmp) {
   int idx[IPT];
   float v[IPT];
-  load_indices<IPT>(col, e0, ne, idx);
+  load_indices<IPT, NT>(col, e0, ne, idx);
   gather_values<IPT>(g, e0, ne, idx, v);
   float s = 0.0f;
 #pragma unroll
@@ -114,42 +134,46 @@ struct RowTileLds {
   float longsum[kTileRows];   // wavefront sums of rows longer than kSeqRow
 };
 
-// One row tile: rows [r0, r1) whose edges are [e0, e0+ne).  The epilogue E
-// provides
-//   typename E::Pre          per-row operands, loaded before the gather
-//   Pre pre(int row)         (clamped row: always a valid index)
-//   float init(const Pre&)   value the row sum starts from
-//   double finish(int row, float sum, const Pre&)   writes the row, returns
-//                            its contribution to the tile's double accumulator
-template <int IPT, class G, class E>
-__device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
-                                           const int32_t* __restrict__ col, int r0, int r1,
-                                           int e0, int ne, const G& g, const E& e,
-                                           RowTileLds<IPT>& lds) {
-  const int tid = threadIdx.x;
-  const int nr = r1 - r0;
-  // --- phase A: every independent load of the tile
+// Registers holding the loads of one tile that do not depend on a gather:
+// its edge indices, and for a row tile its row offsets and each owned row's
+// epilogue operands.  Issued as early as possible (see pr_pull_persistent).
+template <int IPT, class Pre>
+struct TileRegs {
   int idx[IPT];
-  if (ne > 0) load_indices<IPT>(col, e0, ne, idx);
   int offv[kOffLoads];
+  Pre pre[kRowsPerThread];
+};
+
+// Phase A: issue every gather-independent load of tile t, without branches
+// (a branch here would make the compiler's wait counting conservative and
+// serialize the prefetch with the gathers).  Addresses of idle lanes are
+// clamped: an empty tile re-reads its first edge, a hub segment reads the
+// offsets and operands of its own row.
+// `e_max` is the last valid index of `col` (the arrays hold >= 1 element).
+template <int IPT, bool NT, class E>
+__device__ __forceinline__ void issue_tile_loads(const Tile& t, const int32_t* __restrict__ off,
+                                                 const int32_t* __restrict__ col, int e_max,
+                                                 const E& e, TileRegs<IPT, typename E::Pre>& R) {
+  const int tid = threadIdx.x;
+  const int last = max(t.w - t.z - 1, 0);
 #pragma unroll
-  for (int q = 0; q < kOffLoads; q++) offv[q] = off[r0 + min(tid + q * kBlock, nr)];
-  typename E::Pre pre[kRowsPerThread];
-#pragma unroll
-  for (int q = 0; q < kRowsPerThread; q++) pre[q] = e.pre(r0 + min(tid + q * kBlock, nr - 1));
-  // --- phase B: gathers (depend on the indices only)
-  float v[IPT];
-  if (ne > 0) gather_values<IPT>(g, e0, ne, idx, v);
+  for (int j = 0; j < IPT; j++)
+    R.idx[j] = ld_stream<NT>(col + min(t.z + min(tid + j * kBlock, last), e_max));
+  const int nr = (t.y >= 0) ? t.y - t.x : 1;
 #pragma unroll
   for (int q = 0; q < kOffLoads; q++)
-    if (tid + q * kBlock <= nr) lds.off[tid + q * kBlock] = offv[q] - e0;
-  if (ne > 0) {
+    R.offv[q] = ld_stream<NT>(off + t.x + min(tid + q * kBlock, nr));
 #pragma unroll
-    for (int j = 0; j < IPT; j++)
-      if (tid + j * kBlock < ne) lds.val[tid + j * kBlock] = v[j];
-  }
-  __syncthreads();
-  // --- phase C: short rows by their owner thread, in CSR order
+  for (int q = 0; q < kRowsPerThread; q++) R.pre[q] = e.pre(t.x + min(tid + q * kBlock, nr - 1));
+}
+
+// Phases C-E of a row tile whose gathered values are already in lds.val and
+// rebased offsets in lds.off (after a barrier): reduce rows, epilogue.
+template <int IPT, class E>
+__device__ __forceinline__ double reduce_row_tile(int r0, int nr, const E& e,
+                                                  const typename E::Pre (&pre)[kRowsPerThread],
+                                                  RowTileLds<IPT>& lds) {
+  const int tid = threadIdx.x;
   float sum[kRowsPerThread];
   bool have_long = false;
 #pragma unroll
@@ -164,7 +188,6 @@ __device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
         have_long = true;
     }
   }
-  // --- phase D: long rows, one wavefront each; sums to LDS
   if (__syncthreads_or(have_long)) {
     const int lane = tid & (kWave - 1), wid = tid / kWave;
     for (int c = wid * kWave; c < nr; c += kBlock) {
@@ -197,7 +220,6 @@ __device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
       if (i < nr && lds.off[i + 1] - lds.off[i] > kSeqRow) sum[q] = e.init(pre[q]) + lds.longsum[i];
     }
   }
-  // --- phase E: epilogue by the owner thread (coalesced)
   double acc = 0.0;
 #pragma unroll
   for (int q = 0; q < kRowsPerThread; q++) {
@@ -205,6 +227,42 @@ __device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
     if (i < nr) acc += e.finish(r0 + i, sum[q], pre[q]);
   }
   return acc;
+}
+
+// Stage a row tile's gathered values and rebased offsets into LDS.
+template <int IPT, class Pre>
+__device__ __forceinline__ void stage_row_tile(const Tile& t, const TileRegs<IPT, Pre>& R,
+                                               const float (&v)[IPT], RowTileLds<IPT>& lds) {
+  const int tid = threadIdx.x;
+  const int nr = t.y - t.x, ne = t.w - t.z;
+#pragma unroll
+  for (int q = 0; q < kOffLoads; q++)
+    if (tid + q * kBlock <= nr) lds.off[tid + q * kBlock] = R.offv[q] - t.z;
+#pragma unroll
+  for (int j = 0; j < IPT; j++)
+    if (tid + j * kBlock < ne) lds.val[tid + j * kBlock] = v[j];
+}
+
+// One row tile: rows [r0, r1) whose edges are [e0, e0+ne).  The epilogue E
+// provides
+//   typename E::Pre          per-row operands, loaded before the gather
+//   Pre pre(int row)         (clamped row: always a valid index)
+//   float init(const Pre&)   value the row sum starts from
+//   double finish(int row, float sum, const Pre&)   writes the row, returns
+//                            its contribution to the tile's double accumulator
+template <int IPT, bool NT, class G, class E>
+__device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
+                                           const int32_t* __restrict__ col, int e_max, int r0,
+                                           int r1, int e0, int ne, const G& g, const E& e,
+                                           RowTileLds<IPT>& lds) {
+  const Tile t{r0, r1, e0, e0 + ne};
+  TileRegs<IPT, typename E::Pre> R;
+  issue_tile_loads<IPT, NT>(t, off, col, e_max, e, R);
+  float v[IPT];
+  if (ne > 0) gather_values<IPT>(g, e0, ne, R.idx, v);
+  stage_row_tile<IPT>(t, R, v, lds);
+  __syncthreads();
+  return reduce_row_tile<IPT>(r0, r1 - r0, e, R.pre, lds);
 }
 
 }  // namespace gd

@@ -1,0 +1,198 @@
+// pull_iter.h -- device side of the fixed-point pull iteration, generic over a
+// policy (plain / personalised PageRank, weighted PageRank, its adjoint).
+//
+// One iteration = one pull launch over the tile schedule (gathers, per-row
+// epilogue, per-workgroup L1 partial) + one finalize launch (hub rows from
+// their segment partials, deterministic L1 reduction, convergence flag).  The
+// iteration counter and the done flag live on the device (IterState), so every
+// iteration's launches are identical; launches after `done` return at once.
+//
+// A policy P provides
+//   struct P::Args                      kernel arguments (pointers, scalars)
+//   template <int NT> static G gather(const Args&, int odd)
+//   template <int NT> static E epilogue(const Args&, int odd)
+// with G / E as described in pull_engine.h (value(e, u); Pre, pre, init,
+// finish).  `odd` is the parity of the iteration: the gathered vector is
+// double-buffered.
+#ifndef GD_PULL_ITER_H_
+#define GD_PULL_ITER_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gd_common.h"
+#include "pull_engine.h"
+
+namespace gd {
+
+constexpr int kFinBlocks = 128;  // finalize workgroups (<= kBlock: combined by one block)
+
+struct IterState {
+  int iter;         // iterations completed
+  int done;         // set once converged or max_iter reached
+  int max_iter;
+  unsigned ticket;  // finalize workgroups arrived in this iteration
+  double tol;
+};
+
+struct IterArgs {
+  const Tile* __restrict__ tiles;
+  int n_tiles;
+  const int32_t* __restrict__ off;
+  const int32_t* __restrict__ col;
+  int e_max;  // last valid index of col (a 1-element dummy when nnz == 0)
+  float* __restrict__ hub_part;
+  double* __restrict__ err_part;
+  IterState* st;
+};
+
+struct FinArgs {
+  const int32_t* __restrict__ hub_row;
+  const int32_t* __restrict__ hub_slot;
+  const int32_t* __restrict__ hub_nseg;
+  int n_hubs;
+  int n_part;  // per-workgroup L1 partials written by the pull launch
+  double* __restrict__ errors;
+  double* __restrict__ fin_part;
+};
+
+static __global__ void iter_state_init_kernel(IterState* st, int max_iter, double tol) {
+  st->iter = 0;
+  st->done = max_iter <= 0;
+  st->max_iter = max_iter;
+  st->ticket = 0;
+  st->tol = tol;
+}
+
+// One workgroup per tile.
+template <class P, int IPT, int NT>
+__global__ __launch_bounds__(kBlock) void pull_tile_kernel(const typename P::Args a,
+                                                           const IterArgs it) {
+  if (it.st->done) return;
+  const int odd = it.st->iter & 1;
+  __shared__ RowTileLds<IPT> lds;
+  __shared__ double s_err[kWaves];
+  const Tile t = it.tiles[blockIdx.x];
+  const auto g = P::template gather<NT>(a, odd);
+  double err = 0.0;
+  if (t.y >= 0) {
+    const auto e = P::template epilogue<NT>(a, odd);
+    err = row_tile<IPT, (NT > 0)>(it.off, it.col, it.e_max, t.x, t.y, t.z, t.w - t.z, g, e, lds);
+  } else {
+    const float p = segment_sum<IPT, (NT > 0)>(it.col, t.z, t.w - t.z, g, lds.val);
+    if (threadIdx.x == 0) it.hub_part[-t.y - 1] = p;
+  }
+  const double tot = block_sum<kWaves>(err, s_err);
+  if (threadIdx.x == 0) it.err_part[blockIdx.x] = tot;
+}
+
+// Persistent, software-pipelined form: workgroup b walks tiles b, b+G, ...;
+// the next tile's index / offset / epilogue loads are issued right after the
+// current tile's gathers, so their latency overlaps the current tile's LDS
+// reduction and epilogue (vector loads return in issue order, so waiting for
+// the gathers never waits for the prefetch).  Same arithmetic and per-row
+// order as pull_tile_kernel.
+template <class P, int IPT, int NT>
+__global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename P::Args a,
+                                                                 const IterArgs it) {
+  if (it.st->done) return;
+  const int odd = it.st->iter & 1;
+  __shared__ RowTileLds<IPT> lds;
+  __shared__ double s_err[kWaves];
+  const auto g = P::template gather<NT>(a, odd);
+  const auto e = P::template epilogue<NT>(a, odd);
+  using E = decltype(e);
+  using Regs = TileRegs<IPT, typename E::Pre>;
+  double err = 0.0;
+  int t = blockIdx.x;
+  if (t < it.n_tiles) {
+    Tile cur = it.tiles[t];
+    Regs R;
+    issue_tile_loads<IPT, (NT > 0)>(cur, it.off, it.col, it.e_max, e, R);
+    for (;;) {
+      const int tn = t + gridDim.x;
+      const bool has_next = tn < it.n_tiles;
+      const Tile nxt = it.tiles[has_next ? tn : t];
+      const int ne = cur.w - cur.z;
+      float v[IPT];
+      if (ne > 0) gather_values<IPT>(g, cur.z, ne, R.idx, v);
+      Regs Rn;
+      // unconditional (clamped) so the compiler's wait counting stays exact
+      issue_tile_loads<IPT, (NT > 0)>(nxt, it.off, it.col, it.e_max, e, Rn);
+      if (cur.y >= 0) {
+        stage_row_tile<IPT>(cur, R, v, lds);
+        __syncthreads();
+        err += reduce_row_tile<IPT>(cur.x, cur.y - cur.x, e, R.pre, lds);
+      } else {
+        float sacc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < IPT; j++) sacc += ((int)threadIdx.x + j * kBlock < ne) ? v[j] : 0.0f;
+        const float p = block_sum<kWaves>(sacc, lds.val);
+        if (threadIdx.x == 0) it.hub_part[-cur.y - 1] = p;
+      }
+      __syncthreads();  // LDS is reused by the next tile
+      if (!has_next) break;
+      cur = nxt;
+      R = Rn;
+      t = tn;
+    }
+  }
+  const double tot = block_sum<kWaves>(err, s_err);
+  if (threadIdx.x == 0) it.err_part[blockIdx.x] = tot;
+}
+
+// Hub-row epilogues (segment partials summed in segment order) and the L1
+// reduction: each workgroup sums a contiguous slice of the per-workgroup
+// partials, the last workgroup to arrive adds the workgroup sums in index
+// order, so the result does not depend on scheduling.
+template <class P>
+__global__ __launch_bounds__(kBlock) void finalize_kernel(const typename P::Args a,
+                                                          const IterArgs it, const FinArgs f) {
+  IterState* st = it.st;
+  if (st->done) return;
+  __shared__ double s_err[kWaves];
+  __shared__ int s_last;
+  const int odd = st->iter & 1;
+  const auto e = P::template epilogue<0>(a, odd);
+  double err = 0.0;
+  for (int h = blockIdx.x * kBlock + threadIdx.x; h < f.n_hubs; h += gridDim.x * kBlock) {
+    const int v = f.hub_row[h];
+    const auto p = e.pre(v);
+    const float* part = it.hub_part + f.hub_slot[h];
+    float s = e.init(p);
+    for (int k = 0; k < f.hub_nseg[h]; k++) s += part[k];
+    err += e.finish(v, s, p);
+  }
+  const int per = (f.n_part + gridDim.x - 1) / gridDim.x;
+  const int lo = blockIdx.x * per, hi = min(lo + per, f.n_part);
+  for (int i = lo + threadIdx.x; i < hi; i += 4 * kBlock) {
+    const double a0 = it.err_part[i];
+    const double a1 = (i + kBlock < hi) ? it.err_part[i + kBlock] : 0.0;
+    const double a2 = (i + 2 * kBlock < hi) ? it.err_part[i + 2 * kBlock] : 0.0;
+    const double a3 = (i + 3 * kBlock < hi) ? it.err_part[i + 3 * kBlock] : 0.0;
+    err += ((a0 + a1) + a2) + a3;
+  }
+  const double tot = block_sum<kWaves>(err, s_err);
+  if (threadIdx.x == 0) {
+    f.fin_part[blockIdx.x] = tot;
+    __threadfence();  // release fin_part before taking a ticket
+    const unsigned arrived = atomicAdd(&st->ticket, 1u);
+    s_last = (arrived == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();  // acquire: every other workgroup's fin_part is visible
+  const double v = (threadIdx.x < gridDim.x) ? f.fin_part[threadIdx.x] : 0.0;
+  const double total = block_sum<kWaves>(v, s_err);
+  if (threadIdx.x == 0) {
+    const int i = st->iter;
+    f.errors[i] = total;
+    st->iter = i + 1;
+    st->ticket = 0;
+    if (total < st->tol || i + 1 >= st->max_iter) st->done = 1;
+  }
+}
+
+}  // namespace gd
+
+#endif  // GD_PULL_ITER_H_

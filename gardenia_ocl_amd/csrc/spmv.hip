@@ -41,16 +41,16 @@ struct SpEpilogue {
 
 __global__ __launch_bounds__(kBlock) void spmv_pull_kernel(
     const Tile* __restrict__ tiles, const int32_t* __restrict__ Ap,
-    const int32_t* __restrict__ Aj, const float* __restrict__ Ax, const float* __restrict__ x,
-    float* __restrict__ y, float* __restrict__ hub_part) {
+    const int32_t* __restrict__ Aj, int e_max, const float* __restrict__ Ax,
+    const float* __restrict__ x, float* __restrict__ y, float* __restrict__ hub_part) {
   __shared__ RowTileLds<kSpIPT> lds;
   const Tile t = tiles[blockIdx.x];
   const SpGather g{x, Ax};
   if (t.y >= 0) {
     const SpEpilogue e{y};
-    (void)row_tile<kSpIPT>(Ap, Aj, t.x, t.y, t.z, t.w - t.z, g, e, lds);
+    (void)row_tile<kSpIPT, false>(Ap, Aj, e_max, t.x, t.y, t.z, t.w - t.z, g, e, lds);
   } else {
-    const float p = segment_sum<kSpIPT>(Aj, t.z, t.w - t.z, g, lds.val);
+    const float p = segment_sum<kSpIPT, false>(Aj, t.z, t.w - t.z, g, lds.val);
     if (threadIdx.x == 0) hub_part[-t.y - 1] = p;
   }
 }
@@ -83,6 +83,8 @@ struct gd_spmv_plan {
   Schedule sched;
   int n_tiles = 0, n_hubs = 0;
   DeviceBuffer tiles, hub_row, hub_slot, hub_nseg, hub_part;
+  DeviceBuffer dummy_col;  // 1 element, used as Aj when nnz == 0
+  int e_max = 0;
 };
 
 extern "C" {
@@ -118,6 +120,12 @@ int gd_spmv_plan_create(int rows, int cols, int64_t nnz, const int32_t* d_Ap, co
     }
     if (!rc) rc = build_schedule(rows, nnz, h_off.data(), kSpTileNnz, kTileRows, &p->sched);
     if (!rc) rc = check_columns(d_Aj, nnz, cols, p->stream);
+    if (!rc && nnz == 0) {
+      rc = p->dummy_col.alloc(4);
+      if (!rc && hipMemsetAsync(p->dummy_col.ptr, 0, 4, p->stream) != hipSuccess) rc = GD_ERR_DEVICE;
+      p->Aj = p->dummy_col.as<int32_t>();
+    }
+    p->e_max = (int)std::max<int64_t>(nnz - 1, 0);
     if (!rc) {
       p->n_tiles = (int)p->sched.tiles.size();
       p->n_hubs = (int)p->sched.hub_row.size();
@@ -160,7 +168,8 @@ int gd_spmv_plan_run(gd_spmv_plan* p, const float* d_x, float* d_y) {
   if (p->rows == 0) return GD_OK;
   GD_CHECK_ARG(d_y != nullptr && (p->nnz == 0 || d_x != nullptr), "NULL x / y");
   hipLaunchKernelGGL(spmv_pull_kernel, dim3(p->n_tiles), dim3(kBlock), 0, p->stream,
-                     p->tiles.as<Tile>(), p->Ap, p->Aj, p->Ax, d_x, d_y, p->hub_part.as<float>());
+                     p->tiles.as<Tile>(), p->Ap, p->Aj, p->e_max, p->Ax, d_x, d_y,
+                     p->hub_part.as<float>());
   if (p->n_hubs)
     hipLaunchKernelGGL(spmv_hub_kernel, dim3((p->n_hubs + kBlock - 1) / kBlock), dim3(kBlock), 0,
                        p->stream, p->hub_row.as<int32_t>(), p->hub_slot.as<int32_t>(),

@@ -85,8 +85,8 @@ class PageRankPlan:
     """
 
     def __init__(self, in_row_offsets, in_column_indices, out_degree, stream=None, *,
-                 relabel=None, items_per_thread=0):
-        """relabel: None = library default (auto), True / False force it."""
+                 relabel=None, items_per_thread=0, nontemporal=None, persistent=None):
+        """relabel / nontemporal / persistent: None = library default, True / False force it."""
         import torch
 
         for t in (in_row_offsets, in_column_indices, out_degree):
@@ -103,6 +103,10 @@ class PageRankPlan:
         if relabel is not None:
             cfg.relabel = 1 if relabel else 0
         cfg.items_per_thread = int(items_per_thread)
+        if nontemporal is not None:
+            cfg.nontemporal = int(nontemporal)
+        if persistent is not None:
+            cfg.persistent = 1 if persistent else 0
         h = ctypes.c_void_p()
         N.call("gd_pr_plan_create_ex", self.m, self.nnz, in_row_offsets.data_ptr(),
                in_column_indices.data_ptr(), out_degree.data_ptr(),

@@ -64,7 +64,15 @@ typedef struct gd_pr_plan_config {
                            (hot sources become an L2-resident prefix of contrib; rows keep
                            their edge order; scores permuted in/out per run), 0: never,
                            -1: auto (when nnz >= 2^20) */
-  int items_per_thread; /* edges per thread per tile: 8 (2048-edge tiles) or 16; 0: auto */
+  int items_per_thread; /* edges per thread per tile: 8 (2048-edge tiles) or 16 (4096);
+                           0: auto (16) */
+  int nontemporal;      /* 1: streamed arrays (indices, offsets, scores, out-degree) use
+                           non-temporal loads/stores so they do not evict the gathered
+                           vector from L2; 2: the new contrib vector is written
+                           non-temporally too; 0: plain; -1: auto (0) */
+  int persistent;       /* 1: a persistent grid walks the tiles and prefetches the next
+                           tile's loads during the current tile's reduction; 0: one
+                           workgroup per tile; -1: auto (0) */
 } gd_pr_plan_config;
 
 void gd_pr_default_plan_config(gd_pr_plan_config *cfg);
@@ -76,6 +84,9 @@ typedef struct gd_pr_plan_info_t {
   int32_t max_row_len;
   int32_t relabel;        /* 1 if the plan relabelled the vertices */
   int32_t items_per_thread;
+  int32_t nontemporal;
+  int32_t persistent;
+  int32_t pull_grid;      /* workgroups per pull launch */
   float plan_ms;          /* plan construction time (validation, relabel, schedule) */
 } gd_pr_plan_info_t;
 

@@ -17,7 +17,7 @@ def _key(ds):
 
 
 def _gpu_pr(device, in_csr, outdeg, tol, max_iter, seed_vertex=-1, init=None, chunk=0,
-            relabel=None, ipt=0):
+            relabel=None, ipt=0, persistent=None):
     import torch
 
     m = in_csr.m
@@ -27,7 +27,8 @@ def _gpu_pr(device, in_csr, outdeg, tol, max_iter, seed_vertex=-1, init=None, ch
     if init is None:
         init = np.full(m, np.float32(1.0) / np.float32(m), np.float32)
     scores = torch.from_numpy(np.array(init, np.float32)).to(device)
-    plan = PageRankPlan(off, col, deg, relabel=relabel, items_per_thread=ipt)
+    plan = PageRankPlan(off, col, deg, relabel=relabel, items_per_thread=ipt,
+                        persistent=persistent)
     stats, errs = plan.run(scores, tol=tol, max_iter=max_iter, seed_vertex=seed_vertex,
                            chunk=chunk)
     out = scores.cpu().numpy()
@@ -257,3 +258,58 @@ def test_pr_relabel_personalised_seed(device):
     b, sb, _, _ = _gpu_pr(device, in_csr, out_csr.degree, 1e-7, 200, seed_vertex=seed, init=init,
                           relabel=True)
     np.testing.assert_array_equal(a, b)
+
+
+def test_pr_kernel_variants_bit_identical(device):
+    """Persistent/pipelined vs one-workgroup-per-tile, and every cache-hint
+    level, run the same arithmetic in the same order."""
+    import torch
+
+    out_csr, in_csr = graph.rmat(16, 16, seed=11)
+    m = in_csr.m
+    off = torch.from_numpy(in_csr.row_offsets).to(device)
+    col = torch.from_numpy(in_csr.column_indices).to(device)
+    deg = torch.from_numpy(out_csr.degree).to(device)
+    results = []
+    for persistent in (False, True):
+        for nt in (0, 1, 2):
+            for ipt in (8, 16):
+                plan = PageRankPlan(off, col, deg, relabel=True, items_per_thread=ipt,
+                                    nontemporal=nt, persistent=persistent)
+                inf = plan.info()
+                assert inf["persistent"] == int(persistent) and inf["nontemporal"] == nt
+                s = torch.full((m,), 1.0 / m, dtype=torch.float32, device=device)
+                st, errs = plan.run(s, tol=1e-5, max_iter=50)
+                results.append((ipt, s.cpu().numpy(), st["iterations"]))
+                plan.close()
+    for ipt in (8, 16):
+        rs = [r for r in results if r[0] == ipt]
+        for r in rs[1:]:
+            np.testing.assert_array_equal(r[1], rs[0][1])
+            assert r[2] == rs[0][2]
+
+
+def test_pr_persistent_tiny_and_empty_tail(device):
+    """Graphs whose last tiles have no edges (the prefetch clamps to the last
+    valid edge) and graphs with no edges at all."""
+    import torch
+
+    m = 5000
+    src = np.arange(0, 100, dtype=np.int32)
+    dst = (src + 1).astype(np.int32)
+    out_csr = graph.csr_from_edges(m, src, dst)
+    in_csr = graph.csr_from_edges(m, src, dst, transpose=True)
+    ref, it, _ = oracle.pr(m, in_csr.row_offsets, in_csr.column_indices, out_csr.degree,
+                           tol=1e-6, max_iter=100)
+    for persistent in (True, False):
+        s, stats, errs, info = _gpu_pr(device, in_csr, out_csr.degree, 1e-6, 100,
+                                       persistent=persistent)
+        assert info["persistent"] == int(persistent)
+        np.testing.assert_array_equal(s, ref)
+        assert stats["iterations"] == it
+    empty = graph.csr_from_edges(m, np.zeros(0, np.int32), np.zeros(0, np.int32))
+    for persistent in (True, False):
+        s, stats, errs, _ = _gpu_pr(device, empty, np.zeros(m, np.int32), 1e-3, 30,
+                                    persistent=persistent)
+        base = (np.float32(1.0) - np.float32(0.85)) / np.float32(m)  # (1.0f - kDamp) / m
+        np.testing.assert_array_equal(s, np.full(m, base, np.float32))
