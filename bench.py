@@ -56,6 +56,100 @@ def cpu_baseline(in_csr, out_deg, iters_per_pr, target_s):
     }
 
 
+def bench_spmv(log2_rows, nnz_per_row, reps, seed):
+    """ER 2^n x 2^n, nnz_per_row uniform columns, y += A x repeated `reps` times
+    (HIP events around the reps on the plan's stream)."""
+    import torch
+
+    from gardenia_ocl_amd import SpmvPlan, graph
+
+    dev = torch.device("cuda:0")
+    er = graph.er_matrix(log2_rows, nnz_per_row, seed=seed)
+    Ap = torch.from_numpy(er.row_offsets).to(dev)
+    Aj = torch.from_numpy(er.column_indices).to(dev)
+    Ax = torch.from_numpy(er.weight).to(dev)
+    x = torch.from_numpy(graph.uniform_pm1(er.m, seed + 1)).to(dev)
+    y = torch.zeros(er.m, dtype=torch.float32, device=dev)
+    plan = SpmvPlan(Ap, Aj, Ax)
+    for _ in range(3):
+        plan.run(x, y)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(reps):
+        plan.run(x, y)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nnz, M = er.nnz, er.m
+    byts = 8 * nnz + 16 * M + 4
+    return {"workload": f"ER 2^{log2_rows} x 2^{log2_rows}, {nnz_per_row} nnz/row, y += A x",
+            "nnz": nnz, "reps": reps, "ms_per_spmv": round(ms, 5),
+            "gflops": round(2 * nnz / (ms * 1e-3) / 1e9, 2),
+            "achieved_gbs": round(byts / (ms * 1e-3) / 1e9, 1),
+            "frac_of_hbm_peak": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes": byts}
+
+
+def bench_ppr(scale, k, tol, seed, fp64_seeds):
+    """Batched PPR (k seeds in one SpMM-style pull) vs k looped single-seed
+    runs of the plain pull kernel, both to the same per-seed L1 tolerance."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from gardenia_ocl_amd import PageRankPlan, PersonalizedPageRankPlan, graph
+
+    dev = torch.device("cuda:0")
+    _, ic = graph.rmat(scale, 16, seed=seed, want_out=False)
+    V = ic.m
+    od = np.bincount(ic.column_indices, minlength=V).astype(np.int32)
+    seeds = np.random.default_rng(seed + 17).integers(0, V, k).astype(np.int32)
+    off = torch.from_numpy(ic.row_offsets).to(dev)
+    col = torch.from_numpy(ic.column_indices).to(dev)
+    deg = torch.from_numpy(od).to(dev)
+    tseeds = torch.from_numpy(seeds).to(dev)
+    bp = PersonalizedPageRankPlan(off, col, deg, k)
+    X = bp.one_hot(seeds)
+    bp.run(tseeds, X, tol=tol, max_iter=2)  # warm-up
+    X = bp.one_hot(seeds)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bst, _, _ = bp.run(tseeds, X, tol=tol, max_iter=300)
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    pp = PageRankPlan(off, col, deg)
+    s = torch.zeros(V, dtype=torch.float32, device=dev)
+    pp.run(s, tol=tol, max_iter=2, seed_vertex=int(seeds[0]))  # warm-up
+    looped = []
+    its = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j in range(k):
+        s.zero_()
+        s[int(seeds[j])] = 1.0
+        st, _ = pp.run(s, tol=tol, max_iter=300, seed_vertex=int(seeds[j]))
+        its += st["iterations"]
+        if j < fp64_seeds:
+            looped.append(s.cpu().numpy())
+    torch.cuda.synchronize()
+    t_looped = time.perf_counter() - t0
+    # L1 against the FP64 personalised solution for the first seeds
+    Xh = X[:, :fp64_seeds].cpu().numpy().astype(np.float64)
+    X64, _ = oracle.ppr_fp64(V, ic.row_offsets, ic.column_indices, od, seeds[:fp64_seeds],
+                             tol=1e-12, max_iter=400)
+    l1_b = [float(np.abs(Xh[:, j] - X64[:, j]).sum()) for j in range(fp64_seeds)]
+    l1_l = [float(np.abs(looped[j].astype(np.float64) - X64[:, j]).sum())
+            for j in range(fp64_seeds)]
+    return {"workload": f"RMAT-{scale} ({ic.nnz} edges), {k} one-hot seeds, per-seed L1 tol {tol}",
+            "batched_s": round(t_batched, 4), "looped_s": round(t_looped, 4),
+            "speedup": round(t_looped / t_batched, 2), "batched_iterations": bst["iterations"],
+            "looped_iterations_total": its, "k_chunk": bp.info()["k_chunk"],
+            "l1_vs_fp64_batched": l1_b, "l1_vs_fp64_looped": l1_l,
+            "fp64_seeds_checked": fp64_seeds}
+
+
 def traffic_from_profile(scale):
     path = os.path.join(ROOT, "profiles", f"pr_pull_rmat{scale}_traffic.json")
     if not os.path.exists(path):
@@ -79,6 +173,12 @@ def main():
     ap.add_argument("--ipt", type=int, default=0, help="edges per thread per tile (8/16, 0=auto)")
     ap.add_argument("--nt", type=int, default=-1, help="non-temporal level 0/1/2 (-1 auto)")
     ap.add_argument("--persistent", choices=["auto", "on", "off"], default="auto")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the SpMV (ER 2^22 x 32) and batched-PPR (RMAT-22, k=256) figures")
+    ap.add_argument("--ppr-scale", type=int, default=22)
+    ap.add_argument("--ppr-k", type=int, default=256)
+    ap.add_argument("--ppr-tol", type=float, default=1e-6)
+    ap.add_argument("--spmv-log2", type=int, default=22)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -140,6 +240,16 @@ def main():
         "kernel": "pr_pull_kernel", "avg_launch_ms": round(avg_pull_ms, 5),
         "algorithmic_bytes_per_launch": bytes_iter,
     }
+    extras = {}
+    if not args.no_extras:
+        del plan, off, col, deg, scores
+        torch.cuda.empty_cache()
+        log("spmv ...")
+        extras["spmv"] = bench_spmv(args.spmv_log2, 32, 50, args.seed)
+        torch.cuda.empty_cache()
+        log("batched ppr ...")
+        extras["ppr"] = bench_ppr(args.ppr_scale, args.ppr_k, args.ppr_tol, args.seed, 2)
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -159,6 +269,9 @@ def main():
                    "plan_ms": round(info["plan_ms"], 1)},
         "roofline": roof, "cpu_baseline": cpu,
     }
+    if "ppr" in extras:
+        rec["ppr_speedup_vs_looped"] = extras["ppr"]["speedup"]
+    rec.update(extras)
     print(json.dumps(rec), flush=True)
 
 

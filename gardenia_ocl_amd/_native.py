@@ -154,6 +154,14 @@ SIGNATURES = {
                              _P]),
     "gd_ppr_plan_info": (_i, [_P, ctypes.POINTER(PprPlanInfo)]),
     "gd_ppr_plan_destroy": (_i, [_P]),
+    "gd_wpr_plan_create": (_i, [_i, _i64, _P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "gd_wpr_forward": (_i, [_P, _P, ctypes.POINTER(PrOptions), ctypes.POINTER(PrStats), _P]),
+    "gd_wpr_backward": (_i, [_P, _P, _P, _P, _P, ctypes.POINTER(PrOptions),
+                             ctypes.POINTER(PrStats)]),
+    "gd_wpr_plan_destroy": (_i, [_P]),
+    "gd_spmv_bwd_plan_create": (_i, [_i, _i, _i64, _P, _P, _P, ctypes.POINTER(_P)]),
+    "gd_spmv_bwd_run": (_i, [_P, _P, _P, _P, _P, _P]),
+    "gd_spmv_bwd_plan_destroy": (_i, [_P]),
     "gd_spmv_plan_create": (_i, [_i, _i, _i64, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "gd_spmv_plan_run": (_i, [_P, _P, _P]),
     "gd_spmv_plan_info": (_i, [_P, _I64P, _I64P, _I32P]),

@@ -189,6 +189,48 @@ int gd_spmv_plan_destroy(gd_spmv_plan *plan);
 int gd_spmv_solve(int m, int nnz, const int32_t *Ap, const int32_t *Aj, const float *Ax,
                   const float *x, float *y, float *kernel_ms);
 
+/* ------------------------------------------------------------ differentiation */
+
+/* Weighted PageRank and its implicit adjoint (no counterpart in the reference,
+ * whose PageRank is unweighted).  Raw edge weights a (in-CSR order) are
+ * normalised per source: P[v,u] = a_uv / s_u, s_u = sum_v a_uv, and
+ *     x = (1-d)/m + d P x            (a == 1 gives the reference iteration).
+ * For a loss with dL/dx = w the backward pass iterates the adjoint
+ *     lambda = w + d P^T lambda
+ * with the same pull engine over a CSC built once per plan, then writes
+ *     grad[e] = dL/da[e] = d (x_u / s_u) (lambda_v - (P^T lambda)_u)
+ * for every in-CSR edge e = (u -> v).  O(V + E) memory, no stored iterates.
+ * The weight array is borrowed and re-read by every forward / backward call
+ * (it may change between calls, e.g. during training). */
+typedef struct gd_wpr_plan gd_wpr_plan;
+
+int gd_wpr_plan_create(int m, int64_t nnz, const int32_t *d_in_row_offsets,
+                       const int32_t *d_in_column_indices, const float *d_weight, void *stream,
+                       gd_wpr_plan **plan);
+
+/* d_x: in initial scores (e.g. 1/m), out the fixed point. */
+int gd_wpr_forward(gd_wpr_plan *plan, float *d_x, const gd_pr_options *opts, gd_pr_stats *stats,
+                   double *errors);
+
+/* d_x: the forward solution; d_w: dL/dx; out: d_lambda (m), d_grad (nnz, in-CSR order). */
+int gd_wpr_backward(gd_wpr_plan *plan, const float *d_x, const float *d_w, float *d_lambda,
+                    float *d_grad, const gd_pr_options *opts, gd_pr_stats *stats);
+
+int gd_wpr_plan_destroy(gd_wpr_plan *plan);
+
+/* Backward of y = A x (+ y0): dx = A^T g (the SpMV engine over a CSC built once
+ * per plan, values re-permuted from d_Ax on each call) and
+ * dA[k] = g[row(k)] * x[col(k)] in CSR order.  Either output may be NULL. */
+typedef struct gd_spmv_bwd_plan gd_spmv_bwd_plan;
+
+int gd_spmv_bwd_plan_create(int num_rows, int num_cols, int64_t nnz, const int32_t *d_Ap,
+                            const int32_t *d_Aj, void *stream, gd_spmv_bwd_plan **plan);
+
+int gd_spmv_bwd_run(gd_spmv_bwd_plan *plan, const float *d_Ax, const float *d_x, const float *d_g,
+                    float *d_dx, float *d_dA);
+
+int gd_spmv_bwd_plan_destroy(gd_spmv_bwd_plan *plan);
+
 #ifdef __cplusplus
 }
 #endif

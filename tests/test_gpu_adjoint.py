@@ -1,0 +1,152 @@
+"""GPU adjoints: weighted PageRank forward/backward and SpMV backward against
+FP64 references (oracle), central finite differences and the plain kernels."""
+import numpy as np
+import pytest
+
+import oracle
+from gardenia_ocl_amd import PageRankPlan, SpmvPlan, graph
+
+pytestmark = pytest.mark.gpu
+
+
+def _weighted(scale, seed, device):
+    import torch
+
+    out_csr, in_csr = graph.rmat(scale, 16, seed=seed)
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.5, 1.5, in_csr.nnz).astype(np.float32)
+    t = lambda v: torch.from_numpy(v).to(device)  # noqa: E731
+    return out_csr, in_csr, a, t(in_csr.row_offsets), t(in_csr.column_indices), t(a)
+
+
+def test_wpr_unit_weights_equal_plain_pagerank(device):
+    """a == 1: s_u = out-degree and 1.0f * c == c, so the weighted forward is
+    the reference iteration, bit for bit, with the same tiles."""
+    import torch
+
+    from gardenia_ocl_amd.autograd import WeightedPageRankPlan
+
+    out_csr, in_csr = graph.rmat(14, 16, seed=2)
+    m = in_csr.m
+    off = torch.from_numpy(in_csr.row_offsets).to(device)
+    col = torch.from_numpy(in_csr.column_indices).to(device)
+    ones = torch.ones(in_csr.nnz, dtype=torch.float32, device=device)
+    wp = WeightedPageRankPlan(off, col, ones)
+    x = torch.full((m,), 1.0 / m, dtype=torch.float32, device=device)
+    st = wp.forward(x, tol=1e-4, max_iter=100)
+    deg = torch.from_numpy(out_csr.degree).to(device)
+    pr = PageRankPlan(off, col, deg, relabel=False, items_per_thread=8)
+    s = torch.full((m,), 1.0 / m, dtype=torch.float32, device=device)
+    st2, _ = pr.run(s, tol=1e-4, max_iter=100)
+    np.testing.assert_array_equal(x.cpu().numpy(), s.cpu().numpy())
+    assert st["iterations"] == st2["iterations"]
+
+
+def test_wpr_forward_matches_fp64(device):
+    import torch
+
+    from gardenia_ocl_amd.autograd import WeightedPageRankPlan
+
+    out_csr, in_csr, a, off, col, ta = _weighted(14, 5, device)
+    m = in_csr.m
+    wp = WeightedPageRankPlan(off, col, ta)
+    x = torch.full((m,), 1.0 / m, dtype=torch.float32, device=device)
+    st = wp.forward(x, tol=1e-7, max_iter=300)
+    assert st["converged"]
+    x64, _ = oracle.wpr_fp64(m, in_csr.row_offsets, in_csr.column_indices, a.astype(np.float64))
+    xg = x.cpu().numpy().astype(np.float64)
+    assert np.abs(xg - x64).sum() < 5e-6
+    assert np.max(np.abs(xg - x64) / x64) < 1e-4
+
+
+def test_wpr_gradient_matches_fp64_reference_and_fd(device):
+    """dL/da for L = sum_v w_v x_v: FP64 reference gradient over all edges and
+    FP64 central differences (eps = 1e-3) on 32 random edges."""
+    import torch
+
+    from gardenia_ocl_amd.autograd import WeightedPageRankPlan
+
+    out_csr, in_csr, a, off, col, ta = _weighted(12, 7, device)
+    m, nnz = in_csr.m, in_csr.nnz
+    rng = np.random.default_rng(3)
+    w = rng.standard_normal(m)
+    wp = WeightedPageRankPlan(off, col, ta)
+    x = torch.full((m,), 1.0 / m, dtype=torch.float32, device=device)
+    wp.forward(x, tol=1e-9, max_iter=500)
+    lam = torch.empty_like(x)
+    grad = torch.empty(nnz, dtype=torch.float32, device=device)
+    tw = torch.from_numpy(w.astype(np.float32)).to(device)
+    st = wp.backward(x, tw, lam, grad, tol=1e-9, max_iter=500)
+    assert st["converged"]
+    g = grad.cpu().numpy().astype(np.float64)
+    a64 = a.astype(np.float64)
+    x64, _ = oracle.wpr_fp64(m, in_csr.row_offsets, in_csr.column_indices, a64)
+    lam64, g64, _ = oracle.wpr_grad_fp64(m, in_csr.row_offsets, in_csr.column_indices, a64, x64,
+                                         w.astype(np.float32).astype(np.float64))
+    scale = np.abs(g64).max()
+    assert np.abs(g - g64).max() <= 2e-4 * scale
+    np.testing.assert_allclose(lam.cpu().numpy(), lam64, rtol=2e-4, atol=2e-5 * np.abs(lam64).max())
+    eps = 1e-3
+    for e in rng.choice(nnz, 32, replace=False):
+        ap, am = a64.copy(), a64.copy()
+        ap[e] += eps
+        am[e] -= eps
+        xp, _ = oracle.wpr_fp64(m, in_csr.row_offsets, in_csr.column_indices, ap)
+        xm, _ = oracle.wpr_fp64(m, in_csr.row_offsets, in_csr.column_indices, am)
+        fd = (w @ xp - w @ xm) / (2 * eps)
+        assert abs(g[e] - fd) <= 2e-3 * abs(fd) + 2e-4 * scale, (e, g[e], fd)
+
+
+def test_wpr_autograd_function(device):
+    import torch
+
+    from gardenia_ocl_amd.autograd import WeightedPageRankPlan, weighted_pagerank
+
+    out_csr, in_csr, a, off, col, ta = _weighted(11, 9, device)
+    weight = torch.nn.Parameter(ta)
+    plan = WeightedPageRankPlan(off, col, weight.data)
+    x = weighted_pagerank(weight, plan, tol=1e-9, max_iter=500)
+    w = torch.randn(in_csr.m, device=device)
+    loss = (w * x).sum()
+    loss.backward()
+    g = weight.grad.cpu().numpy().astype(np.float64)
+    a64 = a.astype(np.float64)
+    x64, _ = oracle.wpr_fp64(in_csr.m, in_csr.row_offsets, in_csr.column_indices, a64)
+    _, g64, _ = oracle.wpr_grad_fp64(in_csr.m, in_csr.row_offsets, in_csr.column_indices, a64,
+                                     x64, w.cpu().numpy().astype(np.float64))
+    assert np.abs(g - g64).max() <= 2e-4 * np.abs(g64).max()
+
+
+@pytest.mark.parametrize("kind", ["er", "rmat"])
+def test_spmv_backward_matches_fp64(device, kind):
+    import torch
+
+    from gardenia_ocl_amd.autograd import SpmvBackwardPlan, spmv
+
+    if kind == "er":
+        csr = graph.er_matrix(14, 32, seed=2)
+        vals = csr.weight
+    else:
+        _, csr = graph.rmat(14, 16, seed=4)
+        vals = graph.uniform_pm1(csr.nnz, 5)
+    m = csr.m
+    x = graph.uniform_pm1(m, 6)
+    gv = graph.uniform_pm1(m, 7)
+    t = lambda v: torch.from_numpy(np.ascontiguousarray(v)).to(device)  # noqa: E731
+    Ap, Aj = t(csr.row_offsets), t(csr.column_indices)
+    values = torch.nn.Parameter(t(vals))
+    xs = t(x).requires_grad_(True)
+    plan = SpmvPlan(Ap, Aj, values.data)
+    bwd = SpmvBackwardPlan(Ap, Aj)
+    y = spmv(values, xs, plan, bwd)
+    y.backward(t(gv))
+    dx64, dA64 = oracle.spmv_bwd_fp64(m, m, csr.row_offsets, csr.column_indices, vals, x, gv)
+    dx = xs.grad.cpu().numpy().astype(np.float64)
+    dA = values.grad.cpu().numpy().astype(np.float64)
+    # dA[k] = g * x is one FP32 product: exact up to its rounding
+    np.testing.assert_allclose(dA, dA64, rtol=1e-7, atol=0)
+    assert np.abs(dx - dx64).max() <= 1e-5 * max(1.0, np.abs(dx64).max())
+    # forward value too
+    y_ref = oracle.spmv(m, csr.row_offsets, csr.column_indices, vals, x, np.zeros(m, np.float32))
+    np.testing.assert_array_equal(y.detach().cpu().numpy()[csr.degree <= 64],
+                                  y_ref[csr.degree <= 64])
