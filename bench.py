@@ -70,26 +70,31 @@ def bench_spmv(log2_rows, nnz_per_row, reps, seed):
     Ax = torch.from_numpy(er.weight).to(dev)
     x = torch.from_numpy(graph.uniform_pm1(er.m, seed + 1)).to(dev)
     y = torch.zeros(er.m, dtype=torch.float32, device=dev)
-    plan = SpmvPlan(Ap, Aj, Ax)
-    for _ in range(3):
-        plan.run(x, y)
-    st = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(st)
-    for _ in range(reps):
-        plan.run(x, y)
-    e1.record(st)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
     nnz, M = er.nnz, er.m
     byts = 8 * nnz + 16 * M + 4
-    return {"workload": f"ER 2^{log2_rows} x 2^{log2_rows}, {nnz_per_row} nnz/row, y += A x",
-            "nnz": nnz, "reps": reps, "ms_per_spmv": round(ms, 5),
-            "gflops": round(2 * nnz / (ms * 1e-3) / 1e9, 2),
-            "achieved_gbs": round(byts / (ms * 1e-3) / 1e9, 1),
-            "frac_of_hbm_peak": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "algorithmic_bytes": byts}
+    st = torch.cuda.current_stream()
+    out = {"workload": f"ER 2^{log2_rows} x 2^{log2_rows}, {nnz_per_row} nnz/row, y += A x",
+           "nnz": nnz, "reps": reps, "algorithmic_bytes": byts}
+    for mode in ("auto", "tiles"):
+        plan = SpmvPlan(Ap, Aj, Ax, mode=mode)
+        for _ in range(3):
+            plan.run(x, y)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(st)
+        for _ in range(reps):
+            plan.run(x, y)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        key = "" if mode == "auto" else "tiles_"
+        out[key + "mode"] = plan.mode
+        out[key + "ms_per_spmv"] = round(ms, 5)
+        out[key + "gflops"] = round(2 * nnz / (ms * 1e-3) / 1e9, 2)
+        out[key + "achieved_gbs"] = round(byts / (ms * 1e-3) / 1e9, 1)
+        out[key + "frac_of_hbm_peak"] = round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        plan.close()
+    return out
 
 
 def bench_ppr(scale, k, tol, seed, fp64_seeds):

@@ -11,8 +11,9 @@ from ._native import GardeniaError, LIB_PATH
 from .pr import PRSolver, PageRankPlan, EPSILON, K_DAMP, MAX_ITER
 from .spmv import SpmvSolver, SpmvPlan
 from .ppr import PersonalizedPageRankPlan
+from .multigraph import MultiGraphPageRank
 
 __all__ = [
-    "graph", "GardeniaError", "PersonalizedPageRankPlan", "LIB_PATH", "PRSolver", "PageRankPlan", "SpmvSolver", "SpmvPlan",
+    "graph", "GardeniaError", "PersonalizedPageRankPlan", "MultiGraphPageRank", "LIB_PATH", "PRSolver", "PageRankPlan", "SpmvSolver", "SpmvPlan",
     "EPSILON", "K_DAMP", "MAX_ITER",
 ]

@@ -94,6 +94,10 @@ class PrPlanInfo(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class MultiPrConfig(ctypes.Structure):
+    _fields_ = [("use_graph", ctypes.c_int), ("chunk", ctypes.c_int)]
+
+
 class PprPlanConfig(ctypes.Structure):
     _fields_ = [("relabel", ctypes.c_int), ("k_chunk", ctypes.c_int)]
 
@@ -154,6 +158,11 @@ SIGNATURES = {
                              _P]),
     "gd_ppr_plan_info": (_i, [_P, ctypes.POINTER(PprPlanInfo)]),
     "gd_ppr_plan_destroy": (_i, [_P]),
+    "gd_multi_pr_default_config": (None, [ctypes.POINTER(MultiPrConfig)]),
+    "gd_multi_pr_plan_create": (_i, [_i, _P, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "gd_multi_pr_run": (_i, [_P, _P, ctypes.POINTER(PrOptions), ctypes.POINTER(MultiPrConfig),
+                             _P, _P, ctypes.POINTER(ctypes.c_float)]),
+    "gd_multi_pr_plan_destroy": (_i, [_P]),
     "gd_wpr_plan_create": (_i, [_i, _i64, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "gd_wpr_forward": (_i, [_P, _P, ctypes.POINTER(PrOptions), ctypes.POINTER(PrStats), _P]),
     "gd_wpr_backward": (_i, [_P, _P, _P, _P, _P, ctypes.POINTER(PrOptions),
@@ -163,6 +172,9 @@ SIGNATURES = {
     "gd_spmv_bwd_run": (_i, [_P, _P, _P, _P, _P, _P]),
     "gd_spmv_bwd_plan_destroy": (_i, [_P]),
     "gd_spmv_plan_create": (_i, [_i, _i, _i64, _P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "gd_spmv_plan_create_ex": (_i, [_i, _i, _i64, _P, _P, _P, _P, _i, ctypes.POINTER(_P)]),
+    "gd_spmv_plan_mode": (_i, [_P]),
+    "gd_spmv_plan_refresh_values": (_i, [_P]),
     "gd_spmv_plan_run": (_i, [_P, _P, _P]),
     "gd_spmv_plan_info": (_i, [_P, _I64P, _I64P, _I32P]),
     "gd_spmv_plan_destroy": (_i, [_P]),

@@ -21,63 +21,15 @@
 #include "gardenia_hip.h"
 #include "gd_common.h"
 #include "iter_engine.h"
+#include "pr_policy.h"
 #include "pull_engine.h"
 #include "relabel.h"
 
 namespace gd {
 
-struct PrGather {
-  const float* __restrict__ contrib;
-  __device__ __forceinline__ float value(int /*e*/, int u) const { return contrib[u]; }
-};
+}  // namespace gd
 
-// NT: 0 plain, 1 streamed arrays non-temporal, 2 also the new contrib
-template <int NT>
-struct PrEpilogue {
-  float* __restrict__ scores;
-  float* __restrict__ contrib_next;
-  const int32_t* __restrict__ outdeg;
-  float base;
-  float damp;
-  int seed;
-  struct Pre {
-    float old;
-    int deg;
-  };
-  __device__ __forceinline__ Pre pre(int v) const {
-    return Pre{ld_stream<(NT > 0)>(scores + v), ld_stream<(NT > 0)>(outdeg + v)};
-  }
-  __device__ __forceinline__ float init(const Pre&) const { return 0.0f; }
-  // scores[v] = base + kDamp * sum (two roundings), next contrib, |change|
-  // (src/pr/omp_base.cc:38-40, :25-27)
-  __device__ __forceinline__ double finish(int v, float sum, const Pre& p) const {
-    const float b = (seed < 0 || v == seed) ? base : 0.0f;
-    const float nv = b + damp * sum;
-    st_stream<(NT > 0)>(scores + v, nv);
-    st_stream<(NT > 1)>(contrib_next + v, nv / (float)p.deg);
-    return (double)fabsf(nv - p.old);
-  }
-};
-
-struct PrPolicy {
-  struct Args {
-    float* scores;
-    float* c0;
-    float* c1;
-    const int32_t* outdeg;
-    float base;
-    float damp;
-    int seed;
-  };
-  template <int NT>
-  __device__ static PrGather gather(const Args& a, int odd) {
-    return PrGather{odd ? a.c1 : a.c0};
-  }
-  template <int NT>
-  __device__ static PrEpilogue<NT> epilogue(const Args& a, int odd) {
-    return PrEpilogue<NT>{a.scores, odd ? a.c0 : a.c1, a.outdeg, a.base, a.damp, a.seed};
-  }
-};
+namespace gd {
 
 __global__ __launch_bounds__(kBlock) void pr_init_kernel(int m, const float* __restrict__ scores,
                                                          const int32_t* __restrict__ outdeg,

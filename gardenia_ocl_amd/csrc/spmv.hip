@@ -16,6 +16,7 @@
 #include "gardenia_hip.h"
 #include "gd_common.h"
 #include "pull_engine.h"
+#include <hipcub/hipcub.hpp>
 
 namespace gd {
 
@@ -69,6 +70,121 @@ __global__ __launch_bounds__(kBlock) void spmv_hub_kernel(const int32_t* __restr
   y[hub_row[h]] = s;
 }
 
+// ---------------------------------------------------------------- column slices
+//
+// For matrices whose x does not fit in one XCD's L2 (ER 2^22: 16.8 MB of x vs
+// 4 MB) every gather of the tile engine misses L2.  The sliced mode splits the
+// columns into kSlices ranges, one per XCD (workgroups are dispatched to the 8
+// XCDs round-robin, so block b runs slice b % 8 and each L2 holds one 2 MB
+// slice of x), and runs two passes:
+//   1. products p = x[Aj] * Ax over a slice-major copy of the matrix (fully
+//      coalesced; x gathers hit L2);
+//   2. y[r] += products of row r, slice 0's then slice 1's ..., i.e. in the
+//      original column order for rows with sorted columns -- bit-identical to
+//      the reference's sequential sum.
+constexpr int kSlices = 8;
+constexpr int kSliceIPT = 16;
+
+// cnt[s * (rows+1) + r] = entries of row r in column slice s.
+__global__ void slice_count_kernel(int rows, const int32_t* __restrict__ Ap,
+                                   const int32_t* __restrict__ Aj, int width,
+                                   int32_t* __restrict__ cnt, int* __restrict__ unsorted) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
+    int c[kSlices] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int prev = -1, bad = 0;
+    for (int k = Ap[r]; k < Ap[r + 1]; k++) {
+      const int j = Aj[k];
+      bad |= j <= prev;
+      prev = j;
+      c[min(j / width, kSlices - 1)]++;
+    }
+#pragma unroll
+    for (int q = 0; q < kSlices; q++) cnt[(int64_t)q * (rows + 1) + r] = c[q];
+    if (bad) atomicOr(unsorted, 1);
+  }
+}
+
+// Scatter each row's entries into the slice-major arrays (row order kept).
+__global__ void slice_scatter_kernel(int rows, const int32_t* __restrict__ Ap,
+                                     const int32_t* __restrict__ Aj, const float* __restrict__ Ax,
+                                     int width, const int32_t* __restrict__ soff,
+                                     const int64_t* __restrict__ sbase, int32_t* __restrict__ Aj_s,
+                                     float* __restrict__ Ax_s, int values_only) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
+    int64_t pos[kSlices];
+#pragma unroll
+    for (int q = 0; q < kSlices; q++) pos[q] = sbase[q] + soff[(int64_t)q * (rows + 1) + r];
+    for (int k = Ap[r]; k < Ap[r + 1]; k++) {
+      const int j = Aj[k];
+      const int q = min(j / width, kSlices - 1);
+      int64_t at = 0;
+#pragma unroll
+      for (int t = 0; t < kSlices; t++)
+        if (t == q) at = pos[t]++;
+      if (!values_only) Aj_s[at] = j;
+      Ax_s[at] = Ax[k];
+    }
+  }
+}
+
+// Pass 1: block b takes slice b % kSlices, chunk b / kSlices of that slice.
+__global__ __launch_bounds__(kBlock) void slice_products_kernel(
+    const int32_t* __restrict__ Aj_s, const float* __restrict__ Ax_s,
+    const int64_t* __restrict__ sbase, const float* __restrict__ x, float* __restrict__ p) {
+  const int q = blockIdx.x % kSlices;
+  const int64_t c = blockIdx.x / kSlices;
+  const int64_t lo = sbase[q] + c * (kBlock * kSliceIPT);
+  const int64_t hi = min(sbase[q + 1], lo + (int64_t)kBlock * kSliceIPT);
+  if (lo >= hi) return;  // this slice has fewer chunks than the longest one
+  int j[kSliceIPT];
+  float a[kSliceIPT];
+#pragma unroll
+  for (int i = 0; i < kSliceIPT; i++) {
+    const int64_t k = min(lo + threadIdx.x + (int64_t)i * kBlock, hi - 1);
+    j[i] = __builtin_nontemporal_load(Aj_s + k);
+    a[i] = __builtin_nontemporal_load(Ax_s + k);
+  }
+  float v[kSliceIPT];
+#pragma unroll
+  for (int i = 0; i < kSliceIPT; i++) v[i] = x[j[i]] * a[i];
+#pragma unroll
+  for (int i = 0; i < kSliceIPT; i++) {
+    const int64_t k = lo + threadIdx.x + (int64_t)i * kBlock;
+    if (k < hi) p[k] = v[i];
+  }
+}
+
+// Pass 2: y[r] = y[r] + products of row r in slice order (thread per row).
+__global__ __launch_bounds__(kBlock) void slice_rowsum_kernel(int rows,
+                                                              const int32_t* __restrict__ soff,
+                                                              const int64_t* __restrict__ sbase,
+                                                              const float* __restrict__ p,
+                                                              float* __restrict__ y) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  int b[kSlices], e[kSlices];
+#pragma unroll
+  for (int q = 0; q < kSlices; q++) {
+    b[q] = soff[(int64_t)q * (rows + 1) + r];
+    e[q] = soff[(int64_t)q * (rows + 1) + r + 1];
+  }
+  float s = y[r];
+#pragma unroll
+  for (int q = 0; q < kSlices; q++) {
+    const float* pq = p + sbase[q];
+    int k = b[q];
+    for (; k + 4 <= e[q]; k += 4) {
+      const float a0 = pq[k], a1 = pq[k + 1], a2 = pq[k + 2], a3 = pq[k + 3];
+      s += a0;
+      s += a1;
+      s += a2;
+      s += a3;
+    }
+    for (; k < e[q]; k++) s += pq[k];
+  }
+  y[r] = s;
+}
+
 }  // namespace gd
 
 using namespace gd;
@@ -85,12 +201,89 @@ struct gd_spmv_plan {
   DeviceBuffer tiles, hub_row, hub_slot, hub_nseg, hub_part;
   DeviceBuffer dummy_col;  // 1 element, used as Aj when nnz == 0
   int e_max = 0;
+  // column-sliced mode
+  bool sliced = false;
+  int width = 0, slice_blocks = 0;
+  DeviceBuffer soff, sbase, Aj_s, Ax_s, prod;
 };
+
+namespace {
+
+int build_sliced(gd_spmv_plan* p) {
+  const int rows = p->rows;
+  p->width = std::max(1, (p->cols + kSlices - 1) / kSlices);
+  int rc = p->soff.alloc(4 * (size_t)kSlices * (rows + 1));
+  if (!rc) rc = p->sbase.alloc(8 * (kSlices + 1));
+  if (!rc) rc = p->Aj_s.alloc(4 * (size_t)p->nnz);
+  if (!rc) rc = p->Ax_s.alloc(4 * (size_t)p->nnz);
+  if (!rc) rc = p->prod.alloc(4 * (size_t)p->nnz);
+  if (rc) return rc;
+  DeviceBuffer cnt, bad, tmp;
+  rc = cnt.alloc(4 * (size_t)kSlices * (rows + 1));
+  if (!rc) rc = bad.alloc(4);
+  if (rc) return rc;
+  GD_HIP_TRY(hipMemsetAsync(bad.ptr, 0, 4, p->stream));
+  GD_HIP_TRY(hipMemsetAsync(cnt.ptr, 0, 4 * (size_t)kSlices * (rows + 1), p->stream));
+  const int g = std::max(1, std::min((rows + 255) / 256, 16384));
+  hipLaunchKernelGGL(slice_count_kernel, dim3(g), dim3(256), 0, p->stream, rows, p->Ap, p->Aj,
+                     p->width, cnt.as<int32_t>(), bad.as<int>());
+  GD_HIP_TRY(hipGetLastError());
+  int h_bad = 0;
+  GD_HIP_TRY(hipMemcpyAsync(&h_bad, bad.ptr, 4, hipMemcpyDeviceToHost, p->stream));
+  GD_HIP_TRY(hipStreamSynchronize(p->stream));
+  if (h_bad) {
+    set_error("column-sliced SpMV needs sorted columns within each row");
+    return GD_ERR_INVALID;
+  }
+  size_t bytes = 0;
+  GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt.as<int32_t>(),
+                                             p->soff.as<int32_t>(), rows + 1, p->stream));
+  rc = tmp.alloc(bytes);
+  if (rc) return rc;
+  for (int q = 0; q < kSlices; q++)
+    GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(
+        tmp.ptr, bytes, cnt.as<int32_t>() + (size_t)q * (rows + 1),
+        p->soff.as<int32_t>() + (size_t)q * (rows + 1), rows + 1, p->stream));
+  std::vector<int32_t> tot(kSlices);
+  for (int q = 0; q < kSlices; q++)
+    GD_HIP_TRY(hipMemcpyAsync(&tot[q], p->soff.as<int32_t>() + (size_t)q * (rows + 1) + rows, 4,
+                              hipMemcpyDeviceToHost, p->stream));
+  GD_HIP_TRY(hipStreamSynchronize(p->stream));
+  std::vector<int64_t> base(kSlices + 1, 0);
+  int64_t chunks = 0;
+  for (int q = 0; q < kSlices; q++) {
+    base[q + 1] = base[q] + tot[q];
+    chunks = std::max<int64_t>(chunks, (tot[q] + kBlock * kSliceIPT - 1) / (kBlock * kSliceIPT));
+  }
+  if (base[kSlices] != p->nnz) {
+    set_error("column-sliced SpMV: slice counts do not add up to nnz");
+    return GD_ERR_INVALID;
+  }
+  p->slice_blocks = (int)(chunks * kSlices);
+  GD_HIP_TRY(hipMemcpyAsync(p->sbase.ptr, base.data(), 8 * (kSlices + 1), hipMemcpyHostToDevice,
+                            p->stream));
+  hipLaunchKernelGGL(slice_scatter_kernel, dim3(g), dim3(256), 0, p->stream, rows, p->Ap, p->Aj,
+                     p->Ax, p->width, p->soff.as<int32_t>(), p->sbase.as<int64_t>(),
+                     p->Aj_s.as<int32_t>(), p->Ax_s.as<float>(), 0);
+  GD_HIP_TRY(hipGetLastError());
+  GD_HIP_TRY(hipStreamSynchronize(p->stream));
+  return GD_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
 int gd_spmv_plan_create(int rows, int cols, int64_t nnz, const int32_t* d_Ap, const int32_t* d_Aj,
                         const float* d_Ax, void* stream, gd_spmv_plan** out) {
+  return gd_spmv_plan_create_ex(rows, cols, nnz, d_Ap, d_Aj, d_Ax, stream, GD_SPMV_TILES, out);
+}
+
+int gd_spmv_plan_create_ex(int rows, int cols, int64_t nnz, const int32_t* d_Ap,
+                           const int32_t* d_Aj, const float* d_Ax, void* stream, int mode,
+                           gd_spmv_plan** out) {
+  GD_CHECK_ARG(mode == GD_SPMV_AUTO || mode == GD_SPMV_TILES || mode == GD_SPMV_SLICED,
+               "unknown SpMV mode");
   GD_CHECK_ARG(out != nullptr, "plan output pointer is NULL");
   *out = nullptr;
   GD_CHECK_ARG(rows >= 0 && cols >= 0 && nnz >= 0 && nnz < (1ll << 31), "size out of range");
@@ -120,6 +313,16 @@ int gd_spmv_plan_create(int rows, int cols, int64_t nnz, const int32_t* d_Ap, co
     }
     if (!rc) rc = build_schedule(rows, nnz, h_off.data(), kSpTileNnz, kTileRows, &p->sched);
     if (!rc) rc = check_columns(d_Aj, nnz, cols, p->stream);
+    if (!rc && nnz > 0 && mode != GD_SPMV_TILES) {
+      // auto: slice when x spans more than one XCD's L2 and rows are short
+      const bool want = mode == GD_SPMV_SLICED ||
+                        ((int64_t)cols * 4 > (4 << 20) && p->sched.max_row_len <= kSeqRow);
+      if (want) {
+        rc = build_sliced(p);
+        if (!rc) p->sliced = true;
+        if (rc && mode == GD_SPMV_AUTO) rc = GD_OK;  // unsorted rows: tile engine
+      }
+    }
     if (!rc && nnz == 0) {
       rc = p->dummy_col.alloc(4);
       if (!rc && hipMemsetAsync(p->dummy_col.ptr, 0, 4, p->stream) != hipSuccess) rc = GD_ERR_DEVICE;
@@ -167,6 +370,16 @@ int gd_spmv_plan_run(gd_spmv_plan* p, const float* d_x, float* d_y) {
   GD_CHECK_ARG(p != nullptr, "NULL plan");
   if (p->rows == 0) return GD_OK;
   GD_CHECK_ARG(d_y != nullptr && (p->nnz == 0 || d_x != nullptr), "NULL x / y");
+  if (p->sliced) {
+    hipLaunchKernelGGL(slice_products_kernel, dim3(p->slice_blocks), dim3(kBlock), 0, p->stream,
+                       p->Aj_s.as<int32_t>(), p->Ax_s.as<float>(), p->sbase.as<int64_t>(), d_x,
+                       p->prod.as<float>());
+    hipLaunchKernelGGL(slice_rowsum_kernel, dim3((p->rows + kBlock - 1) / kBlock), dim3(kBlock),
+                       0, p->stream, p->rows, p->soff.as<int32_t>(), p->sbase.as<int64_t>(),
+                       p->prod.as<float>(), d_y);
+    GD_HIP_TRY(hipGetLastError());
+    return GD_OK;
+  }
   hipLaunchKernelGGL(spmv_pull_kernel, dim3(p->n_tiles), dim3(kBlock), 0, p->stream,
                      p->tiles.as<Tile>(), p->Ap, p->Aj, p->e_max, p->Ax, d_x, d_y,
                      p->hub_part.as<float>());
@@ -176,6 +389,21 @@ int gd_spmv_plan_run(gd_spmv_plan* p, const float* d_x, float* d_y) {
                        p->hub_nseg.as<int32_t>(), p->n_hubs, p->hub_part.as<float>(), d_y);
   GD_HIP_TRY(hipGetLastError());
   return GD_OK;
+}
+
+int gd_spmv_plan_refresh_values(gd_spmv_plan* p) {
+  GD_CHECK_ARG(p != nullptr, "NULL plan");
+  if (!p->sliced) return GD_OK;  // the tile engine reads the caller's values directly
+  const int g = std::max(1, std::min((p->rows + 255) / 256, 16384));
+  hipLaunchKernelGGL(slice_scatter_kernel, dim3(g), dim3(256), 0, p->stream, p->rows, p->Ap,
+                     p->Aj, p->Ax, p->width, p->soff.as<int32_t>(), p->sbase.as<int64_t>(),
+                     p->Aj_s.as<int32_t>(), p->Ax_s.as<float>(), 1);
+  GD_HIP_TRY(hipGetLastError());
+  return GD_OK;
+}
+
+int gd_spmv_plan_mode(const gd_spmv_plan* p) {
+  return p == nullptr ? -1 : (p->sliced ? GD_SPMV_SLICED : GD_SPMV_TILES);
 }
 
 int gd_spmv_plan_info(const gd_spmv_plan* p, int64_t* n_tiles, int64_t* n_hubs,
@@ -218,8 +446,8 @@ int gd_spmv_solve(int m, int nnz, const int32_t* Ap, const int32_t* Aj, const fl
     GD_HIP_TRY(hipMemcpy(dAx.ptr, Ax, 4 * (size_t)nnz, hipMemcpyHostToDevice));
   }
   gd_spmv_plan* plan = nullptr;
-  rc = gd_spmv_plan_create(m, m, nnz, dAp.as<int32_t>(), dAj.as<int32_t>(), dAx.as<float>(),
-                           nullptr, &plan);
+  rc = gd_spmv_plan_create_ex(m, m, nnz, dAp.as<int32_t>(), dAj.as<int32_t>(), dAx.as<float>(),
+                              nullptr, GD_SPMV_AUTO, &plan);
   if (rc) return rc;
   hipEvent_t t0, t1;
   GD_HIP_TRY(hipEventCreate(&t0));

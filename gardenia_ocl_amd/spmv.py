@@ -45,7 +45,12 @@ class SpmvPlan:
     Borrowed tensors: keep them alive and unchanged while the plan lives.
     """
 
-    def __init__(self, Ap, Aj, Ax, num_cols=None, stream=None):
+    MODES = {"auto": 0, "tiles": 1, "sliced": 2}
+
+    def __init__(self, Ap, Aj, Ax, num_cols=None, stream=None, mode="tiles"):
+        """mode: "tiles" (the tile engine), "sliced" (XCD column slices, sorted
+        rows only), "auto" (sliced when x exceeds one XCD's L2 and rows are
+        short)."""
         import torch
 
         for t, dt in ((Ap, torch.int32), (Aj, torch.int32), (Ax, torch.float32)):
@@ -59,9 +64,15 @@ class SpmvPlan:
         self._keep = (Ap, Aj, Ax)
         self.stream = stream if stream is not None else torch.cuda.current_stream()
         h = ctypes.c_void_p()
-        N.call("gd_spmv_plan_create", self.rows, self.cols, self.nnz, Ap.data_ptr(), Aj.data_ptr(),
-               Ax.data_ptr(), ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h))
+        N.call("gd_spmv_plan_create_ex", self.rows, self.cols, self.nnz, Ap.data_ptr(),
+               Aj.data_ptr(), Ax.data_ptr(), ctypes.c_void_p(self.stream.cuda_stream),
+               self.MODES[mode], ctypes.byref(h))
         self._h = h
+        self.mode = {1: "tiles", 2: "sliced"}[N.lib().gd_spmv_plan_mode(h)]
+
+    def refresh_values(self):
+        """Re-read Ax after it changed (needed by the sliced mode)."""
+        N.call("gd_spmv_plan_refresh_values", self._h)
 
     def info(self) -> dict:
         t, hb = ctypes.c_int64(), ctypes.c_int64()

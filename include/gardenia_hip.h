@@ -171,7 +171,8 @@ int gd_ppr_plan_destroy(gd_ppr_plan *plan);
 
 typedef struct gd_spmv_plan gd_spmv_plan;
 
-/* Plan over a device-resident CSR matrix (num_rows x num_cols), borrowed. */
+/* Plan over a device-resident CSR matrix (num_rows x num_cols), borrowed;
+ * the tile mode (see gd_spmv_plan_create_ex for the others). */
 int gd_spmv_plan_create(int num_rows, int num_cols, int64_t nnz, const int32_t *d_Ap,
                         const int32_t *d_Aj, const float *d_Ax, void *stream, gd_spmv_plan **plan);
 
@@ -185,9 +186,68 @@ int gd_spmv_plan_info(const gd_spmv_plan *plan, int64_t *n_tiles, int64_t *n_hub
 
 int gd_spmv_plan_destroy(gd_spmv_plan *plan);
 
-/* One-shot form of SpmvSolver with host arrays (src/spmv/spmv.h:29). */
+/* SpMV execution modes.
+ *   GD_SPMV_TILES   the tile engine over the caller's CSR (any matrix).
+ *   GD_SPMV_SLICED  columns split into 8 slices, one per XCD: pass 1 forms the
+ *                   products over a slice-major copy of the matrix (each XCD's
+ *                   L2 holds one slice of x), pass 2 adds each row's products
+ *                   to y in column order.  Needs sorted columns per row;
+ *                   bit-identical to the reference for every row.  The copy
+ *                   holds the values: call gd_spmv_plan_refresh_values after
+ *                   changing d_Ax.
+ *   GD_SPMV_AUTO    SLICED when x spans more than one XCD's L2 (cols*4 > 4 MB),
+ *                   every row has <= 64 nonzeros and the columns are sorted,
+ *                   else TILES. */
+enum { GD_SPMV_AUTO = 0, GD_SPMV_TILES = 1, GD_SPMV_SLICED = 2 };
+
+int gd_spmv_plan_create_ex(int num_rows, int num_cols, int64_t nnz, const int32_t *d_Ap,
+                           const int32_t *d_Aj, const float *d_Ax, void *stream, int mode,
+                           gd_spmv_plan **plan);
+
+/* Mode the plan runs in (GD_SPMV_TILES or GD_SPMV_SLICED), -1 for NULL. */
+int gd_spmv_plan_mode(const gd_spmv_plan *plan);
+
+/* Re-read d_Ax into the plan's sliced copy (no-op for the tile mode). */
+int gd_spmv_plan_refresh_values(gd_spmv_plan *plan);
+
+/* One-shot form of SpmvSolver with host arrays (src/spmv/spmv.h:29); mode AUTO. */
 int gd_spmv_solve(int m, int nnz, const int32_t *Ap, const int32_t *Aj, const float *Ax,
                   const float *x, float *y, float *kernel_ms);
+
+/* ------------------------------------------------------------ many graphs, one launch */
+
+/* PageRank on many independent graphs at once (no counterpart in the
+ * reference, which solves one graph per process).  The graphs are packed into
+ * one block-diagonal union at plan time; each keeps its own tiles, base score,
+ * iteration count and convergence test, so every graph's scores and iteration
+ * count equal a single-graph plan's (same tiles, no relabel).  One pull launch
+ * per iteration covers all graphs; a chunk of iterations is captured into a
+ * hipGraph once and replayed until every graph has converged.  Work runs on a
+ * private stream; the calls are synchronous. */
+typedef struct gd_multi_pr_plan gd_multi_pr_plan;
+
+typedef struct gd_multi_pr_config {
+  int use_graph; /* 1: replay a captured hipGraph per chunk; 0: launch eagerly */
+  int chunk;     /* iterations per chunk / captured graph (0: 8) */
+} gd_multi_pr_config;
+
+void gd_multi_pr_default_config(gd_multi_pr_config *cfg);
+
+/* Per graph g: m[g] vertices, nnz[g] edges, device in-CSR and out-degree
+ * (borrowed only during the call: the plan keeps its own packed copy). */
+int gd_multi_pr_plan_create(int n_graphs, const int32_t *m, const int64_t *nnz,
+                            const int32_t *const *d_in_row_offsets,
+                            const int32_t *const *d_in_column_indices,
+                            const int32_t *const *d_out_degree, void *stream,
+                            gd_multi_pr_plan **plan);
+
+/* d_scores[g]: device scores of graph g, in/out.  iterations / final_errors
+ * (host, n_graphs each, may be NULL); solve_ms (may be NULL). */
+int gd_multi_pr_run(gd_multi_pr_plan *plan, float *const *d_scores, const gd_pr_options *opts,
+                    const gd_multi_pr_config *cfg, int *iterations, double *final_errors,
+                    float *solve_ms);
+
+int gd_multi_pr_plan_destroy(gd_multi_pr_plan *plan);
 
 /* ------------------------------------------------------------ differentiation */
 

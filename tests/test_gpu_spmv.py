@@ -130,3 +130,83 @@ def test_spmv_empty_rows_and_matrix(device):
     empty = graph.csr_from_edges(m, np.zeros(0, np.int32), np.zeros(0, np.int32))
     y, _ = _gpu_spmv(device, empty, np.zeros(0, np.float32), x, y0)
     np.testing.assert_array_equal(y, y0)
+
+
+def _gpu_spmv_mode(device, csr, Ax, x, y0, mode, reps=1):
+    import torch
+
+    Ap = torch.from_numpy(csr.row_offsets).to(device)
+    Aj = torch.from_numpy(csr.column_indices).to(device)
+    tAx = torch.from_numpy(np.ascontiguousarray(Ax, np.float32)).to(device)
+    plan = SpmvPlan(Ap, Aj, tAx, num_cols=len(x), mode=mode)
+    tx = torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(device)
+    ty = torch.from_numpy(np.array(y0, np.float32)).to(device)
+    for _ in range(reps):
+        plan.run(tx, ty)
+    torch.cuda.synchronize()
+    return ty.cpu().numpy(), plan, (Ap, Aj, tAx, tx, ty)
+
+
+def test_spmv_sliced_er_bit_exact(golden, device):
+    arrays, meta = golden
+    md = meta["spmv"]["er12"]
+    er = graph.er_matrix(12, 32, seed=md["seed_matrix"])
+    sd = md["seed_values"]
+    Ax, x, y0 = (graph.uniform_pm1(n, sd + k) for k, n in ((0, er.nnz), (1, er.m), (2, er.m)))
+    y, plan, _ = _gpu_spmv_mode(device, er, Ax, x, y0, "sliced")
+    assert plan.mode == "sliced"
+    np.testing.assert_array_equal(y, arrays["er12/spmv/y"])
+    big = graph.er_matrix(21, 32, seed=13)
+    x = graph.uniform_pm1(big.m, 14)
+    y, plan, _ = _gpu_spmv_mode(device, big, big.weight, x, np.zeros(big.m, np.float32), "auto",
+                                reps=3)
+    assert plan.mode == "sliced"  # x = 8 MB > one XCD's L2, rows of 32
+    ref = np.zeros(big.m, np.float32)
+    for _ in range(3):
+        ref = oracle.spmv(big.m, big.row_offsets, big.column_indices, big.weight, x, ref)
+    np.testing.assert_array_equal(y, ref)
+
+
+def test_spmv_sliced_long_rows_are_sequential(golden, device):
+    """Pass 2 sums every row in column order, so even rows longer than 64 are
+    bit-identical to the reference (RMAT-12 in-CSR, max row > 1000)."""
+    arrays, meta = golden
+    _, ic = graph.rmat(12, 16, seed=1)
+    assert ic.degree.max() > 64
+    sd = meta["spmv"]["rmat12_in"]["seed"]
+    Ax, x, y0 = (graph.uniform_pm1(n, sd + k) for k, n in ((0, ic.nnz), (1, ic.m), (2, ic.m)))
+    y, plan, _ = _gpu_spmv_mode(device, ic, Ax, x, y0, "sliced")
+    np.testing.assert_array_equal(y, arrays["rmat12_in/spmv/y"])
+
+
+def test_spmv_sliced_refresh_values_and_fallback(device):
+    import torch
+
+    er = graph.er_matrix(14, 32, seed=21)
+    x = graph.uniform_pm1(er.m, 22)
+    y, plan, keep = _gpu_spmv_mode(device, er, er.weight, x, np.zeros(er.m, np.float32), "sliced")
+    Ap, Aj, tAx, tx, ty = keep
+    tAx.mul_(-0.5)
+    plan.refresh_values()
+    ty.zero_()
+    plan.run(tx, ty)
+    torch.cuda.synchronize()
+    ref = oracle.spmv(er.m, er.row_offsets, er.column_indices, tAx.cpu().numpy(), x,
+                      np.zeros(er.m, np.float32))
+    np.testing.assert_array_equal(ty.cpu().numpy(), ref)
+    # unsorted rows: forced sliced mode is rejected, auto falls back to tiles
+    from gardenia_ocl_amd import GardeniaError
+
+    rng = np.random.default_rng(0)
+    m = 3000
+    src = rng.integers(0, m, 20000).astype(np.int32)
+    dst = rng.integers(0, m, 20000).astype(np.int32)
+    un = graph.csr_from_edges(m, src, dst, sorted=False, remove_redundents=False)
+    Ax = graph.uniform_pm1(un.nnz, 3)
+    xx = graph.uniform_pm1(m, 4)
+    with pytest.raises(GardeniaError, match="sorted columns"):
+        _gpu_spmv_mode(device, un, Ax, xx, np.zeros(m, np.float32), "sliced")
+    y, plan, _ = _gpu_spmv_mode(device, un, Ax, xx, np.zeros(m, np.float32), "auto")
+    assert plan.mode == "tiles"
+    np.testing.assert_array_equal(
+        y, oracle.spmv(m, un.row_offsets, un.column_indices, Ax, xx, np.zeros(m, np.float32)))
