@@ -97,13 +97,66 @@ def bench_spmv(log2_rows, nnz_per_row, reps, seed):
     return out
 
 
+def fp64_pagerank(off, col, outdeg, *, damping=0.85, seeds=None, tol=1e-13, max_iter=1000):
+    """Independent FP64 reference on the device: power iteration
+    x <- base + d * A x with torch's CSR SpMV in float64, A[v, u] = 1/outdeg[u]
+    over the in-CSR (dangling mass dropped, as in src/pr/omp_base.cc).
+    seeds=None: uniform teleport (1-d)/V from x = 1/V; else one column per
+    seed, teleport (1-d) to the seed from the one-hot start.  Returns
+    (x [V] or [V, k], iterations)."""
+    import torch
+
+    V = int(outdeg.shape[0])
+    inv = 1.0 / outdeg.to(torch.float64).clamp(min=1)
+    A = torch.sparse_csr_tensor(off, col, inv[col.long()], (V, V))
+    if seeds is None:
+        base = torch.full((V, 1), (1.0 - damping) / V, dtype=torch.float64, device=off.device)
+        x = torch.full((V, 1), 1.0 / V, dtype=torch.float64, device=off.device)
+    else:
+        k = len(seeds)
+        sd = torch.as_tensor(seeds, dtype=torch.int64, device=off.device)
+        ar = torch.arange(k, device=off.device)
+        base = torch.zeros((V, k), dtype=torch.float64, device=off.device)
+        base[sd, ar] = 1.0 - damping
+        x = torch.zeros((V, k), dtype=torch.float64, device=off.device)
+        x[sd, ar] = 1.0
+    it = 0
+    while it < max_iter:
+        xn = base + damping * (A @ x)
+        err = float((xn - x).abs().sum(0).max())
+        x = xn
+        it += 1
+        if err < tol:
+            break
+    del A
+    return (x[:, 0] if seeds is None else x), it
+
+
+def bench_tol_run(plan, off, col, deg, tol, max_iter):
+    """The same plan run to an L1 tolerance from 1/V; L1 distance of the FP32
+    scores to the FP64 fixed point."""
+    import numpy as np
+    import torch
+
+    V = int(deg.shape[0])
+    s = torch.full((V,), float(np.float32(1.0) / np.float32(V)), dtype=torch.float32,
+                   device=off.device)
+    st, errs = plan.run(s, tol=tol, max_iter=max_iter, profile=False)
+    x64, it64 = fp64_pagerank(off, col, deg, tol=1e-13)
+    l1 = float((s.double() - x64).abs().sum())
+    rel = float(((s.double() - x64).abs() / x64.clamp(min=1e-300)).max())
+    return {"tol": tol, "iterations": st["iterations"], "converged": bool(st["converged"]),
+            "final_l1_change": st["final_error"], "near_tie": bool(st["near_tie"]),
+            "solve_ms": round(st["solve_ms"], 3), "l1_vs_fp64": l1,
+            "max_rel_vs_fp64": rel, "fp64_iterations": it64}
+
+
 def bench_ppr(scale, k, tol, seed, fp64_seeds):
     """Batched PPR (k seeds in one SpMM-style pull) vs k looped single-seed
     runs of the plain pull kernel, both to the same per-seed L1 tolerance."""
     import numpy as np
     import torch
 
-    import oracle
     from gardenia_ocl_amd import PageRankPlan, PersonalizedPageRankPlan, graph
 
     dev = torch.device("cuda:0")
@@ -137,22 +190,141 @@ def bench_ppr(scale, k, tol, seed, fp64_seeds):
         st, _ = pp.run(s, tol=tol, max_iter=300, seed_vertex=int(seeds[j]))
         its += st["iterations"]
         if j < fp64_seeds:
-            looped.append(s.cpu().numpy())
+            looped.append(s.clone())
     torch.cuda.synchronize()
     t_looped = time.perf_counter() - t0
     # L1 against the FP64 personalised solution for the first seeds
-    Xh = X[:, :fp64_seeds].cpu().numpy().astype(np.float64)
-    X64, _ = oracle.ppr_fp64(V, ic.row_offsets, ic.column_indices, od, seeds[:fp64_seeds],
-                             tol=1e-12, max_iter=400)
-    l1_b = [float(np.abs(Xh[:, j] - X64[:, j]).sum()) for j in range(fp64_seeds)]
-    l1_l = [float(np.abs(looped[j].astype(np.float64) - X64[:, j]).sum())
-            for j in range(fp64_seeds)]
+    X64, _ = fp64_pagerank(off, col, deg, seeds=seeds[:fp64_seeds], tol=1e-13)
+    l1_b = [float((X[:, j].double() - X64[:, j]).abs().sum()) for j in range(fp64_seeds)]
+    l1_l = [float((looped[j].double() - X64[:, j]).abs().sum()) for j in range(fp64_seeds)]
     return {"workload": f"RMAT-{scale} ({ic.nnz} edges), {k} one-hot seeds, per-seed L1 tol {tol}",
             "batched_s": round(t_batched, 4), "looped_s": round(t_looped, 4),
             "speedup": round(t_looped / t_batched, 2), "batched_iterations": bst["iterations"],
             "looped_iterations_total": its, "k_chunk": bp.info()["k_chunk"],
             "l1_vs_fp64_batched": l1_b, "l1_vs_fp64_looped": l1_l,
             "fp64_seeds_checked": fp64_seeds}
+
+
+def bench_multigraph(n, lo, hi, tol, seed):
+    """n RMAT graphs of scales uniform in [lo, hi]: one multi-graph plan (one
+    pull + one finalize launch per iteration for all graphs, replayed from a
+    hipGraph) vs one single-graph plan per graph run back to back."""
+    import numpy as np
+    import torch
+
+    from gardenia_ocl_amd import MultiGraphPageRank, PageRankPlan, graph
+
+    dev = torch.device("cuda:0")
+    scales = np.random.default_rng(seed + 5).integers(lo, hi + 1, n)
+    gs = []
+    for i, sc in enumerate(scales):
+        _, ic = graph.rmat(int(sc), 16, seed=seed + 100 + i, want_out=False)
+        od = np.bincount(ic.column_indices, minlength=ic.m).astype(np.int32)
+        gs.append(tuple(torch.from_numpy(a).to(dev) for a in (ic.row_offsets, ic.column_indices, od)))
+    E = int(sum(int(g[1].shape[0]) for g in gs))
+    init = [float(np.float32(1.0) / np.float32(int(g[2].shape[0]))) for g in gs]
+    scores = [torch.empty(int(g[2].shape[0]), dtype=torch.float32, device=dev) for g in gs]
+    mg = MultiGraphPageRank(gs)
+    out = {"workload": f"{n} RMAT graphs, scales {lo}-{hi} (seeded), L1 tol {tol}",
+           "scales": [int(x) for x in scales], "total_edges": E}
+    for use_graph in (True, False):
+        for rep in range(2):  # first run captures / warms up
+            for s, v in zip(scores, init):
+                s.fill_(v)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            its, errs, ms = mg.run(scores, tol=tol, max_iter=100, use_graph=use_graph)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+        key = "hipgraph" if use_graph else "stream"
+        out[f"{key}_ms"] = round(wall * 1e3, 3)
+        out[f"{key}_solve_ms"] = round(ms, 3)
+    out["iterations_max"] = int(its.max())
+    out["iterations_total"] = int(its.sum())
+    plans = [PageRankPlan(*g) for g in gs]
+    for p, s, v in zip(plans, scores, init):  # warm-up
+        s.fill_(v)
+        p.run(s, tol=tol, max_iter=100)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its_l = 0
+    for p, s, v in zip(plans, scores, init):
+        s.fill_(v)
+        st, _ = p.run(s, tol=tol, max_iter=100)
+        its_l += st["iterations"]
+    torch.cuda.synchronize()
+    out["looped_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+    out["looped_iterations_total"] = its_l
+    out["speedup_vs_looped"] = round(out["looped_ms"] / out["hipgraph_ms"], 2)
+    out["gteps_hipgraph"] = round(int(sum(int(g[1].shape[0]) * int(i) for g, i in zip(gs, its)))
+                                  / (out["hipgraph_ms"] * 1e-3) / 1e9, 2)
+    for p in plans:
+        p.close()
+    mg.close()
+    return out
+
+
+def bench_adjoint(scale, spmv_log2, seed):
+    """Weighted PageRank forward + implicit adjoint (RMAT, weights U(0.5,1.5))
+    and SpMV backward (dx = A^T g, dA = g[row] x[col]) on an ER matrix."""
+    import numpy as np
+    import torch
+
+    from gardenia_ocl_amd import SpmvPlan, graph
+    from gardenia_ocl_amd.autograd import SpmvBackwardPlan, WeightedPageRankPlan
+
+    dev = torch.device("cuda:0")
+    _, ic = graph.rmat(scale, 16, seed=seed, want_out=False)
+    m, nnz = ic.m, ic.nnz
+    a = np.random.default_rng(seed).uniform(0.5, 1.5, nnz).astype(np.float32)
+    off = torch.from_numpy(ic.row_offsets).to(dev)
+    col = torch.from_numpy(ic.column_indices).to(dev)
+    wt = torch.from_numpy(a).to(dev)
+    wp = WeightedPageRankPlan(off, col, wt)
+    x = torch.full((m,), 1.0 / m, dtype=torch.float32, device=dev)
+    w = torch.from_numpy(np.random.default_rng(seed + 1).standard_normal(m).astype(np.float32)).to(dev)
+    lam = torch.empty_like(x)
+    grad = torch.empty(nnz, dtype=torch.float32, device=dev)
+    wp.forward(x, tol=1e-7, max_iter=300)  # warm-up
+    wp.backward(x, w, lam, grad, tol=1e-7, max_iter=300)
+    x.fill_(1.0 / m)
+    f = wp.forward(x, tol=1e-7, max_iter=300)
+    b = wp.backward(x, w, lam, grad, tol=1e-7, max_iter=300)
+    out = {"workload": f"weighted PR RMAT-{scale} ({nnz} edges), tol 1e-7; SpMV backward "
+                       f"ER 2^{spmv_log2} x 32",
+           "forward_iterations": f["iterations"], "forward_ms": round(f["solve_ms"], 3),
+           "backward_iterations": b["iterations"], "backward_ms": round(b["solve_ms"], 3),
+           "forward_gteps": round(nnz * f["iterations"] / (f["solve_ms"] * 1e-3) / 1e9, 2),
+           "backward_gteps": round(nnz * b["iterations"] / (b["solve_ms"] * 1e-3) / 1e9, 2)}
+    wp.close()
+    del off, col, wt, grad
+    torch.cuda.empty_cache()
+    er = graph.er_matrix(spmv_log2, 32, seed=seed)
+    Ap = torch.from_numpy(er.row_offsets).to(dev)
+    Aj = torch.from_numpy(er.column_indices).to(dev)
+    Ax = torch.from_numpy(er.weight).to(dev)
+    xs = torch.from_numpy(graph.uniform_pm1(er.m, seed + 1)).to(dev)
+    g = torch.from_numpy(graph.uniform_pm1(er.m, seed + 2)).to(dev)
+    dx = torch.empty_like(xs)
+    dA = torch.empty_like(Ax)
+    bwd = SpmvBackwardPlan(Ap, Aj)
+    for _ in range(2):
+        bwd.run(Ax, xs, g, dx, dA)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record(st)
+    for _ in range(reps):
+        bwd.run(Ax, xs, g, dx, dA)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    # dx pull over the CSC: 8nnz + 16M + 4; dA edge map: 4(M+1) + 4nnz + 4nnz + 8M
+    byts = (8 * er.nnz + 16 * er.m + 4) + (4 * (er.m + 1) + 8 * er.nnz + 8 * er.m)
+    out.update({"spmv_bwd_ms": round(ms, 4),
+                "spmv_bwd_achieved_gbs": round(byts / (ms * 1e-3) / 1e9, 1)})
+    bwd.close()
+    return out
 
 
 def traffic_from_profile(scale):
@@ -179,11 +351,14 @@ def main():
     ap.add_argument("--nt", type=int, default=-1, help="non-temporal level 0/1/2 (-1 auto)")
     ap.add_argument("--persistent", choices=["auto", "on", "off"], default="auto")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the SpMV (ER 2^22 x 32) and batched-PPR (RMAT-22, k=256) figures")
+                    help="skip the tol-1e-6 run, SpMV (ER 2^22 x 32), batched-PPR (RMAT-22, "
+                         "k=256), multi-graph and adjoint figures")
     ap.add_argument("--ppr-scale", type=int, default=22)
     ap.add_argument("--ppr-k", type=int, default=256)
     ap.add_argument("--ppr-tol", type=float, default=1e-6)
     ap.add_argument("--spmv-log2", type=int, default=22)
+    ap.add_argument("--tol-run", type=float, default=1e-6,
+                    help="L1 tolerance of the extra run to convergence (with its FP64 L1)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -247,6 +422,8 @@ def main():
     }
     extras = {}
     if not args.no_extras:
+        log(f"run to tol {args.tol_run} ...")
+        extras["tol_run"] = bench_tol_run(plan, off, col, deg, args.tol_run, 200)
         del plan, off, col, deg, scores
         torch.cuda.empty_cache()
         log("spmv ...")
@@ -254,6 +431,12 @@ def main():
         torch.cuda.empty_cache()
         log("batched ppr ...")
         extras["ppr"] = bench_ppr(args.ppr_scale, args.ppr_k, args.ppr_tol, args.seed, 2)
+        torch.cuda.empty_cache()
+        log("multi-graph ...")
+        extras["multigraph"] = bench_multigraph(64, 12, 18, 1e-4, args.seed)
+        torch.cuda.empty_cache()
+        log("adjoint ...")
+        extras["adjoint"] = bench_adjoint(20, args.spmv_log2, args.seed)
         torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu_baseline:

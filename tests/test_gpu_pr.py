@@ -84,7 +84,8 @@ def test_pr_rmat_vs_reference(golden, device, scale):
     assert np.max(np.abs(s - ref) / ref) <= 1e-5
     if scale == 12:
         np.testing.assert_allclose(s, arrays["rmat12/scores"], rtol=1e-5, atol=0)
-    assert info["hubs"] == int(np.sum(in_csr.degree > 2048))
+    # rows longer than one tile (256 threads x items_per_thread edges) are hubs
+    assert info["hubs"] == int(np.sum(in_csr.degree > 256 * info["items_per_thread"]))
 
 
 def test_pr_first_iteration_short_rows_bit_exact(device):
@@ -163,7 +164,7 @@ def _star(n_leaves, hub=0):
     return out_csr, in_csr
 
 
-@pytest.mark.parametrize("n_leaves", [1, 63, 64, 65, 2047, 2048, 2049, 100_000])
+@pytest.mark.parametrize("n_leaves", [1, 63, 64, 65, 2047, 2048, 2049, 4096, 4097, 100_000])
 def test_pr_star_hub_boundaries(device, n_leaves):
     """Hub of n in-edges, all addends equal: the reference's sequential FP32 sum
     drifts (its L1 change plateaus at ~3e-6 for n=65, ~7e-5 for n=2048 and the
@@ -181,7 +182,7 @@ def test_pr_star_hub_boundaries(device, n_leaves):
         assert stats["iterations"] == it
     if n_leaves <= 64:  # every row sequential: same arithmetic as the reference
         np.testing.assert_array_equal(s, ref)
-    assert info["hubs"] == (1 if n_leaves > 2048 else 0)
+    assert info["hubs"] == (1 if n_leaves > 256 * info["items_per_thread"] else 0)
 
 
 def test_pr_empty_and_edgeless(device):
