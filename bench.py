@@ -200,6 +200,7 @@ def bench_ppr(scale, k, tol, seed, fp64_seeds):
     return {"workload": f"RMAT-{scale} ({ic.nnz} edges), {k} one-hot seeds, per-seed L1 tol {tol}",
             "batched_s": round(t_batched, 4), "looped_s": round(t_looped, 4),
             "speedup": round(t_looped / t_batched, 2), "batched_iterations": bst["iterations"],
+            "batched_column_iterations_total": int(bp.column_iterations().sum()),
             "looped_iterations_total": its, "k_chunk": bp.info()["k_chunk"],
             "l1_vs_fp64_batched": l1_b, "l1_vs_fp64_looped": l1_l,
             "fp64_seeds_checked": fp64_seeds}
@@ -357,6 +358,9 @@ def main():
     ap.add_argument("--ppr-k", type=int, default=256)
     ap.add_argument("--ppr-tol", type=float, default=1e-6)
     ap.add_argument("--spmv-log2", type=int, default=22)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 1 GiB device-to-device 5 times (scales rocprofv3's "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     ap.add_argument("--tol-run", type=float, default=1e-6,
                     help="L1 tolerance of the extra run to convergence (with its FP64 L1)")
     args = ap.parse_args()
@@ -372,6 +376,14 @@ def main():
     from gardenia_ocl_amd import PageRankPlan, graph
 
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        a = torch.ones(256 * 2**20, dtype=torch.float32, device=dev)  # 1 GiB
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 1 GiB 5 times")
+        return
     t0 = time.perf_counter()
     _, in_csr = graph.rmat(args.scale, args.edgefactor, seed=args.seed, want_out=False)
     V, E = in_csr.m, in_csr.nnz

@@ -99,7 +99,8 @@ class MultiPrConfig(ctypes.Structure):
 
 
 class PprPlanConfig(ctypes.Structure):
-    _fields_ = [("relabel", ctypes.c_int), ("k_chunk", ctypes.c_int)]
+    _fields_ = [("relabel", ctypes.c_int), ("k_chunk", ctypes.c_int),
+                ("column_stop", ctypes.c_int), ("compact", ctypes.c_int)]
 
 
 class PprPlanInfo(ctypes.Structure):
@@ -157,6 +158,7 @@ SIGNATURES = {
     "gd_ppr_plan_run": (_i, [_P, _P, _P, ctypes.POINTER(PrOptions), ctypes.POINTER(PrStats), _P,
                              _P]),
     "gd_ppr_plan_info": (_i, [_P, ctypes.POINTER(PprPlanInfo)]),
+    "gd_ppr_plan_column_iterations": (_i, [_P, _P]),
     "gd_ppr_plan_destroy": (_i, [_P]),
     "gd_multi_pr_default_config": (None, [ctypes.POINTER(MultiPrConfig)]),
     "gd_multi_pr_plan_create": (_i, [_i, _P, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),

@@ -63,6 +63,8 @@ struct WprEpilogue {
 };
 
 struct WprPolicy {
+  // as plain PageRank: with unit weights the forward is its arithmetic exactly
+  static constexpr bool kExactHubs = true;
   struct Args {
     float* x;
     float* c0;
@@ -107,6 +109,7 @@ struct AdjEpilogue {
 };
 
 struct AdjPolicy {
+  static constexpr bool kExactHubs = false;
   struct Args {
     float* lam;
     float* c0;
@@ -290,6 +293,7 @@ int gd_wpr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* 
     e->nt = 0;
     e->persistent = false;
   }
+  p->fwd.exact_len = 2048;  // hub rows in edge order, as the plain PageRank plan
   rc = p->fwd.build(m, nnz, d_off, d_col, p->stream);
   if (!rc) rc = p->fwd.setup_grid<WprPolicy>();
   if (!rc) rc = p->adj.build(m, nnz, p->tr.off.as<int32_t>(), p->tr.row.as<int32_t>(), p->stream);

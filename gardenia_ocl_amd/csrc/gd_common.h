@@ -35,6 +35,7 @@ void set_error(const std::string& msg);
 // Work unit of the pull engine (one workgroup each).
 //   row tile:    x = first row, y = end row (exclusive) > x, z = first edge, w = end edge
 //   hub segment: x = row, y = -(partial slot + 1) < 0,     z = first edge, w = end edge
+//   (with exact hubs a hub "segment" is the whole row, summed in edge order)
 struct Tile {
   int32_t x, y, z, w;
 };
@@ -43,7 +44,10 @@ struct Tile {
 // consecutive rows are packed into tiles of at most `tile_nnz` edges and
 // `tile_rows` rows; a row longer than `tile_nnz` becomes a hub and is split
 // into ceil(deg/tile_nnz) segments whose partial sums are combined, in segment
-// order, by a finalize pass.
+// order, by a finalize pass.  With exact_len > 0 (<= tile_nnz) every row longer
+// than exact_len is instead one whole-row hub tile (summed in edge order by one
+// workgroup); the longest of them come first, the rest are spread evenly
+// among the row tiles.
 struct Schedule {
   std::vector<Tile> tiles;
   std::vector<int32_t> hub_row, hub_slot, hub_nseg;
@@ -54,7 +58,7 @@ struct Schedule {
 // Builds the schedule from host row offsets.  Returns GD_ERR_INVALID (and sets
 // the error text) if the offsets are not a valid CSR prefix sum of length m+1.
 int build_schedule(int m, int64_t nnz, const int32_t* row_offsets, int tile_nnz, int tile_rows,
-                   Schedule* out);
+                   Schedule* out, int exact_len = 0);
 
 // Checks on the device that every column index lies in [0, ncols); returns
 // GD_OK, GD_ERR_INVALID or GD_ERR_DEVICE.

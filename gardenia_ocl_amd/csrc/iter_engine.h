@@ -23,6 +23,7 @@ struct IterEngine {
   int ipt = 8;               // edges per thread per tile (8 or 16)
   int nt = 1;                // cache-hint level (see gd_pr_plan_config.nontemporal)
   bool persistent = true;
+  int exact_len = 0;         // > 0: rows longer than this are whole-row exact hub tiles
   int pull_grid = 0;
   const int32_t* off = nullptr;
   const int32_t* col = nullptr;
@@ -58,7 +59,7 @@ struct IterEngine {
     GD_HIP_TRY(hipMemcpyAsync(h_off.data(), off, 4 * ((size_t)m + 1), hipMemcpyDeviceToHost,
                               stream));
     GD_HIP_TRY(hipStreamSynchronize(stream));
-    rc = build_schedule(m, nnz, h_off.data(), kBlock * ipt, kTileRows, &sched);
+    rc = build_schedule(m, nnz, h_off.data(), kBlock * ipt, kTileRows, &sched, exact_len);
     if (rc) return rc;
     n_tiles = (int)sched.tiles.size();
     n_hubs = (int)sched.hub_row.size();
@@ -105,9 +106,11 @@ struct IterEngine {
                     err_part.as<double>(), state.as<IterState>()};
   }
 
+  // exact hubs finish their rows in the pull launch: no hub epilogues here
   FinArgs fin_args() {
-    return FinArgs{hub_row.as<int32_t>(), hub_slot.as<int32_t>(), hub_nseg.as<int32_t>(), n_hubs,
-                   pull_grid, errors.as<double>(), fin_part.as<double>()};
+    return FinArgs{hub_row.as<int32_t>(), hub_slot.as<int32_t>(), hub_nseg.as<int32_t>(),
+                   exact_len > 0 ? 0 : n_hubs, pull_grid, errors.as<double>(),
+                   fin_part.as<double>()};
   }
 
   template <class P, int IPT, int NT>

@@ -6,10 +6,10 @@
 // schedule (the same tiles a single-graph plan would build, so results are
 // bit-identical to solving each graph alone), its own base score (1-d)/m_g,
 // its own iteration counter / parity / convergence flag.  One pull launch
-// covers the tiles of every graph (tiles of converged graphs exit at once),
-// then one finalize workgroup per graph adds its hub rows and its tiles'
-// L1 partials in a fixed order and updates that graph's state -- no
-// cross-workgroup synchronisation.  A chunk of iterations (pull + finalize
+// covers the tiles of every graph (tiles of converged graphs exit at once;
+// rows of > 2048 edges are whole-row tiles summed in the reference's order),
+// then one finalize workgroup per graph adds its tiles' L1 partials in a fixed
+// order and updates that graph's state -- no cross-workgroup synchronisation.  A chunk of iterations (pull + finalize
 // launches with identical arguments) is captured once into a hipGraph and
 // replayed until every graph has converged; small graphs are launch-bound, so
 // this is where the time goes.
@@ -67,38 +67,31 @@ __global__ __launch_bounds__(kBlock) void mg_pull_kernel(const MgArgs a) {
   const int odd = a.gs[g].iter & 1;
   __shared__ RowTileLds<kMgIPT> lds;
   __shared__ double s_err[kWaves];
+  __shared__ ExactLds xs;
   const Tile t = a.tiles[blockIdx.x];
   const PrGather gat{odd ? a.c1 : a.c0};
   double err = 0.0;
+  const PrEpilogue<0> e{a.scores, odd ? a.c0 : a.c1, a.outdeg, a.gs[g].base, a.damp, -1};
   if (t.y >= 0) {
-    const PrEpilogue<0> e{a.scores, odd ? a.c0 : a.c1, a.outdeg, a.gs[g].base, a.damp, -1};
     err = row_tile<kMgIPT, false>(a.off, a.col, a.e_max, t.x, t.y, t.z, t.w - t.z, gat, e, lds);
-  } else {
-    const float p = segment_sum<kMgIPT, false>(a.col, t.z, t.w - t.z, gat, lds.val);
-    if (threadIdx.x == 0) a.hub_part[-t.y - 1] = p;
+  } else {  // whole hub row in the reference's order, as the single-graph plan
+    const auto pre = e.pre(t.x);
+    const float sum = exact_seq_sum<kMgIPT, false>(a.col, a.e_max, t.z, t.w - t.z, gat,
+                                                   e.init(pre), lds.val, xs);
+    if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
   }
   const double tot = block_sum<kWaves>(err, s_err);
   if (threadIdx.x == 0) a.err_part[blockIdx.x] = tot;
 }
 
-// One workgroup per graph: hub rows, L1 over the graph's tiles (fixed order),
-// state update.
+// One workgroup per graph: L1 over the graph's tiles (fixed order; hub rows
+// were finished by their pull workgroups), state update.
 __global__ __launch_bounds__(kBlock) void mg_finalize_kernel(const MgArgs a) {
   const int g = blockIdx.x;
   GraphState& st = a.gs[g];
   if (st.done) return;
   __shared__ double s_err[kWaves];
-  const int odd = st.iter & 1;
-  const PrEpilogue<0> e{a.scores, odd ? a.c0 : a.c1, a.outdeg, st.base, a.damp, -1};
   double err = 0.0;
-  for (int h = st.hub_lo + threadIdx.x; h < st.hub_hi; h += kBlock) {
-    const int v = a.hub_row[h];
-    const auto p = e.pre(v);
-    const float* part = a.hub_part + a.hub_slot[h];
-    float s = 0.0f;
-    for (int k = 0; k < a.hub_nseg[h]; k++) s += part[k];
-    err += e.finish(v, s, p);
-  }
   for (int t = st.tile_lo + threadIdx.x; t < st.tile_hi; t += kBlock) err += a.err_part[t];
   const double tot = block_sum<kWaves>(err, s_err);
   if (threadIdx.x == 0) {
@@ -241,7 +234,8 @@ int gd_multi_pr_plan_create(int n_graphs, const int32_t* m, const int64_t* nnz,
                               hipMemcpyDeviceToHost, p->stream));
     GD_HIP_TRY(hipStreamSynchronize(p->stream));
     Schedule s;
-    rc = build_schedule(m[g], nnz[g], h_off.data(), kBlock * kMgIPT, kTileRows, &s);
+    rc = build_schedule(m[g], nnz[g], h_off.data(), kBlock * kMgIPT, kTileRows, &s,
+                        kBlock * kMgIPT);  // rows > 2048 edges: whole-row exact hubs
     if (rc) return fail(rc);
     hipLaunchKernelGGL(mg_pack_kernel, dim3(grid_for(std::max<int64_t>(nnz[g], m[g]))),
                        dim3(256), 0, p->stream, d_off[g], d_col[g], d_outdeg[g], m[g], nnz[g], vb,

@@ -16,6 +16,14 @@
 // per-column L1 partials for the whole sweep, and two small kernels reduce
 // them (deterministically) to per-column L1 values, their maximum and the
 // convergence flag.
+//
+// Columns stop one by one: after the first iteration whose L1 change of
+// column j is below tol, column j is frozen (its scores are final, exactly as
+// a single-seed run would stop).  Frozen columns keep their values; between
+// host checks the running columns are moved to the front of every vertex's
+// k-vector (an in-place column permutation of X and the gathered contrib), so
+// the lanes of frozen columns stop gathering.  The permutation is undone when
+// the run ends.
 
 #include <hip/hip_runtime.h>
 
@@ -106,9 +114,11 @@ __device__ __forceinline__ void sum_edges(const int32_t* __restrict__ col,
   }
 }
 
+// Frozen columns (fz) keep their score and add nothing to the L1 change.
 template <int VEC>
 __device__ __forceinline__ void row_update(int v, const float (&acc)[VEC], const int (&seed)[VEC],
-                                           float* __restrict__ X, float* __restrict__ Cn,
+                                           const int (&fz)[VEC], float* __restrict__ X,
+                                           float* __restrict__ Cn,
                                            const int32_t* __restrict__ outdeg, int64_t k,
                                            int col0, float teleport, float damp,
                                            double (&err)[VEC]) {
@@ -119,9 +129,10 @@ __device__ __forceinline__ void row_update(int v, const float (&acc)[VEC], const
 #pragma unroll
   for (int j = 0; j < VEC; j++) {
     const float b = (seed[j] == v) ? teleport : 0.0f;
-    nx[j] = b + damp * acc[j];
+    const float upd = b + damp * acc[j];
+    nx[j] = fz[j] ? old.v[j] : upd;
     nc[j] = nx[j] / deg;
-    err[j] += (double)fabsf(nx[j] - old.v[j]);
+    err[j] += fz[j] ? 0.0 : (double)fabsf(nx[j] - old.v[j]);
   }
   store_vec<VEC>(X + o, nx);
   store_vec<VEC>(Cn + o, nc);
@@ -131,10 +142,10 @@ template <int VEC>
 __global__ __launch_bounds__(kPBlock) void ppr_pull_kernel(
     const Tile* __restrict__ tiles, int n_tiles, const int32_t* __restrict__ off,
     const int32_t* __restrict__ col, const int32_t* __restrict__ outdeg,
-    const int32_t* __restrict__ seeds, int64_t k, float* __restrict__ X,
-    float* __restrict__ C0, float* __restrict__ C1, float* __restrict__ seg_part,
-    double* __restrict__ err_part, const PprState* __restrict__ st, float teleport,
-    float damp) {
+    const int32_t* __restrict__ seeds, const int32_t* __restrict__ frozen, int ka, int64_t k,
+    float* __restrict__ X, float* __restrict__ C0, float* __restrict__ C1,
+    float* __restrict__ seg_part, double* __restrict__ err_part,
+    const PprState* __restrict__ st, float teleport, float damp) {
   if (st->done) return;
   const int odd = st->iter & 1;
   const float* __restrict__ C = odd ? C1 : C0;
@@ -142,29 +153,34 @@ __global__ __launch_bounds__(kPBlock) void ppr_pull_kernel(
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int col0 = blockIdx.y * 64 * VEC + lane * VEC;
-  int seed[VEC];
+  int seed[VEC], fz[VEC];
 #pragma unroll
-  for (int j = 0; j < VEC; j++) seed[j] = seeds[col0 + j];
+  for (int j = 0; j < VEC; j++) {
+    seed[j] = seeds[col0 + j];
+    fz[j] = frozen[col0 + j];
+  }
   double err[VEC];
 #pragma unroll
   for (int j = 0; j < VEC; j++) err[j] = 0.0;
   const int nw = gridDim.x * kPWaves;
-  for (int t = blockIdx.x * kPWaves + wid; t < n_tiles; t += nw) {
-    const Tile tl = tiles[t];
-    if (tl.y >= 0) {
-      for (int r = tl.x; r < tl.y; r++) {
+  if (col0 < ka) {  // lanes past the running columns gather nothing
+    for (int t = blockIdx.x * kPWaves + wid; t < n_tiles; t += nw) {
+      const Tile tl = tiles[t];
+      if (tl.y >= 0) {
+        for (int r = tl.x; r < tl.y; r++) {
+          float acc[VEC];
+#pragma unroll
+          for (int j = 0; j < VEC; j++) acc[j] = 0.0f;
+          sum_edges<VEC>(col, C, k, col0, off[r], off[r + 1], acc);
+          row_update<VEC>(r, acc, seed, fz, X, Cn, outdeg, k, col0, teleport, damp, err);
+        }
+      } else {
         float acc[VEC];
 #pragma unroll
         for (int j = 0; j < VEC; j++) acc[j] = 0.0f;
-        sum_edges<VEC>(col, C, k, col0, off[r], off[r + 1], acc);
-        row_update<VEC>(r, acc, seed, X, Cn, outdeg, k, col0, teleport, damp, err);
+        sum_edges<VEC>(col, C, k, col0, tl.z, tl.w, acc);
+        store_vec<VEC>(seg_part + (int64_t)(-tl.y - 1) * k + col0, acc);
       }
-    } else {
-      float acc[VEC];
-#pragma unroll
-      for (int j = 0; j < VEC; j++) acc[j] = 0.0f;
-      sum_edges<VEC>(col, C, k, col0, tl.z, tl.w, acc);
-      store_vec<VEC>(seg_part + (int64_t)(-tl.y - 1) * k + col0, acc);
     }
   }
   // workgroup partial per column: waves in order, through LDS
@@ -185,32 +201,38 @@ template <int VEC>
 __global__ __launch_bounds__(kPBlock) void ppr_hub_kernel(
     const int32_t* __restrict__ hub_row, const int32_t* __restrict__ hub_slot,
     const int32_t* __restrict__ hub_nseg, int n_hubs, const float* __restrict__ seg_part,
-    const int32_t* __restrict__ outdeg, const int32_t* __restrict__ seeds, int64_t k,
-    float* __restrict__ X, float* __restrict__ C0, float* __restrict__ C1,
-    double* __restrict__ hub_err, const PprState* __restrict__ st, float teleport, float damp) {
+    const int32_t* __restrict__ outdeg, const int32_t* __restrict__ seeds,
+    const int32_t* __restrict__ frozen, int ka, int64_t k, float* __restrict__ X,
+    float* __restrict__ C0, float* __restrict__ C1, double* __restrict__ hub_err,
+    const PprState* __restrict__ st, float teleport, float damp) {
   if (st->done) return;
   const int odd = st->iter & 1;
   float* __restrict__ Cn = odd ? C0 : C1;
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int col0 = blockIdx.y * 64 * VEC + lane * VEC;
-  int seed[VEC];
+  int seed[VEC], fz[VEC];
 #pragma unroll
-  for (int j = 0; j < VEC; j++) seed[j] = seeds[col0 + j];
+  for (int j = 0; j < VEC; j++) {
+    seed[j] = seeds[col0 + j];
+    fz[j] = frozen[col0 + j];
+  }
   double err[VEC];
 #pragma unroll
   for (int j = 0; j < VEC; j++) err[j] = 0.0;
-  for (int h = blockIdx.x * kPWaves + wid; h < n_hubs; h += gridDim.x * kPWaves) {
-    float acc[VEC];
+  if (col0 < ka) {
+    for (int h = blockIdx.x * kPWaves + wid; h < n_hubs; h += gridDim.x * kPWaves) {
+      float acc[VEC];
 #pragma unroll
-    for (int j = 0; j < VEC; j++) acc[j] = 0.0f;
-    const int s0 = hub_slot[h], ns = hub_nseg[h];
-    for (int q = 0; q < ns; q++) {
-      const Vec<VEC> p = load_vec<VEC>(seg_part + (int64_t)(s0 + q) * k + col0);
+      for (int j = 0; j < VEC; j++) acc[j] = 0.0f;
+      const int s0 = hub_slot[h], ns = hub_nseg[h];
+      for (int q = 0; q < ns; q++) {
+        const Vec<VEC> p = load_vec<VEC>(seg_part + (int64_t)(s0 + q) * k + col0);
 #pragma unroll
-      for (int j = 0; j < VEC; j++) acc[j] += p.v[j];
+        for (int j = 0; j < VEC; j++) acc[j] += p.v[j];
+      }
+      row_update<VEC>(hub_row[h], acc, seed, fz, X, Cn, outdeg, k, col0, teleport, damp, err);
     }
-    row_update<VEC>(hub_row[h], acc, seed, X, Cn, outdeg, k, col0, teleport, damp, err);
   }
   __shared__ double s_err[kPWaves][64 * VEC];
 #pragma unroll
@@ -227,11 +249,12 @@ __global__ __launch_bounds__(kPBlock) void ppr_hub_kernel(
 // First level: fin[s][c] = sum over the rows of err_part in slice s (fixed order).
 __global__ __launch_bounds__(kPBlock) void ppr_err_slices_kernel(const double* __restrict__ part,
                                                                  int rows, int64_t k,
+                                                                 int64_t kact,
                                                                  double* __restrict__ fin,
                                                                  const PprState* __restrict__ st) {
   if (st->done) return;
   const int64_t c = (int64_t)blockIdx.x * kPBlock + threadIdx.x;
-  if (c >= k) return;
+  if (c >= kact) return;
   const int per = (rows + kSlices - 1) / kSlices;
   const int lo = blockIdx.y * per, hi = min(lo + per, rows);
   double s = 0.0;
@@ -239,20 +262,26 @@ __global__ __launch_bounds__(kPBlock) void ppr_err_slices_kernel(const double* _
   fin[(int64_t)blockIdx.y * k + c] = s;
 }
 
-// Second level: per-column L1 (slices in order), max over columns, state update.
-__global__ __launch_bounds__(kPBlock) void ppr_finalize_kernel(const double* __restrict__ fin,
-                                                               int64_t k,
-                                                               double* __restrict__ col_err,
-                                                               PprState* __restrict__ st,
-                                                               double* __restrict__ errors) {
+// Second level: per-column L1 (slices in order) of the columns still running
+// (c < kact, not frozen), their maximum, the per-column stop and the state
+// update.  With column_stop a column whose L1 change is below tol is frozen.
+__global__ __launch_bounds__(kPBlock) void ppr_finalize_kernel(
+    const double* __restrict__ fin, int64_t k, int64_t kact, double* __restrict__ col_err,
+    int32_t* __restrict__ frozen, int32_t* __restrict__ col_iters, int column_stop,
+    PprState* __restrict__ st, double* __restrict__ errors) {
   if (st->done) return;
   __shared__ double s_max[kPBlock];
+  const int it = st->iter;
+  const double tol = st->tol;
   double mx = 0.0;
-  for (int64_t c = threadIdx.x; c < k; c += kPBlock) {
+  for (int64_t c = threadIdx.x; c < kact; c += kPBlock) {
+    if (frozen[c]) continue;
     double s = 0.0;
     for (int q = 0; q < 2 * kSlices; q++) s += fin[(int64_t)q * k + c];
     col_err[c] = s;
+    col_iters[c] = it + 1;
     mx = fmax(mx, s);
+    if (column_stop && s < tol) frozen[c] = 1;
   }
   s_max[threadIdx.x] = mx;
   __syncthreads();
@@ -261,10 +290,45 @@ __global__ __launch_bounds__(kPBlock) void ppr_finalize_kernel(const double* __r
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const int it = st->iter;
     errors[it] = s_max[0];
     st->iter = it + 1;
-    if (s_max[0] < st->tol || it + 1 >= st->max_iter) st->done = 1;
+    // every running column below tol <=> the max over them is
+    if (s_max[0] < tol || it + 1 >= st->max_iter) st->done = 1;
+  }
+}
+
+__global__ void ppr_reset_cols_kernel(int k, int32_t* __restrict__ frozen,
+                                      int32_t* __restrict__ col_iters) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < k) {
+    frozen[j] = 0;
+    col_iters[j] = 0;
+  }
+}
+
+// In-place column permutation of the rows of A (and of B when not NULL):
+// row[c] <- row[newcol[c]] for c < kact, newcol a permutation of [0, kact).
+// One wavefront per row through kact floats of LDS; every workgroup runs the
+// same number of loop trips, so the barriers are uniform.
+__global__ __launch_bounds__(kPBlock) void ppr_permute_cols_kernel(int m, int64_t k, int kact,
+                                                                   const int32_t* __restrict__ newcol,
+                                                                   float* __restrict__ A,
+                                                                   float* __restrict__ B) {
+  extern __shared__ float s_rows[];  // kPWaves * kact
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  float* my = s_rows + (size_t)wid * kact;
+  const int passes = B ? 2 : 1;
+  for (int base = blockIdx.x * kPWaves; base < m; base += gridDim.x * kPWaves) {
+    const int v = base + wid;
+    for (int pass = 0; pass < passes; pass++) {
+      float* row = (pass ? B : A) + (int64_t)min(v, m - 1) * k;
+      if (v < m)
+        for (int c = lane; c < kact; c += 64) my[c] = row[c];
+      __syncthreads();
+      if (v < m)
+        for (int c = lane; c < kact; c += 64) row[c] = my[newcol[c]];
+      __syncthreads();
+    }
   }
 }
 
@@ -282,8 +346,10 @@ __global__ void ppr_init_kernel(int64_t n, int64_t k, const float* __restrict__ 
   }
 }
 
-// rows of X between the user's vertex order and the plan's
+// rows of X between the user's vertex order and the plan's; on the way back
+// (cperm not NULL) internal column j goes to user column cperm[j]
 __global__ void ppr_permute_rows_kernel(int m, int64_t k, const int32_t* __restrict__ perm,
+                                        const int32_t* __restrict__ cperm,
                                         const float* __restrict__ src, float* __restrict__ dst,
                                         int to_internal) {
   const int lane = threadIdx.x & 63;
@@ -292,7 +358,7 @@ __global__ void ppr_permute_rows_kernel(int m, int64_t k, const int32_t* __restr
     const int v = perm[i];
     const float* s = src + (int64_t)(to_internal ? v : i) * k;
     float* d = dst + (int64_t)(to_internal ? i : v) * k;
-    for (int64_t j = lane; j < k; j += 64) d[j] = s[j];
+    for (int64_t j = lane; j < k; j += 64) d[cperm ? cperm[j] : j] = s[j];
   }
 }
 
@@ -326,8 +392,11 @@ struct gd_ppr_plan {
   Schedule sched;
   int n_tiles = 0, n_hubs = 0, grid = 0, hub_grid = 0;
   DeviceBuffer tiles, hub_row, hub_slot, hub_nseg, seg_part, err_part, hub_err, fin, col_err,
-      C0, C1, Xint, seeds, state, errors;
+      C0, C1, Xint, seeds, state, errors, frozen, col_iters, newcol, cperm;
   int errors_cap = 0;
+  bool column_stop = true;
+  bool compact = true;
+  std::vector<int32_t> last_col_iters;  // user column order
   float plan_ms = 0.0f;
 };
 
@@ -408,6 +477,10 @@ int ppr_build(gd_ppr_plan* p, const int32_t* d_off, const int32_t* d_col,
   if (!rc) rc = p->C1.alloc(4 * mk);
   if (!rc && p->relabel) rc = p->Xint.alloc(4 * mk);
   if (!rc) rc = p->seeds.alloc(4 * (size_t)p->k);
+  if (!rc) rc = p->frozen.alloc(4 * (size_t)p->k);
+  if (!rc) rc = p->col_iters.alloc(4 * (size_t)p->k);
+  if (!rc) rc = p->newcol.alloc(4 * (size_t)p->k);
+  if (!rc) rc = p->cperm.alloc(4 * (size_t)p->k);
   if (!rc) rc = p->state.alloc(sizeof(PprState));
   if (rc) return rc;
   GD_HIP_TRY(hipMemcpyAsync(p->tiles.ptr, s.tiles.data(), sizeof(Tile) * p->n_tiles,
@@ -425,30 +498,93 @@ int ppr_build(gd_ppr_plan* p, const int32_t* d_off, const int32_t* d_col,
   return GD_OK;
 }
 
+// One iteration over the running columns [0, ka) (whole chunks of 64*VEC).
 template <int VEC>
-void ppr_launch_iteration(gd_ppr_plan* p, float* X, PprState* st, float teleport, float damp) {
-  const int chunks = p->k / (64 * VEC);
+void ppr_launch_iteration(gd_ppr_plan* p, float* X, PprState* st, float teleport, float damp,
+                          int ka) {
+  const int kc = 64 * VEC;
+  const int chunks = (ka + kc - 1) / kc;
+  const int64_t kact = (int64_t)chunks * kc;
   float* C0 = p->C0.as<float>();
   float* C1 = p->C1.as<float>();
+  const int32_t* frozen = p->frozen.as<int32_t>();
   hipLaunchKernelGGL(ppr_pull_kernel<VEC>, dim3(p->grid, chunks), dim3(kPBlock), 0, p->stream,
                      p->tiles.as<Tile>(), p->n_tiles, p->off, p->col, p->outdeg,
-                     p->seeds.as<int32_t>(), (int64_t)p->k, X, C0, C1, p->seg_part.as<float>(),
-                     p->err_part.as<double>(), st, teleport, damp);
+                     p->seeds.as<int32_t>(), frozen, ka, (int64_t)p->k, X, C0, C1,
+                     p->seg_part.as<float>(), p->err_part.as<double>(), st, teleport, damp);
   if (p->n_hubs)
     hipLaunchKernelGGL(ppr_hub_kernel<VEC>, dim3(p->hub_grid, chunks), dim3(kPBlock), 0,
                        p->stream, p->hub_row.as<int32_t>(), p->hub_slot.as<int32_t>(),
                        p->hub_nseg.as<int32_t>(), p->n_hubs, p->seg_part.as<float>(), p->outdeg,
-                       p->seeds.as<int32_t>(), (int64_t)p->k, X, C0, C1,
+                       p->seeds.as<int32_t>(), frozen, ka, (int64_t)p->k, X, C0, C1,
                        p->hub_err.as<double>(), st, teleport, damp);
-  const dim3 sg((p->k + kPBlock - 1) / kPBlock, kSlices);
+  const dim3 sg((unsigned)((kact + kPBlock - 1) / kPBlock), kSlices);
   double* fin = p->fin.as<double>();
   hipLaunchKernelGGL(ppr_err_slices_kernel, sg, dim3(kPBlock), 0, p->stream,
-                     p->err_part.as<double>(), p->grid, (int64_t)p->k, fin, st);
+                     p->err_part.as<double>(), p->grid, (int64_t)p->k, kact, fin, st);
   hipLaunchKernelGGL(ppr_err_slices_kernel, sg, dim3(kPBlock), 0, p->stream,
-                     p->hub_err.as<double>(), p->hub_grid, (int64_t)p->k,
+                     p->hub_err.as<double>(), p->hub_grid, (int64_t)p->k, kact,
                      fin + (size_t)kSlices * p->k, st);
   hipLaunchKernelGGL(ppr_finalize_kernel, dim3(1), dim3(kPBlock), 0, p->stream, fin,
-                     (int64_t)p->k, p->col_err.as<double>(), st, p->errors.as<double>());
+                     (int64_t)p->k, kact, p->col_err.as<double>(), p->frozen.as<int32_t>(),
+                     p->col_iters.as<int32_t>(), p->column_stop ? 1 : 0, st,
+                     p->errors.as<double>());
+}
+
+void ppr_launch(gd_ppr_plan* p, float* X, PprState* st, float teleport, float damp, int ka) {
+  if (p->kc == 256)
+    ppr_launch_iteration<4>(p, X, st, teleport, damp, ka);
+  else if (p->kc == 128)
+    ppr_launch_iteration<2>(p, X, st, teleport, damp, ka);
+  else
+    ppr_launch_iteration<1>(p, X, st, teleport, damp, ka);
+}
+
+// Applies `order` (new column c <- old column order[c], c < ka) to the rows of
+// X and of the contrib buffer the next iteration gathers, and to the k-length
+// per-column arrays; cperm (internal -> user column) follows.
+int ppr_compact(gd_ppr_plan* p, float* X, float* Cnext, int ka, const std::vector<int32_t>& order,
+                std::vector<int32_t>& cperm) {
+  GD_HIP_TRY(hipMemcpyAsync(p->newcol.ptr, order.data(), 4 * (size_t)ka, hipMemcpyHostToDevice,
+                            p->stream));
+  const int m = p->m;
+  const int grid = std::max(1, std::min(16384, (m + kPWaves - 1) / kPWaves));
+  hipLaunchKernelGGL(ppr_permute_cols_kernel, dim3(grid), dim3(kPBlock),
+                     (size_t)kPWaves * ka * sizeof(float), p->stream, m, (int64_t)p->k, ka,
+                     p->newcol.as<int32_t>(), X, Cnext);
+  GD_HIP_TRY(hipGetLastError());
+  std::vector<int32_t> seeds(ka), fz(ka), its(ka);
+  std::vector<double> ce(ka);
+  GD_HIP_TRY(hipMemcpyAsync(seeds.data(), p->seeds.ptr, 4 * (size_t)ka, hipMemcpyDeviceToHost,
+                            p->stream));
+  GD_HIP_TRY(hipMemcpyAsync(fz.data(), p->frozen.ptr, 4 * (size_t)ka, hipMemcpyDeviceToHost,
+                            p->stream));
+  GD_HIP_TRY(hipMemcpyAsync(its.data(), p->col_iters.ptr, 4 * (size_t)ka, hipMemcpyDeviceToHost,
+                            p->stream));
+  GD_HIP_TRY(hipMemcpyAsync(ce.data(), p->col_err.ptr, 8 * (size_t)ka, hipMemcpyDeviceToHost,
+                            p->stream));
+  GD_HIP_TRY(hipStreamSynchronize(p->stream));
+  std::vector<int32_t> seeds2(ka), fz2(ka), its2(ka), cp2(ka);
+  std::vector<double> ce2(ka);
+  for (int c = 0; c < ka; c++) {
+    const int o = order[c];
+    seeds2[c] = seeds[o];
+    fz2[c] = fz[o];
+    its2[c] = its[o];
+    ce2[c] = ce[o];
+    cp2[c] = cperm[o];
+  }
+  for (int c = 0; c < ka; c++) cperm[c] = cp2[c];
+  GD_HIP_TRY(hipMemcpyAsync(p->seeds.ptr, seeds2.data(), 4 * (size_t)ka, hipMemcpyHostToDevice,
+                            p->stream));
+  GD_HIP_TRY(hipMemcpyAsync(p->frozen.ptr, fz2.data(), 4 * (size_t)ka, hipMemcpyHostToDevice,
+                            p->stream));
+  GD_HIP_TRY(hipMemcpyAsync(p->col_iters.ptr, its2.data(), 4 * (size_t)ka, hipMemcpyHostToDevice,
+                            p->stream));
+  GD_HIP_TRY(hipMemcpyAsync(p->col_err.ptr, ce2.data(), 8 * (size_t)ka, hipMemcpyHostToDevice,
+                            p->stream));
+  GD_HIP_TRY(hipStreamSynchronize(p->stream));  // host vectors go out of scope
+  return GD_OK;
 }
 
 }  // namespace
@@ -458,6 +594,8 @@ extern "C" {
 void gd_ppr_default_plan_config(gd_ppr_plan_config* c) {
   c->relabel = -1;
   c->k_chunk = 0;
+  c->column_stop = -1;
+  c->compact = -1;
 }
 
 int gd_ppr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* d_col,
@@ -488,6 +626,9 @@ int gd_ppr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* 
   p->kc = c.k_chunk ? c.k_chunk : (k % 256 == 0 ? 256 : (k % 128 == 0 ? 128 : 64));
   p->stream = (hipStream_t)stream;
   p->relabel = (c.relabel == 1) || (c.relabel == -1 && nnz >= (1 << 20));
+  p->column_stop = c.column_stop != 0;
+  // LDS of the in-place column permutation: kPWaves rows of k floats <= 64 KB
+  p->compact = p->column_stop && c.compact != 0 && k <= 4096;
   int rc = GD_OK;
   if (m > 0) {
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -519,6 +660,13 @@ int gd_ppr_plan_info(const gd_ppr_plan* p, gd_ppr_plan_info_t* info) {
   info->relabel = p->relabel ? 1 : 0;
   info->grid = p->grid;
   info->plan_ms = p->plan_ms;
+  return GD_OK;
+}
+
+int gd_ppr_plan_column_iterations(const gd_ppr_plan* p, int32_t* iterations) {
+  GD_CHECK_ARG(p != nullptr && iterations != nullptr, "NULL plan / output");
+  for (int j = 0; j < p->k; j++)
+    iterations[j] = j < (int)p->last_col_iters.size() ? p->last_col_iters[j] : 0;
   return GD_OK;
 }
 
@@ -564,59 +712,101 @@ int gd_ppr_plan_run(gd_ppr_plan* p, const int32_t* d_seeds, float* d_X,
   }
   const float damp = o.damping, teleport = 1.0f - damp;
   const int chunk = o.chunk > 0 ? o.chunk : (o.tol > 0 ? 4 : std::max(1, o.max_iter));
+  const int k = p->k;
   hipEvent_t t0, t1;
   GD_HIP_TRY(hipEventCreate(&t0));
   GD_HIP_TRY(hipEventCreate(&t1));
   PprState* st = p->state.as<PprState>();
   float* X = p->relabel ? p->Xint.as<float>() : d_X;
-  const int64_t n = (int64_t)m * p->k;
+  const int64_t n = (int64_t)m * k;
   GD_HIP_TRY(hipEventRecord(t0, p->stream));
-  hipLaunchKernelGGL(ppr_map_seeds_kernel, dim3((p->k + 255) / 256), dim3(256), 0, p->stream,
-                     p->k, d_seeds, p->relabel ? p->rl.inv.as<int32_t>() : nullptr,
+  hipLaunchKernelGGL(ppr_map_seeds_kernel, dim3((k + 255) / 256), dim3(256), 0, p->stream, k,
+                     d_seeds, p->relabel ? p->rl.inv.as<int32_t>() : nullptr,
                      p->seeds.as<int32_t>());
   if (p->relabel)
     hipLaunchKernelGGL(ppr_permute_rows_kernel, dim3(std::min(16384, (m + 3) / 4)), dim3(256), 0,
-                       p->stream, m, (int64_t)p->k, p->rl.perm.as<int32_t>(), d_X, X, 1);
+                       p->stream, m, (int64_t)k, p->rl.perm.as<int32_t>(), nullptr, d_X, X, 1);
   hipLaunchKernelGGL(ppr_init_kernel, dim3((int)std::min<int64_t>((n + 255) / 256, 65536)),
-                     dim3(256), 0, p->stream, n, (int64_t)p->k, X, p->outdeg,
-                     p->C0.as<float>(), st, o.max_iter, o.tol);
+                     dim3(256), 0, p->stream, n, (int64_t)k, X, p->outdeg, p->C0.as<float>(),
+                     st, o.max_iter, o.tol);
+  hipLaunchKernelGGL(ppr_reset_cols_kernel, dim3((k + 255) / 256), dim3(256), 0, p->stream, k,
+                     p->frozen.as<int32_t>(), p->col_iters.as<int32_t>());
   GD_HIP_TRY(hipGetLastError());
+  std::vector<int32_t> cperm(k);  // internal column -> user column
+  for (int j = 0; j < k; j++) cperm[j] = j;
+  int ka = k;  // columns [0, ka) may still be running; the rest are frozen
   int launched = 0;
   PprState h{};
+  std::vector<int32_t> fz;
   while (launched < o.max_iter) {
     const int nl = std::min(chunk, o.max_iter - launched);
-    for (int q = 0; q < nl; q++) {
-      if (p->kc == 256)
-        ppr_launch_iteration<4>(p, X, st, teleport, damp);
-      else if (p->kc == 128)
-        ppr_launch_iteration<2>(p, X, st, teleport, damp);
-      else
-        ppr_launch_iteration<1>(p, X, st, teleport, damp);
-    }
+    for (int q = 0; q < nl; q++) ppr_launch(p, X, st, teleport, damp, ka);
     GD_HIP_TRY(hipGetLastError());
     launched += nl;
     GD_HIP_TRY(hipMemcpyAsync(&h, st, sizeof(PprState), hipMemcpyDeviceToHost, p->stream));
     GD_HIP_TRY(hipStreamSynchronize(p->stream));
     if (h.done) break;
+    if (!p->compact) continue;
+    fz.resize(ka);
+    GD_HIP_TRY(hipMemcpy(fz.data(), p->frozen.ptr, 4 * (size_t)ka, hipMemcpyDeviceToHost));
+    int nf = 0;
+    for (int c = 0; c < ka; c++) nf += fz[c] != 0;
+    // a permutation costs about one pass over X and the contrib; it pays once a
+    // few iterations skip the frozen columns' gathers
+    if (nf < std::max(32, ka / 8) || nf >= ka) continue;
+    std::vector<int32_t> order;
+    order.reserve(ka);
+    for (int c = 0; c < ka; c++)
+      if (!fz[c]) order.push_back(c);
+    for (int c = 0; c < ka; c++)
+      if (fz[c]) order.push_back(c);
+    float* Cnext = (h.iter & 1) ? p->C1.as<float>() : p->C0.as<float>();
+    const int rc = ppr_compact(p, X, Cnext, ka, order, cperm);
+    if (rc) return rc;
+    ka -= nf;
   }
-  if (p->relabel)
+  // back to the user's vertex and column order
+  const bool permuted = ka < k;
+  if (permuted)
+    GD_HIP_TRY(hipMemcpyAsync(p->cperm.ptr, cperm.data(), 4 * (size_t)k, hipMemcpyHostToDevice,
+                              p->stream));
+  if (p->relabel) {
     hipLaunchKernelGGL(ppr_permute_rows_kernel, dim3(std::min(16384, (m + 3) / 4)), dim3(256), 0,
-                       p->stream, m, (int64_t)p->k, p->rl.perm.as<int32_t>(), X, d_X, 0);
+                       p->stream, m, (int64_t)k, p->rl.perm.as<int32_t>(),
+                       permuted ? p->cperm.as<int32_t>() : nullptr, X, d_X, 0);
+  } else if (permuted) {
+    std::vector<int32_t> inv(k);  // user column u <- internal column inv[u]
+    for (int j = 0; j < k; j++) inv[cperm[j]] = j;
+    GD_HIP_TRY(hipMemcpyAsync(p->newcol.ptr, inv.data(), 4 * (size_t)k, hipMemcpyHostToDevice,
+                              p->stream));
+    const int grid = std::max(1, std::min(16384, (m + kPWaves - 1) / kPWaves));
+    hipLaunchKernelGGL(ppr_permute_cols_kernel, dim3(grid), dim3(kPBlock),
+                       (size_t)kPWaves * k * sizeof(float), p->stream, m, (int64_t)k, k,
+                       p->newcol.as<int32_t>(), X, (float*)nullptr);
+    GD_HIP_TRY(hipStreamSynchronize(p->stream));  // inv goes out of scope
+  }
+  GD_HIP_TRY(hipGetLastError());
   GD_HIP_TRY(hipEventRecord(t1, p->stream));
   GD_HIP_TRY(hipEventSynchronize(t1));
   GD_HIP_TRY(hipEventElapsedTime(&s.solve_ms, t0, t1));
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   s.iterations = h.iter;
-  std::vector<double> errs(std::max(1, h.iter));
+  std::vector<int32_t> its(k);
+  std::vector<double> ce(k);
+  GD_HIP_TRY(hipMemcpy(its.data(), p->col_iters.ptr, 4 * (size_t)k, hipMemcpyDeviceToHost));
+  GD_HIP_TRY(hipMemcpy(ce.data(), p->col_err.ptr, 8 * (size_t)k, hipMemcpyDeviceToHost));
+  p->last_col_iters.assign(k, 0);
+  for (int j = 0; j < k; j++) p->last_col_iters[cperm[j]] = its[j];
   if (h.iter > 0) {
+    std::vector<double> errs(h.iter);
     GD_HIP_TRY(hipMemcpy(errs.data(), p->errors.ptr, 8 * (size_t)h.iter, hipMemcpyDeviceToHost));
     s.final_error = errs[h.iter - 1];
     s.converged = s.final_error < o.tol;
     if (errors)
       for (int i = 0; i < h.iter; i++) errors[i] = errs[i];
     if (col_errors)
-      GD_HIP_TRY(hipMemcpy(col_errors, p->col_err.ptr, 8 * (size_t)p->k, hipMemcpyDeviceToHost));
+      for (int j = 0; j < k; j++) col_errors[cperm[j]] = ce[j];
   }
   if (stats) *stats = s;
   return GD_OK;

@@ -58,6 +58,12 @@ struct gd_pr_plan {
 
 namespace {
 
+// Rows longer than this are whole-row hub tiles summed with exact_seq_sum:
+// the reference's sequential FP32 sum drifts on long rows (3e-4 relative on a
+// 238,668-edge RMAT-24 row), and matching it keeps every score within 1e-5
+// of the reference; shorter rows stay tree-summed (<= 3e-6 difference there).
+constexpr int kExactHubLen = 2048;
+
 int build_plan(gd_pr_plan* p, const int32_t* d_off, const int32_t* d_col,
                const int32_t* d_outdeg) {
   const int m = p->m;
@@ -150,6 +156,8 @@ int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t
   p->eng.ipt = c.items_per_thread ? c.items_per_thread : 16;
   p->eng.nt = c.nontemporal < 0 ? 0 : c.nontemporal;
   p->eng.persistent = c.persistent == 1;
+  // rows longer than 2048 edges are summed in the reference's order and rounding
+  p->eng.exact_len = kExactHubLen;
   int rc = GD_OK;
   if (m > 0) {
     hipEvent_t t0 = nullptr, t1 = nullptr;

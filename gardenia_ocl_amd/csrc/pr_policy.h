@@ -44,6 +44,9 @@ struct PrEpilogue {
 };
 
 struct PrPolicy {
+  // hub rows in the reference's order and rounding: its sequential FP32 sum of
+  // a 10^5-edge row drifts by ~1e-4 relative, which a tree sum does not match
+  static constexpr bool kExactHubs = true;
   struct Args {
     float* scores;
     float* c0;

@@ -129,7 +129,7 @@ __device__ __forceinline__ float segment_sum(const int32_t* __restrict__ col, in
 // LDS of one row tile.
 template <int IPT>
 struct RowTileLds {
-  float val[kBlock * IPT];    // gathered per-edge values
+  float val[kBlock * IPT + kBlock];  // gathered per-edge values (+ padding, exact_seq_sum)
   int off[kTileRows + 1];     // row offsets, rebased to the tile on use
   float longsum[kTileRows];   // wavefront sums of rows longer than kSeqRow
 };
@@ -263,6 +263,188 @@ __device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
   stage_row_tile<IPT>(t, R, v, lds);
   __syncthreads();
   return reduce_row_tile<IPT>(r0, r1 - r0, e, R.pre, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Sequential-order sums of long rows.
+//
+// exact_seq_sum returns s0 + v[0] + v[1] + ... + v[n-1] rounded exactly as the
+// sequential FP32 loop `s = s0; for (i) s += v[i];` rounds it (the reference's
+// per-row loop, src/pr/omp_base.cc:34-37), computed by the whole workgroup:
+//   While the running sum s lies in one binade, 2^k <= |s| < 2^(k+1) with
+//   ulp u = 2^(k-23), and the exact s + v[i] stays strictly inside it, the
+//   rounded step is s + q_i*u with q_i = v[i]/u rounded to the nearest
+//   integer: s advances by an integer number of ulps.  The workgroup scans the
+//   q_i (exact integers in double) and advances s over every step up to the
+//   first one that leaves the binade, rounds a tie (fraction exactly 1/2,
+//   whose rounding depends on the running parity) or starts from a subnormal /
+//   non-finite s; that step is one ordinary FP32 addition, and the scan
+//   restarts after it.  From s == +0 the sum jumps to the first nonzero term.  For sums of positive terms (PageRank) the
+//   restarts are the ~log2(n) binade crossings plus rare ties.
+// ---------------------------------------------------------------------------
+
+constexpr int kExactHead = 256;  // terms of a row added serially before the scan
+
+struct ExactLds {
+  double wsum[kWaves];
+  int wmin[kWaves];
+  float s;
+};
+
+__device__ __forceinline__ double wave_incl_scan(double v) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const double t = __shfl_up(v, o, kWave);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// Exclusive prefix over the workgroup (lanes, then waves) and the total.  The
+// addends are integers below 2^39 in magnitude: exact in any order.
+__device__ __forceinline__ double block_excl_scan(double v, double& total, ExactLds& x) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const double inc = wave_incl_scan(v);
+  if (lane == kWave - 1) x.wsum[wid] = inc;
+  __syncthreads();
+  double before = 0.0, tot = 0.0;
+#pragma unroll
+  for (int w = 0; w < kWaves; w++) {
+    const double ws = x.wsum[w];
+    before += (w < wid) ? ws : 0.0;
+    tot += ws;
+  }
+  __syncthreads();
+  total = tot;
+  return before + (inc - v);
+}
+
+__device__ __forceinline__ int block_min(int v, ExactLds& x) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
+  if (lane == 0) x.wmin[wid] = v;
+  __syncthreads();
+  int r = x.wmin[0];
+#pragma unroll
+  for (int w = 1; w < kWaves; w++) r = min(r, x.wmin[w]);
+  __syncthreads();
+  return r;
+}
+
+// v / u rounded to the nearest integer (clamped to +-2^26: such a step leaves
+// the binade anyway); tie: the fraction is exactly 1/2.
+__device__ __forceinline__ double ulps_of(float v, double inv_u, bool& tie) {
+  const double a = (double)v * inv_u;  // exact: scaling by a power of two
+  const double f = floor(a);
+  const double fr = a - f;
+  tie = (fr == 0.5);
+  return fmin(fmax(f + (fr > 0.5 ? 1.0 : 0.0), -0x1p26), 0x1p26);
+}
+
+// Position of chunk element i in the padded LDS buffer: thread t reads its IPT
+// consecutive elements at stride IPT+1 words (no bank conflicts).
+template <int IPT>
+__device__ __forceinline__ int exact_pos(int i) {
+  return i + i / IPT;
+}
+
+// Sum of the gathered values of edges [e0, e0+ne) in edge order, rounded as the
+// sequential FP32 loop starting from s.  All threads return the same value.
+// `val` holds kBlock*IPT + kBlock floats.
+template <int IPT, bool NT, class G>
+__device__ float exact_seq_sum(const int32_t* __restrict__ col, int e_max, int e0, int ne,
+                               const G& g, float s, float* __restrict__ val, ExactLds& x) {
+  constexpr int C = kBlock * IPT;
+  const int tid = threadIdx.x;
+  for (int c0 = 0; c0 < ne; c0 += C) {
+    const int nc = min(C, ne - c0);
+    int idx[IPT];
+    float v[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; j++)
+      idx[j] = ld_stream<NT>(col + min(e0 + c0 + min(tid + j * kBlock, nc - 1), e_max));
+#pragma unroll
+    for (int j = 0; j < IPT; j++) v[j] = g.value(e0 + c0 + min(tid + j * kBlock, nc - 1), idx[j]);
+    __syncthreads();  // the previous chunk's readers are done with val
+#pragma unroll
+    for (int j = 0; j < IPT; j++)
+      if (tid + j * kBlock < nc) val[exact_pos<IPT>(tid + j * kBlock)] = v[j];
+    __syncthreads();
+    const int lo = tid * IPT, hi = min(lo + IPT, nc);  // this thread's elements
+    int start = 0;
+    if (c0 == 0) {
+      // the first terms move s across a binade at almost every doubling: one
+      // lane adds them in order (the loop itself, no restarts)
+      const int ns = min(nc, kExactHead);
+      if (tid == 0) {
+        float a = s;
+        for (int i = 0; i < ns; i++) a += val[exact_pos<IPT>(i)];
+        x.s = a;
+      }
+      __syncthreads();
+      s = x.s;
+      __syncthreads();
+      start = ns;
+    }
+    while (start < nc) {  // s and start are uniform across the workgroup
+      if (!(fabsf(s) >= 0x1p-125f) || !isfinite(s)) {  // zero, subnormal, inf/NaN
+        if (s == 0.0f && !signbit(s)) {
+          // +0 + (+-0) == +0: jump to the first nonzero term (x != 0: +0 + x == x)
+          int nz = 0x7fffffff;
+          for (int i = max(lo, start); i < hi; i++)
+            if (val[exact_pos<IPT>(i)] != 0.0f) {
+              nz = i;
+              break;
+            }
+          const int jz = block_min(nz, x);
+          if (jz == 0x7fffffff) {
+            start = nc;
+          } else {
+            s = val[exact_pos<IPT>(jz)];
+            start = jz + 1;
+          }
+          continue;
+        }
+        s = s + val[exact_pos<IPT>(start)];
+        start++;
+        continue;
+      }
+      const int ex = ilogbf(s);
+      const double inv_u = ldexp(1.0, 23 - ex), u = ldexp(1.0, ex - 23);
+      const double P = (double)s * inv_u;  // integer, 2^23 <= |P| < 2^24
+      double L = 0.0;
+      for (int i = max(lo, start); i < hi; i++) {
+        bool tie;
+        L += ulps_of(val[exact_pos<IPT>(i)], inv_u, tie);
+      }
+      double total;
+      double run = P + block_excl_scan(L, total, x);
+      int bad = 0x7fffffff;
+      for (int i = max(lo, start); i < hi; i++) {
+        bool tie;
+        const double r = run + ulps_of(val[exact_pos<IPT>(i)], inv_u, tie);
+        if (tie || !(fabs(r) > 0x1p23 && fabs(r) < 0x1p24)) {
+          bad = i;
+          break;
+        }
+        run = r;
+      }
+      const int jb = block_min(bad, x);
+      if (jb == 0x7fffffff) {
+        s = (float)((P + total) * u);  // exact: an integer below 2^24 times u
+        start = nc;
+      } else {
+        if (bad == jb) x.s = (float)(run * u) + val[exact_pos<IPT>(jb)];  // the FP32 step
+        __syncthreads();
+        s = x.s;
+        __syncthreads();
+        start = jb + 1;
+      }
+    }
+  }
+  return s;
 }
 
 }  // namespace gd

@@ -9,6 +9,10 @@
 //
 // A policy P provides
 //   struct P::Args                      kernel arguments (pointers, scalars)
+//   static constexpr bool kExactHubs    hub rows (whole-row tiles) summed in
+//                                       edge order with sequential FP32
+//                                       rounding (exact_seq_sum), else hub
+//                                       segments tree-summed + finalize
 //   template <int NT> static G gather(const Args&, int odd)
 //   template <int NT> static E epilogue(const Args&, int odd)
 // with G / E as described in pull_engine.h (value(e, u); Pre, pre, init,
@@ -72,12 +76,19 @@ __global__ __launch_bounds__(kBlock) void pull_tile_kernel(const typename P::Arg
   const int odd = it.st->iter & 1;
   __shared__ RowTileLds<IPT> lds;
   __shared__ double s_err[kWaves];
+  __shared__ ExactLds xs;
   const Tile t = it.tiles[blockIdx.x];
   const auto g = P::template gather<NT>(a, odd);
   double err = 0.0;
   if (t.y >= 0) {
     const auto e = P::template epilogue<NT>(a, odd);
     err = row_tile<IPT, (NT > 0)>(it.off, it.col, it.e_max, t.x, t.y, t.z, t.w - t.z, g, e, lds);
+  } else if constexpr (P::kExactHubs) {
+    const auto e = P::template epilogue<NT>(a, odd);
+    const auto pre = e.pre(t.x);
+    const float sum = exact_seq_sum<IPT, (NT > 0)>(it.col, it.e_max, t.z, t.w - t.z, g,
+                                                   e.init(pre), lds.val, xs);
+    if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
   } else {
     const float p = segment_sum<IPT, (NT > 0)>(it.col, t.z, t.w - t.z, g, lds.val);
     if (threadIdx.x == 0) it.hub_part[-t.y - 1] = p;
@@ -99,6 +110,7 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
   const int odd = it.st->iter & 1;
   __shared__ RowTileLds<IPT> lds;
   __shared__ double s_err[kWaves];
+  __shared__ ExactLds xs;
   const auto g = P::template gather<NT>(a, odd);
   const auto e = P::template epilogue<NT>(a, odd);
   using E = decltype(e);
@@ -123,6 +135,11 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
         stage_row_tile<IPT>(cur, R, v, lds);
         __syncthreads();
         err += reduce_row_tile<IPT>(cur.x, cur.y - cur.x, e, R.pre, lds);
+      } else if constexpr (P::kExactHubs) {
+        const auto pre = e.pre(cur.x);
+        const float sum = exact_seq_sum<IPT, (NT > 0)>(it.col, it.e_max, cur.z, ne, g,
+                                                       e.init(pre), lds.val, xs);
+        if (threadIdx.x == 0) err += e.finish(cur.x, sum, pre);
       } else {
         float sacc = 0.0f;
 #pragma unroll

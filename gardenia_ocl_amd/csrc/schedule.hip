@@ -1,4 +1,5 @@
 // schedule.hip -- host schedule builder, CSR validation kernel, buffers, errors.
+#include <algorithm>
 #include <cstdio>
 #include <string>
 
@@ -30,14 +31,15 @@ void DeviceBuffer::release() {
 }
 
 int build_schedule(int m, int64_t nnz, const int32_t* off, int tile_nnz, int tile_rows,
-                   Schedule* s) {
+                   Schedule* s, int exact_len) {
   s->tiles.clear();
   s->hub_row.clear();
   s->hub_slot.clear();
   s->hub_nseg.clear();
   s->n_slots = 0;
   s->max_row_len = 0;
-  if (m < 0 || nnz < 0 || tile_nnz <= 0 || tile_rows <= 0) {
+  if (m < 0 || nnz < 0 || tile_nnz <= 0 || tile_rows <= 0 || exact_len < 0 ||
+      exact_len > tile_nnz) {
     set_error("build_schedule: bad sizes");
     return GD_ERR_INVALID;
   }
@@ -54,6 +56,18 @@ int build_schedule(int m, int64_t nnz, const int32_t* off, int tile_nnz, int til
       return GD_ERR_INVALID;
     }
     if (d > s->max_row_len) s->max_row_len = d;
+    if (exact_len > 0 && d > exact_len) {  // whole-row hub tile
+      if (r > cur) s->tiles.push_back(Tile{cur, r, off[cur], off[r]});
+      const int32_t h = (int32_t)s->hub_row.size();
+      s->hub_row.push_back(r);
+      s->hub_slot.push_back(h);
+      s->hub_nseg.push_back(1);
+      s->tiles.push_back(Tile{r, -(h + 1), off[r], off[r + 1]});
+      s->n_slots += 1;
+      cur = r + 1;
+      cur_nnz = 0;
+      continue;
+    }
     if (d > tile_nnz) {
       if (r > cur) s->tiles.push_back(Tile{cur, r, off[cur], off[r]});
       const int32_t nseg = (d + tile_nnz - 1) / tile_nnz;
@@ -78,6 +92,26 @@ int build_schedule(int m, int64_t nnz, const int32_t* off, int tile_nnz, int til
     cur_nnz += d;
   }
   if (m > cur) s->tiles.push_back(Tile{cur, m, off[cur], off[m]});
+  if (exact_len > 0 && !s->hub_row.empty()) {
+    // Whole-row hubs of more than kLongHub edges first, longest first (they run
+    // longest); the other hubs spread evenly among the row tiles, so their
+    // scan work overlaps other tiles' gathers instead of running back to back.
+    constexpr int32_t kLongHub = 65536;
+    std::vector<Tile> hubs, rows;
+    for (const Tile& t : s->tiles) (t.y < 0 ? hubs : rows).push_back(t);
+    std::stable_sort(hubs.begin(), hubs.end(),
+                     [](const Tile& a, const Tile& b) { return a.w - a.z > b.w - b.z; });
+    size_t nlong = 0;
+    while (nlong < hubs.size() && hubs[nlong].w - hubs[nlong].z > kLongHub) nlong++;
+    s->tiles.assign(hubs.begin(), hubs.begin() + nlong);
+    const size_t H = hubs.size() - nlong, R = rows.size();
+    size_t h = 0;
+    for (size_t r = 0; r <= R; r++) {
+      // hub j goes before row tile floor(j * R / H)
+      while (h < H && h * R <= r * H) s->tiles.push_back(hubs[nlong + h++]);
+      if (r < R) s->tiles.push_back(rows[r]);
+    }
+  }
   return GD_OK;
 }
 

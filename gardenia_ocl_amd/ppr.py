@@ -23,7 +23,11 @@ class PersonalizedPageRankPlan:
     """
 
     def __init__(self, in_row_offsets, in_column_indices, out_degree, k, stream=None, *,
-                 relabel=None, k_chunk=0):
+                 relabel=None, k_chunk=0, column_stop=True, compact=True):
+        """column_stop: each column stops at its own first iteration with L1
+        change < tol (as k single-seed runs would); False: all columns run
+        until the largest change is < tol.  compact: move stopped columns
+        behind the running ones so they are no longer gathered (k <= 4096)."""
         import torch
 
         for t in (in_row_offsets, in_column_indices, out_degree):
@@ -39,6 +43,8 @@ class PersonalizedPageRankPlan:
         if relabel is not None:
             cfg.relabel = 1 if relabel else 0
         cfg.k_chunk = int(k_chunk)
+        cfg.column_stop = 1 if column_stop else 0
+        cfg.compact = 1 if compact else 0
         h = ctypes.c_void_p()
         N.call("gd_ppr_plan_create", self.m, self.nnz, in_row_offsets.data_ptr(),
                in_column_indices.data_ptr(), out_degree.data_ptr(), self.k,
@@ -62,7 +68,7 @@ class PersonalizedPageRankPlan:
     def run(self, seeds, X, *, damping=K_DAMP, tol=EPSILON, max_iter=MAX_ITER, chunk=0):
         """Iterate on X (float32 CUDA tensor [m, k], in/out) for int32 CUDA
         ``seeds`` [k].  Returns (stats, per-iteration max-column L1, last
-        per-column L1)."""
+        per-column L1 of each column's last iteration)."""
         import torch
 
         if not (X.is_cuda and X.dtype == torch.float32 and X.is_contiguous()
@@ -79,6 +85,12 @@ class PersonalizedPageRankPlan:
                ctypes.byref(stats), errors.ctypes.data_as(ctypes.c_void_p),
                col_err.ctypes.data_as(ctypes.c_void_p))
         return stats.as_dict(), errors[: stats.iterations].copy(), col_err
+
+    def column_iterations(self) -> np.ndarray:
+        """Iterations each column ran in the last run (int32 [k])."""
+        out = np.zeros(self.k, np.int32)
+        N.call("gd_ppr_plan_column_iterations", self._h, out.ctypes.data_as(ctypes.c_void_p))
+        return out
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

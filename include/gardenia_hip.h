@@ -135,6 +135,12 @@ typedef struct gd_ppr_plan_config {
   int relabel; /* as gd_pr_plan_config.relabel */
   int k_chunk; /* columns per wavefront pass: 64, 128 or 256 (one 256*k_chunk/64-byte
                   gather per edge); 0: the largest that divides k */
+  int column_stop; /* 1 (default, -1): each column stops after its first iteration whose
+                      L1 change is below tol, as k independent runs would; 0: every
+                      column iterates until the largest L1 change is below tol */
+  int compact;     /* 1 (default, -1; needs k <= 4096): stopped columns are moved behind
+                      the running ones between host checks, so their lanes stop gathering;
+                      0: stopped columns stay in place (masked) */
 } gd_ppr_plan_config;
 
 void gd_ppr_default_plan_config(gd_ppr_plan_config *cfg);
@@ -156,14 +162,19 @@ int gd_ppr_plan_create(int m, int64_t nnz, const int32_t *d_in_row_offsets,
 
 /* d_seeds: k device int32 vertex ids; d_X: m*k device floats, in: initial
  * scores (typically one-hot at the seeds), out: result.  errors (host, may be
- * NULL) gets max over columns of the L1 change per iteration; col_errors
- * (host, may be NULL) the last iteration's per-column L1 (k values).
+ * NULL) gets, per iteration, the max L1 change over the columns still running;
+ * col_errors (host, may be NULL) each column's L1 change in its last iteration
+ * (k values).  stats->iterations is the longest column's count.
  * opts->seed_vertex is ignored. */
 int gd_ppr_plan_run(gd_ppr_plan *plan, const int32_t *d_seeds, float *d_X,
                     const gd_pr_options *opts, gd_pr_stats *stats, double *errors,
                     double *col_errors);
 
 int gd_ppr_plan_info(const gd_ppr_plan *plan, gd_ppr_plan_info_t *info);
+
+/* Iterations each column ran in the last gd_ppr_plan_run (k host int32, user
+ * column order). */
+int gd_ppr_plan_column_iterations(const gd_ppr_plan *plan, int32_t *iterations);
 
 int gd_ppr_plan_destroy(gd_ppr_plan *plan);
 

@@ -104,3 +104,63 @@ def test_ppr_rejects_bad_arguments(device):
     X = torch.zeros((in_csr.m, 64), dtype=torch.float32, device=device)
     with pytest.raises(GardeniaError, match="seed vertex out of range"):
         plan.run(bad, X)
+
+
+def _mixed_seeds(in_csr, out_deg, k, rng_seed):
+    """Seeds whose columns stop at very different iterations: vertices with no
+    out-edges (their mass stays put: 2 iterations) mixed with random ones."""
+    rng = np.random.default_rng(rng_seed)
+    sinks = np.flatnonzero(out_deg == 0)
+    rand = rng.integers(0, in_csr.m, k)
+    pick = rng.random(k) < 0.4
+    rand[pick] = rng.choice(sinks, int(pick.sum()))
+    return rand.astype(np.int32)
+
+
+@pytest.mark.parametrize("k,kc,relabel,compact", [(128, 64, False, True), (128, 64, False, False),
+                                                  (256, 256, True, True), (256, 128, False, True),
+                                                  (192, 64, True, True)])
+def test_ppr_column_stop_matches_single_seed_runs(device, k, kc, relabel, compact):
+    """Each column stops after its own first iteration with L1 change < tol,
+    the same iteration the scalar single-seed loop stops at, with the same
+    scores bit for bit (max in-degree <= 4096: sequential per-column sums).
+    Moving stopped columns behind the running ones (compact) changes nothing."""
+    import torch
+
+    out_csr, in_csr = graph.rmat(12, 16, seed=3)
+    m = in_csr.m
+    seeds = _mixed_seeds(in_csr, out_csr.degree, k, 5)
+    plan, keep = _plan(device, in_csr, out_csr.degree, k, relabel=relabel, k_chunk=kc,
+                       compact=compact)
+    X = plan.one_hot(seeds)
+    st, errs, col_err = plan.run(torch.from_numpy(seeds).to(device), X, tol=1e-6, max_iter=100,
+                                 chunk=2)
+    its = plan.column_iterations()
+    assert st["converged"] and st["iterations"] == its.max()
+    assert its.min() <= 3 < 20 < its.max()  # columns really stop at different times
+    Xh = X.cpu().numpy()
+    for j in range(k):
+        ref, it, ref_errs = oracle.ppr(m, in_csr.row_offsets, in_csr.column_indices,
+                                       out_csr.degree, int(seeds[j]), tol=1e-6, max_iter=100)
+        assert its[j] == it, (j, its[j], it)
+        np.testing.assert_array_equal(Xh[:, j], ref)
+        assert abs(col_err[j] - ref_errs[it - 1]) <= 1e-12 + 1e-9 * ref_errs[it - 1]
+
+
+def test_ppr_all_columns_mode_runs_to_largest_change(device):
+    """column_stop=False: every column runs until the largest change < tol."""
+    import torch
+
+    out_csr, in_csr = graph.rmat(12, 16, seed=3)
+    m, k = in_csr.m, 64
+    seeds = _mixed_seeds(in_csr, out_csr.degree, k, 6)
+    plan, keep = _plan(device, in_csr, out_csr.degree, k, column_stop=False)
+    X = plan.one_hot(seeds)
+    st, errs, col_err = plan.run(torch.from_numpy(seeds).to(device), X, tol=1e-6, max_iter=100)
+    its = plan.column_iterations()
+    assert np.all(its == st["iterations"]) and errs[-1] < 1e-6 <= errs[-2]
+    Xh = X.cpu().numpy()
+    for j in (0, 7, 63):
+        ref, it, _ = oracle.ppr(m, in_csr.row_offsets, in_csr.column_indices, out_csr.degree,
+                                int(seeds[j]), tol=0.0, max_iter=st["iterations"])
+        np.testing.assert_array_equal(Xh[:, j], ref)
