@@ -73,7 +73,8 @@ class PrStats(ctypes.Structure):
 
 class PrPlanConfig(ctypes.Structure):
     _fields_ = [("relabel", ctypes.c_int), ("items_per_thread", ctypes.c_int),
-                ("nontemporal", ctypes.c_int), ("persistent", ctypes.c_int)]
+                ("nontemporal", ctypes.c_int), ("persistent", ctypes.c_int),
+                ("sort_tiles", ctypes.c_int)]
 
 
 class PrPlanInfo(ctypes.Structure):
@@ -88,6 +89,7 @@ class PrPlanInfo(ctypes.Structure):
         ("persistent", ctypes.c_int32),
         ("pull_grid", ctypes.c_int32),
         ("plan_ms", ctypes.c_float),
+        ("sort_tiles", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -9,7 +9,8 @@ degree histogram), so a repeated workload is tuned once.
 Knobs (see include/gardenia_hip.h):
   PageRank:     relabel (L2-resident hot sources), items_per_thread (tile size,
                 i.e. the hub-row threshold 256*ipt), nontemporal (cache-hint
-                level), persistent (pipelined persistent grid)
+                level), persistent (pipelined persistent grid), sort_tiles (gather
+                each tile in source order)
   batched PPR:  k_chunk (columns per wavefront pass), relabel
 
 Usage:
@@ -31,6 +32,7 @@ PR_SPACE = {
     "items_per_thread": [8, 16],
     "nontemporal": [0, 1],
     "persistent": [False, True],
+    "sort_tiles": [False, True],
 }
 PPR_SPACE = {"k_chunk": [64, 128, 256], "relabel": [False, True]}
 

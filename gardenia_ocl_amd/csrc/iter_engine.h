@@ -13,6 +13,7 @@
 #include "gardenia_hip.h"
 #include "gd_common.h"
 #include "pull_iter.h"
+#include "tile_sort.h"
 
 namespace gd {
 
@@ -24,11 +25,13 @@ struct IterEngine {
   int nt = 1;                // cache-hint level (see gd_pr_plan_config.nontemporal)
   bool persistent = true;
   int exact_len = 0;         // > 0: rows longer than this are whole-row exact hub tiles
+  bool sort_tiles = false;   // gather each tile's edges in source order (tile_sort.h)
+  int32_t* col_owned = nullptr;  // the caller's private copy of col: sorted in place
   int pull_grid = 0;
   const int32_t* off = nullptr;
   const int32_t* col = nullptr;
   int e_max = 0;
-  DeviceBuffer dummy_col;
+  DeviceBuffer dummy_col, col_sorted, pos;
   Schedule sched;
   int n_tiles = 0, n_hubs = 0, fin_blocks = 1;
   DeviceBuffer tiles, hub_row, hub_slot, hub_nseg, hub_part, err_part, fin_part, state, errors;
@@ -63,6 +66,19 @@ struct IterEngine {
     if (rc) return rc;
     n_tiles = (int)sched.tiles.size();
     n_hubs = (int)sched.hub_row.size();
+    rc = pos.alloc(sort_tiles ? 2 * (size_t)std::max<int64_t>(nnz, 1) : 2);
+    if (rc) return rc;
+    if (sort_tiles && nnz > 0) {
+      int32_t* dst = col_owned;
+      if (!dst) {
+        rc = col_sorted.alloc(4 * (size_t)nnz);
+        if (rc) return rc;
+        dst = col_sorted.as<int32_t>();
+      }
+      rc = build_tile_sort(sched, kBlock * ipt, col, dst, pos.as<uint16_t>(), stream);
+      if (rc) return rc;
+      col = dst;
+    }
     rc = tiles.alloc(sizeof(Tile) * n_tiles);
     if (!rc) rc = hub_row.alloc(4 * (size_t)std::max(1, n_hubs));
     if (!rc) rc = hub_slot.alloc(4 * (size_t)std::max(1, n_hubs));
@@ -90,8 +106,8 @@ struct IterEngine {
       GD_HIP_TRY(hipGetDevice(&dev));
       GD_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
       const void* fn = (ipt == 16)
-                           ? reinterpret_cast<const void*>(&pull_persistent_kernel<P, 16, 1>)
-                           : reinterpret_cast<const void*>(&pull_persistent_kernel<P, 8, 1>);
+                           ? reinterpret_cast<const void*>(&pull_persistent_kernel<P, 16, 1, true>)
+                           : reinterpret_cast<const void*>(&pull_persistent_kernel<P, 8, 1, true>);
       GD_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0));
       pull_grid = std::max(1, std::min(n_tiles, cus * std::max(1, per_cu)));
     } else {
@@ -102,8 +118,8 @@ struct IterEngine {
   }
 
   IterArgs iter_args() {
-    return IterArgs{tiles.as<Tile>(), n_tiles, off, col, e_max, hub_part.as<float>(),
-                    err_part.as<double>(), state.as<IterState>()};
+    return IterArgs{tiles.as<Tile>(), n_tiles, off, col, e_max, pos.as<uint16_t>(),
+                    hub_part.as<float>(), err_part.as<double>(), state.as<IterState>()};
   }
 
   // exact hubs finish their rows in the pull launch: no hub epilogues here
@@ -113,14 +129,22 @@ struct IterEngine {
                    fin_part.as<double>()};
   }
 
+  template <class P, int IPT, int NT, bool SORTED>
+  void launch_pull_s(const typename P::Args& a, const IterArgs& it) {
+    if (persistent)
+      hipLaunchKernelGGL((pull_persistent_kernel<P, IPT, NT, SORTED>), dim3(pull_grid),
+                         dim3(kBlock), 0, stream, a, it);
+    else
+      hipLaunchKernelGGL((pull_tile_kernel<P, IPT, NT, SORTED>), dim3(n_tiles), dim3(kBlock), 0,
+                         stream, a, it);
+  }
+
   template <class P, int IPT, int NT>
   void launch_pull_t(const typename P::Args& a, const IterArgs& it) {
-    if (persistent)
-      hipLaunchKernelGGL((pull_persistent_kernel<P, IPT, NT>), dim3(pull_grid), dim3(kBlock), 0,
-                         stream, a, it);
+    if (sort_tiles)
+      launch_pull_s<P, IPT, NT, true>(a, it);
     else
-      hipLaunchKernelGGL((pull_tile_kernel<P, IPT, NT>), dim3(n_tiles), dim3(kBlock), 0, stream,
-                         a, it);
+      launch_pull_s<P, IPT, NT, false>(a, it);
   }
 
   template <class P>

@@ -90,6 +90,8 @@ int build_plan(gd_pr_plan* p, const int32_t* d_off, const int32_t* d_col,
     col = p->rl.col.as<int32_t>();
     p->outdeg = p->rl.outdeg.as<int32_t>();
   }
+  // the relabelled copy of col is the plan's own: tiles are sorted in place
+  p->eng.col_owned = p->relabel ? p->rl.col.as<int32_t>() : nullptr;
   rc = p->contrib0.alloc(4 * (size_t)m);
   if (!rc) rc = p->contrib1.alloc(4 * (size_t)m);
   if (!rc) rc = p->eng.build(m, p->nnz, off, col, p->stream);
@@ -121,6 +123,7 @@ void gd_pr_default_plan_config(gd_pr_plan_config* c) {
   c->items_per_thread = 0;
   c->nontemporal = -1;
   c->persistent = -1;
+  c->sort_tiles = -1;
 }
 
 int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t* d_col,
@@ -156,6 +159,7 @@ int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t
   p->eng.ipt = c.items_per_thread ? c.items_per_thread : 16;
   p->eng.nt = c.nontemporal < 0 ? 0 : c.nontemporal;
   p->eng.persistent = c.persistent == 1;
+  p->eng.sort_tiles = c.sort_tiles != 0;
   // rows longer than 2048 edges are summed in the reference's order and rounding
   p->eng.exact_len = kExactHubLen;
   int rc = GD_OK;
@@ -197,6 +201,7 @@ int gd_pr_plan_info(const gd_pr_plan* p, gd_pr_plan_info_t* info) {
   info->persistent = p->eng.persistent ? 1 : 0;
   info->pull_grid = p->eng.pull_grid;
   info->plan_ms = p->plan_ms;
+  info->sort_tiles = p->eng.sort_tiles ? 1 : 0;
   return GD_OK;
 }
 

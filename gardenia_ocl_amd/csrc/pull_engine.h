@@ -103,23 +103,38 @@ __device__ __forceinline__ void load_indices(const int32_t* __restrict__ col, in
     idx[j] = ld_stream<NT>(col + e0 + min((int)threadIdx.x + j * kBlock, last));
 }
 
-template <int IPT, class G>
-__device__ __forceinline__ void gather_values(const G& g, int e0, int ne, const int (&idx)[IPT],
-                                              float (&v)[IPT]) {
+// Tiles whose edges are stored sorted by source (SORTED) carry, per edge, its
+// offset in the tile: pos[] (16 bits).  The gather functor always sees the
+// edge's CSR index.
+template <int IPT, bool NT>
+__device__ __forceinline__ void load_positions(const uint16_t* __restrict__ pos, int e0, int ne,
+                                               int (&p)[IPT]) {
   const int last = ne - 1;
 #pragma unroll
-  for (int j = 0; j < IPT; j++) v[j] = g.value(e0 + min((int)threadIdx.x + j * kBlock, last), idx[j]);
+  for (int j = 0; j < IPT; j++)
+    p[j] = (int)ld_stream<NT>(pos + e0 + min((int)threadIdx.x + j * kBlock, last));
+}
+
+template <int IPT, bool SORTED, class G>
+__device__ __forceinline__ void gather_values(const G& g, int e0, int ne, const int (&idx)[IPT],
+                                              const int (&p)[IPT], float (&v)[IPT]) {
+  const int last = ne - 1;
+#pragma unroll
+  for (int j = 0; j < IPT; j++)
+    v[j] = g.value(e0 + (SORTED ? p[j] : min((int)threadIdx.x + j * kBlock, last)), idx[j]);
 }
 
 // Sum of the gathered values of a hub segment [e0, e0+ne), ne >= 1: per-thread
 // partial in j order, wave butterfly, then waves in order.  Valid in thread 0.
-template <int IPT, bool NT, class G>
-__device__ __forceinline__ float segment_sum(const int32_t* __restrict__ col, int e0, int ne,
+template <int IPT, bool NT, bool SORTED, class G>
+__device__ __forceinline__ float segment_sum(const int32_t* __restrict__ col,
+                                             const uint16_t* __restrict__ pos, int e0, int ne,
                                              const G& g, float* s_tmp) {
-  int idx[IPT];
+  int idx[IPT], p[IPT];
   float v[IPT];
   load_indices<IPT, NT>(col, e0, ne, idx);
-  gather_values<IPT>(g, e0, ne, idx, v);
+  if constexpr (SORTED) load_positions<IPT, NT>(pos, e0, ne, p);
+  gather_values<IPT, SORTED>(g, e0, ne, idx, p, v);
   float s = 0.0f;
 #pragma unroll
   for (int j = 0; j < IPT; j++) s += ((int)threadIdx.x + j * kBlock < ne) ? v[j] : 0.0f;
@@ -140,6 +155,7 @@ struct RowTileLds {
 template <int IPT, class Pre>
 struct TileRegs {
   int idx[IPT];
+  int pos[IPT];  // sorted tiles only
   int offv[kOffLoads];
   Pre pre[kRowsPerThread];
 };
@@ -150,15 +166,21 @@ struct TileRegs {
 // clamped: an empty tile re-reads its first edge, a hub segment reads the
 // offsets and operands of its own row.
 // `e_max` is the last valid index of `col` (the arrays hold >= 1 element).
-template <int IPT, bool NT, class E>
+template <int IPT, bool NT, bool SORTED, class E>
 __device__ __forceinline__ void issue_tile_loads(const Tile& t, const int32_t* __restrict__ off,
-                                                 const int32_t* __restrict__ col, int e_max,
+                                                 const int32_t* __restrict__ col,
+                                                 const uint16_t* __restrict__ pos, int e_max,
                                                  const E& e, TileRegs<IPT, typename E::Pre>& R) {
   const int tid = threadIdx.x;
   const int last = max(t.w - t.z - 1, 0);
 #pragma unroll
   for (int j = 0; j < IPT; j++)
     R.idx[j] = ld_stream<NT>(col + min(t.z + min(tid + j * kBlock, last), e_max));
+  if constexpr (SORTED) {
+#pragma unroll
+    for (int j = 0; j < IPT; j++)
+      R.pos[j] = (int)ld_stream<NT>(pos + min(t.z + min(tid + j * kBlock, last), e_max));
+  }
   const int nr = (t.y >= 0) ? t.y - t.x : 1;
 #pragma unroll
   for (int q = 0; q < kOffLoads; q++)
@@ -230,7 +252,7 @@ __device__ __forceinline__ double reduce_row_tile(int r0, int nr, const E& e,
 }
 
 // Stage a row tile's gathered values and rebased offsets into LDS.
-template <int IPT, class Pre>
+template <int IPT, bool SORTED, class Pre>
 __device__ __forceinline__ void stage_row_tile(const Tile& t, const TileRegs<IPT, Pre>& R,
                                                const float (&v)[IPT], RowTileLds<IPT>& lds) {
   const int tid = threadIdx.x;
@@ -240,7 +262,7 @@ __device__ __forceinline__ void stage_row_tile(const Tile& t, const TileRegs<IPT
     if (tid + q * kBlock <= nr) lds.off[tid + q * kBlock] = R.offv[q] - t.z;
 #pragma unroll
   for (int j = 0; j < IPT; j++)
-    if (tid + j * kBlock < ne) lds.val[tid + j * kBlock] = v[j];
+    if (tid + j * kBlock < ne) lds.val[SORTED ? R.pos[j] : tid + j * kBlock] = v[j];
 }
 
 // One row tile: rows [r0, r1) whose edges are [e0, e0+ne).  The epilogue E
@@ -250,17 +272,18 @@ __device__ __forceinline__ void stage_row_tile(const Tile& t, const TileRegs<IPT
 //   float init(const Pre&)   value the row sum starts from
 //   double finish(int row, float sum, const Pre&)   writes the row, returns
 //                            its contribution to the tile's double accumulator
-template <int IPT, bool NT, class G, class E>
+template <int IPT, bool NT, bool SORTED, class G, class E>
 __device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
-                                           const int32_t* __restrict__ col, int e_max, int r0,
+                                           const int32_t* __restrict__ col,
+                                           const uint16_t* __restrict__ pos, int e_max, int r0,
                                            int r1, int e0, int ne, const G& g, const E& e,
                                            RowTileLds<IPT>& lds) {
   const Tile t{r0, r1, e0, e0 + ne};
   TileRegs<IPT, typename E::Pre> R;
-  issue_tile_loads<IPT, NT>(t, off, col, e_max, e, R);
+  issue_tile_loads<IPT, NT, SORTED>(t, off, col, pos, e_max, e, R);
   float v[IPT];
-  if (ne > 0) gather_values<IPT>(g, e0, ne, R.idx, v);
-  stage_row_tile<IPT>(t, R, v, lds);
+  if (ne > 0) gather_values<IPT, SORTED>(g, e0, ne, R.idx, R.pos, v);
+  stage_row_tile<IPT, SORTED>(t, R, v, lds);
   __syncthreads();
   return reduce_row_tile<IPT>(r0, r1 - r0, e, R.pre, lds);
 }
@@ -275,7 +298,7 @@ __device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
 //   ulp u = 2^(k-23), and the exact s + v[i] stays strictly inside it, the
 //   rounded step is s + q_i*u with q_i = v[i]/u rounded to the nearest
 //   integer: s advances by an integer number of ulps.  The workgroup scans the
-//   q_i (exact integers in double) and advances s over every step up to the
+//   q_i (32-bit integers) and advances s over every step up to the
 //   first one that leaves the binade, rounds a tie (fraction exactly 1/2,
 //   whose rounding depends on the running parity) or starts from a subnormal /
 //   non-finite s; that step is one ordinary FP32 addition, and the scan
@@ -286,38 +309,34 @@ __device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
 constexpr int kExactHead = 256;  // terms of a row added serially before the scan
 
 struct ExactLds {
-  double wsum[kWaves];
+  int32_t wsum[kWaves];
   int wmin[kWaves];
   float s;
 };
 
-__device__ __forceinline__ double wave_incl_scan(double v) {
-  const int lane = threadIdx.x & (kWave - 1);
+// Exclusive prefix over the workgroup (lanes, then waves) and the total, in
+// wrapping 32-bit arithmetic.  Only prefixes of steps that all stay inside the
+// binade are used, and those are exact (each below 2^25 in magnitude).
+__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t& total, ExactLds& x) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  uint32_t inc = (uint32_t)v;
 #pragma unroll
   for (int o = 1; o < kWave; o <<= 1) {
-    const double t = __shfl_up(v, o, kWave);
-    if (lane >= o) v += t;
+    const uint32_t t = (uint32_t)__shfl_up((int)inc, o, kWave);
+    if (lane >= o) inc += t;
   }
-  return v;
-}
-
-// Exclusive prefix over the workgroup (lanes, then waves) and the total.  The
-// addends are integers below 2^39 in magnitude: exact in any order.
-__device__ __forceinline__ double block_excl_scan(double v, double& total, ExactLds& x) {
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  const double inc = wave_incl_scan(v);
-  if (lane == kWave - 1) x.wsum[wid] = inc;
+  if (lane == kWave - 1) x.wsum[wid] = (int32_t)inc;
   __syncthreads();
-  double before = 0.0, tot = 0.0;
+  uint32_t before = 0, tot = 0;
 #pragma unroll
   for (int w = 0; w < kWaves; w++) {
-    const double ws = x.wsum[w];
-    before += (w < wid) ? ws : 0.0;
+    const uint32_t ws = (uint32_t)x.wsum[w];
+    before += (w < wid) ? ws : 0u;
     tot += ws;
   }
   __syncthreads();
-  total = tot;
-  return before + (inc - v);
+  total = (int32_t)tot;
+  return (int32_t)(before + (inc - (uint32_t)v));
 }
 
 __device__ __forceinline__ int block_min(int v, ExactLds& x) {
@@ -333,14 +352,28 @@ __device__ __forceinline__ int block_min(int v, ExactLds& x) {
   return r;
 }
 
-// v / u rounded to the nearest integer (clamped to +-2^26: such a step leaves
-// the binade anyway); tie: the fraction is exactly 1/2.
-__device__ __forceinline__ double ulps_of(float v, double inv_u, bool& tie) {
-  const double a = (double)v * inv_u;  // exact: scaling by a power of two
-  const double f = floor(a);
-  const double fr = a - f;
-  tie = (fr == 0.5);
-  return fmin(fmax(f + (fr > 0.5 ? 1.0 : 0.0), -0x1p26), 0x1p26);
+// v in units of u = 2^(ex-23), rounded to the nearest integer, from the bits of
+// v (|q| clamped to 2^26: such a step leaves the binade anyway).  tie: the
+// fraction is exactly 1/2 (the rounding then depends on the running sum's
+// parity); odd: v is inf or NaN.  Both make the step irregular.
+__device__ __forceinline__ int32_t ulps_of(float v, int ex, bool& irregular) {
+  const uint32_t b = __float_as_uint(v);
+  const int eb = (int)((b >> 23) & 0xffu);
+  const uint32_t mv = eb ? ((b & 0x7fffffu) | 0x800000u) : (b & 0x7fffffu);  // v = mv*2^(ev-23)
+  const int d = ex - (eb ? eb - 127 : -126);                                  // q = mv / 2^d
+  int32_t q;
+  bool tie = false;
+  if (d <= 0) {
+    q = (d <= -3) ? (1 << 26) : (int32_t)min((uint32_t)1 << 26, mv << (-d));
+  } else if (d >= 26) {
+    q = 0;
+  } else {
+    const uint32_t half = 1u << (d - 1), rem = mv & ((1u << d) - 1u);
+    tie = (rem == half);
+    q = (int32_t)((mv >> d) + (rem > half ? 1u : 0u));
+  }
+  irregular = tie || eb == 0xff;
+  return (b >> 31) ? -q : q;
 }
 
 // Position of chunk element i in the padded LDS buffer: thread t reads its IPT
@@ -353,24 +386,31 @@ __device__ __forceinline__ int exact_pos(int i) {
 // Sum of the gathered values of edges [e0, e0+ne) in edge order, rounded as the
 // sequential FP32 loop starting from s.  All threads return the same value.
 // `val` holds kBlock*IPT + kBlock floats.
-template <int IPT, bool NT, class G>
-__device__ float exact_seq_sum(const int32_t* __restrict__ col, int e_max, int e0, int ne,
-                               const G& g, float s, float* __restrict__ val, ExactLds& x) {
+template <int IPT, bool NT, bool SORTED, class G>
+__device__ float exact_seq_sum(const int32_t* __restrict__ col, const uint16_t* __restrict__ pos,
+                               int e_max, int e0, int ne, const G& g, float s,
+                               float* __restrict__ val, ExactLds& x) {
   constexpr int C = kBlock * IPT;
   const int tid = threadIdx.x;
   for (int c0 = 0; c0 < ne; c0 += C) {
     const int nc = min(C, ne - c0);
-    int idx[IPT];
+    int idx[IPT], p[IPT];
     float v[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; j++)
       idx[j] = ld_stream<NT>(col + min(e0 + c0 + min(tid + j * kBlock, nc - 1), e_max));
+    if constexpr (SORTED) {  // chunks are sorted separately: offsets are chunk-relative
 #pragma unroll
-    for (int j = 0; j < IPT; j++) v[j] = g.value(e0 + c0 + min(tid + j * kBlock, nc - 1), idx[j]);
+      for (int j = 0; j < IPT; j++)
+        p[j] = (int)ld_stream<NT>(pos + min(e0 + c0 + min(tid + j * kBlock, nc - 1), e_max));
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; j++)
+      v[j] = g.value(e0 + c0 + (SORTED ? p[j] : min(tid + j * kBlock, nc - 1)), idx[j]);
     __syncthreads();  // the previous chunk's readers are done with val
 #pragma unroll
     for (int j = 0; j < IPT; j++)
-      if (tid + j * kBlock < nc) val[exact_pos<IPT>(tid + j * kBlock)] = v[j];
+      if (tid + j * kBlock < nc) val[exact_pos<IPT>(SORTED ? p[j] : tid + j * kBlock)] = v[j];
     __syncthreads();
     const int lo = tid * IPT, hi = min(lo + IPT, nc);  // this thread's elements
     int start = 0;
@@ -412,20 +452,21 @@ __device__ float exact_seq_sum(const int32_t* __restrict__ col, int e_max, int e
         continue;
       }
       const int ex = ilogbf(s);
-      const double inv_u = ldexp(1.0, 23 - ex), u = ldexp(1.0, ex - 23);
-      const double P = (double)s * inv_u;  // integer, 2^23 <= |P| < 2^24
-      double L = 0.0;
+      // s = P * 2^(ex-23) with 2^23 <= |P| < 2^24
+      const int32_t P = (int32_t)(__float_as_uint(s) & 0x7fffffu) | 0x800000;
+      const int32_t Ps = signbit(s) ? -P : P;
+      uint32_t L = 0;
       for (int i = max(lo, start); i < hi; i++) {
-        bool tie;
-        L += ulps_of(val[exact_pos<IPT>(i)], inv_u, tie);
+        bool irr;
+        L += (uint32_t)ulps_of(val[exact_pos<IPT>(i)], ex, irr);
       }
-      double total;
-      double run = P + block_excl_scan(L, total, x);
+      int32_t total;
+      int32_t run = (int32_t)((uint32_t)Ps + (uint32_t)block_excl_scan((int32_t)L, total, x));
       int bad = 0x7fffffff;
       for (int i = max(lo, start); i < hi; i++) {
-        bool tie;
-        const double r = run + ulps_of(val[exact_pos<IPT>(i)], inv_u, tie);
-        if (tie || !(fabs(r) > 0x1p23 && fabs(r) < 0x1p24)) {
+        bool irr;
+        const int32_t r = (int32_t)((uint32_t)run + (uint32_t)ulps_of(val[exact_pos<IPT>(i)], ex, irr));
+        if (irr || !(abs(r) > (1 << 23) && abs(r) < (1 << 24))) {
           bad = i;
           break;
         }
@@ -433,10 +474,11 @@ __device__ float exact_seq_sum(const int32_t* __restrict__ col, int e_max, int e
       }
       const int jb = block_min(bad, x);
       if (jb == 0x7fffffff) {
-        s = (float)((P + total) * u);  // exact: an integer below 2^24 times u
+        // exact: an integer below 2^24 in magnitude times a power of two
+        s = ldexpf((float)(int32_t)((uint32_t)Ps + (uint32_t)total), ex - 23);
         start = nc;
       } else {
-        if (bad == jb) x.s = (float)(run * u) + val[exact_pos<IPT>(jb)];  // the FP32 step
+        if (bad == jb) x.s = ldexpf((float)run, ex - 23) + val[exact_pos<IPT>(jb)];  // FP32 step
         __syncthreads();
         s = x.s;
         __syncthreads();

@@ -45,6 +45,7 @@ struct IterArgs {
   const int32_t* __restrict__ off;
   const int32_t* __restrict__ col;
   int e_max;  // last valid index of col (a 1-element dummy when nnz == 0)
+  const uint16_t* __restrict__ pos;  // sorted tiles: each edge's offset in its tile
   float* __restrict__ hub_part;
   double* __restrict__ err_part;
   IterState* st;
@@ -68,8 +69,9 @@ static __global__ void iter_state_init_kernel(IterState* st, int max_iter, doubl
   st->tol = tol;
 }
 
-// One workgroup per tile.
-template <class P, int IPT, int NT>
+// One workgroup per tile.  SORTED: each tile's edges are stored in source
+// order with their tile offsets (tile_sort.h); sums are still in CSR order.
+template <class P, int IPT, int NT, bool SORTED>
 __global__ __launch_bounds__(kBlock) void pull_tile_kernel(const typename P::Args a,
                                                            const IterArgs it) {
   if (it.st->done) return;
@@ -82,15 +84,16 @@ __global__ __launch_bounds__(kBlock) void pull_tile_kernel(const typename P::Arg
   double err = 0.0;
   if (t.y >= 0) {
     const auto e = P::template epilogue<NT>(a, odd);
-    err = row_tile<IPT, (NT > 0)>(it.off, it.col, it.e_max, t.x, t.y, t.z, t.w - t.z, g, e, lds);
+    err = row_tile<IPT, (NT > 0), SORTED>(it.off, it.col, it.pos, it.e_max, t.x, t.y, t.z,
+                                          t.w - t.z, g, e, lds);
   } else if constexpr (P::kExactHubs) {
     const auto e = P::template epilogue<NT>(a, odd);
     const auto pre = e.pre(t.x);
-    const float sum = exact_seq_sum<IPT, (NT > 0)>(it.col, it.e_max, t.z, t.w - t.z, g,
-                                                   e.init(pre), lds.val, xs);
+    const float sum = exact_seq_sum<IPT, (NT > 0), SORTED>(it.col, it.pos, it.e_max, t.z,
+                                                           t.w - t.z, g, e.init(pre), lds.val, xs);
     if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
   } else {
-    const float p = segment_sum<IPT, (NT > 0)>(it.col, t.z, t.w - t.z, g, lds.val);
+    const float p = segment_sum<IPT, (NT > 0), SORTED>(it.col, it.pos, t.z, t.w - t.z, g, lds.val);
     if (threadIdx.x == 0) it.hub_part[-t.y - 1] = p;
   }
   const double tot = block_sum<kWaves>(err, s_err);
@@ -103,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void pull_tile_kernel(const typename P::Arg
 // reduction and epilogue (vector loads return in issue order, so waiting for
 // the gathers never waits for the prefetch).  Same arithmetic and per-row
 // order as pull_tile_kernel.
-template <class P, int IPT, int NT>
+template <class P, int IPT, int NT, bool SORTED>
 __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename P::Args a,
                                                                  const IterArgs it) {
   if (it.st->done) return;
@@ -120,25 +123,25 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
   if (t < it.n_tiles) {
     Tile cur = it.tiles[t];
     Regs R;
-    issue_tile_loads<IPT, (NT > 0)>(cur, it.off, it.col, it.e_max, e, R);
+    issue_tile_loads<IPT, (NT > 0), SORTED>(cur, it.off, it.col, it.pos, it.e_max, e, R);
     for (;;) {
       const int tn = t + gridDim.x;
       const bool has_next = tn < it.n_tiles;
       const Tile nxt = it.tiles[has_next ? tn : t];
       const int ne = cur.w - cur.z;
       float v[IPT];
-      if (ne > 0) gather_values<IPT>(g, cur.z, ne, R.idx, v);
+      if (ne > 0) gather_values<IPT, SORTED>(g, cur.z, ne, R.idx, R.pos, v);
       Regs Rn;
       // unconditional (clamped) so the compiler's wait counting stays exact
-      issue_tile_loads<IPT, (NT > 0)>(nxt, it.off, it.col, it.e_max, e, Rn);
+      issue_tile_loads<IPT, (NT > 0), SORTED>(nxt, it.off, it.col, it.pos, it.e_max, e, Rn);
       if (cur.y >= 0) {
-        stage_row_tile<IPT>(cur, R, v, lds);
+        stage_row_tile<IPT, SORTED>(cur, R, v, lds);
         __syncthreads();
         err += reduce_row_tile<IPT>(cur.x, cur.y - cur.x, e, R.pre, lds);
       } else if constexpr (P::kExactHubs) {
         const auto pre = e.pre(cur.x);
-        const float sum = exact_seq_sum<IPT, (NT > 0)>(it.col, it.e_max, cur.z, ne, g,
-                                                       e.init(pre), lds.val, xs);
+        const float sum = exact_seq_sum<IPT, (NT > 0), SORTED>(it.col, it.pos, it.e_max, cur.z,
+                                                               ne, g, e.init(pre), lds.val, xs);
         if (threadIdx.x == 0) err += e.finish(cur.x, sum, pre);
       } else {
         float sacc = 0.0f;

@@ -49,9 +49,9 @@ __global__ __launch_bounds__(kBlock) void spmv_pull_kernel(
   const SpGather g{x, Ax};
   if (t.y >= 0) {
     const SpEpilogue e{y};
-    (void)row_tile<kSpIPT, false>(Ap, Aj, e_max, t.x, t.y, t.z, t.w - t.z, g, e, lds);
+    (void)row_tile<kSpIPT, false, false>(Ap, Aj, nullptr, e_max, t.x, t.y, t.z, t.w - t.z, g, e, lds);
   } else {
-    const float p = segment_sum<kSpIPT, false>(Aj, t.z, t.w - t.z, g, lds.val);
+    const float p = segment_sum<kSpIPT, false, false>(Aj, nullptr, t.z, t.w - t.z, g, lds.val);
     if (threadIdx.x == 0) hub_part[-t.y - 1] = p;
   }
 }

@@ -1,0 +1,85 @@
+// tile_sort.hip -- per-segment sort of edges by source (see tile_sort.h).
+#include "tile_sort.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace gd {
+
+namespace {
+
+constexpr int kSortBlock = 256;
+
+// One workgroup per segment [seg.x, seg.x + seg.y): keys (source << 12 |
+// offset) sorted by an LDS bitonic network, padded to a power of two with
+// UINT64_MAX.  Sorting the 64-bit key orders by source, then by offset.
+__global__ __launch_bounds__(kSortBlock) void segment_sort_kernel(const int2* __restrict__ segs,
+                                                                  const int32_t* __restrict__ col,
+                                                                  int32_t* __restrict__ col_sorted,
+                                                                  uint16_t* __restrict__ pos) {
+  __shared__ uint64_t key[kSortMax];
+  const int2 sg = segs[blockIdx.x];
+  const int e0 = sg.x, n = sg.y;
+  int np = 1;
+  while (np < n) np <<= 1;
+  for (int i = threadIdx.x; i < np; i += kSortBlock)
+    key[i] = (i < n) ? (((uint64_t)(uint32_t)col[e0 + i] << 12) | (uint64_t)i) : ~0ull;
+  __syncthreads();
+  for (int k = 2; k <= np; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < np; i += kSortBlock) {
+        const int l = i ^ j;
+        if (l > i) {
+          const uint64_t a = key[i], b = key[l];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            key[i] = b;
+            key[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // every thread read its keys before the last barrier; col may alias col_sorted
+  for (int i = threadIdx.x; i < n; i += kSortBlock) {
+    const uint64_t v = key[i];
+    col_sorted[e0 + i] = (int32_t)(v >> 12);
+    pos[e0 + i] = (uint16_t)(v & 0xfffu);
+  }
+}
+
+}  // namespace
+
+int build_tile_sort(const Schedule& s, int chunk, const int32_t* d_col, int32_t* d_col_sorted,
+                    uint16_t* d_pos, hipStream_t stream) {
+  if (chunk <= 0 || chunk > kSortMax) {
+    set_error("build_tile_sort: chunk must be in [1, 4096]");
+    return GD_ERR_INVALID;
+  }
+  std::vector<int2> segs;
+  segs.reserve(s.tiles.size() + 16);
+  for (const Tile& t : s.tiles) {
+    for (int32_t e = t.z; e < t.w; e += chunk) {
+      const int32_t n = std::min(chunk, t.w - e);
+      if (n > kSortMax) {
+        set_error("build_tile_sort: a tile is longer than 4096 edges");
+        return GD_ERR_INVALID;
+      }
+      segs.push_back(make_int2(e, n));
+    }
+  }
+  if (segs.empty()) return GD_OK;
+  DeviceBuffer d_segs;
+  int rc = d_segs.alloc(sizeof(int2) * segs.size());
+  if (rc) return rc;
+  GD_HIP_TRY(hipMemcpyAsync(d_segs.ptr, segs.data(), sizeof(int2) * segs.size(),
+                            hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(segment_sort_kernel, dim3((unsigned)segs.size()), dim3(kSortBlock), 0, stream,
+                     d_segs.as<int2>(), d_col, d_col_sorted, d_pos);
+  GD_HIP_TRY(hipGetLastError());
+  GD_HIP_TRY(hipStreamSynchronize(stream));  // d_segs is freed on return
+  return GD_OK;
+}
+
+}  // namespace gd

@@ -73,13 +73,19 @@ typedef struct gd_pr_plan_config {
   int persistent;       /* 1: a persistent grid walks the tiles and prefetches the next
                            tile's loads during the current tile's reduction; 0: one
                            workgroup per tile; -1: auto (0) */
+  int sort_tiles;       /* 1: each tile's edges are gathered in increasing source order
+                           (lanes reading neighbouring sources share cache lines), each
+                           value placed back at its CSR position in LDS: same results,
+                           fewer L2 requests; costs 2 bytes per edge of plan memory and of
+                           index stream; 0: CSR order; -1: auto (1) */
 } gd_pr_plan_config;
 
 void gd_pr_default_plan_config(gd_pr_plan_config *cfg);
 
 typedef struct gd_pr_plan_info_t {
   int64_t n_tiles;        /* workgroups per pull launch */
-  int64_t n_hubs;         /* rows longer than a tile */
+  int64_t n_hubs;         /* rows summed as hubs: longer than 2048 edges (whole-row
+                             tiles in the reference's summation order) */
   int64_t n_hub_segments; /* tiles spent on hub rows */
   int32_t max_row_len;
   int32_t relabel;        /* 1 if the plan relabelled the vertices */
@@ -88,6 +94,7 @@ typedef struct gd_pr_plan_info_t {
   int32_t persistent;
   int32_t pull_grid;      /* workgroups per pull launch */
   float plan_ms;          /* plan construction time (validation, relabel, schedule) */
+  int32_t sort_tiles;     /* 1 if tiles are gathered in source order */
 } gd_pr_plan_info_t;
 
 /* Plan over a device-resident in-CSR (rows = destinations, columns = sources)

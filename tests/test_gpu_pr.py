@@ -269,9 +269,10 @@ def test_pr_relabel_personalised_seed(device):
 
 
 def test_pr_kernel_variants_bit_identical(device):
-    """Persistent/pipelined vs one-workgroup-per-tile, every cache-hint level
-    and both tile sizes run the same arithmetic in the same order (RMAT-16 has
-    whole-row exact hubs, so both kernels' hub paths are covered)."""
+    """Persistent/pipelined vs one-workgroup-per-tile, every cache-hint level,
+    both tile sizes and gathering in source order (sorted tiles) run the same
+    arithmetic in the same order (RMAT-16 has whole-row exact hubs, so both
+    kernels' hub paths are covered)."""
     import torch
 
     out_csr, in_csr = graph.rmat(16, 16, seed=11)
@@ -283,18 +284,45 @@ def test_pr_kernel_variants_bit_identical(device):
     for persistent in (False, True):
         for nt in (0, 1, 2):
             for ipt in (8, 16):
-                plan = PageRankPlan(off, col, deg, relabel=True, items_per_thread=ipt,
-                                    nontemporal=nt, persistent=persistent)
-                inf = plan.info()
-                assert inf["persistent"] == int(persistent) and inf["nontemporal"] == nt
-                s = torch.full((m,), 1.0 / m, dtype=torch.float32, device=device)
-                st, errs = plan.run(s, tol=1e-5, max_iter=50)
-                results.append((ipt, s.cpu().numpy(), st["iterations"]))
-                plan.close()
+                for srt in (False, True):
+                    if srt and nt == 2:
+                        continue
+                    plan = PageRankPlan(off, col, deg, relabel=True, items_per_thread=ipt,
+                                        nontemporal=nt, persistent=persistent, sort_tiles=srt)
+                    inf = plan.info()
+                    assert inf["persistent"] == int(persistent) and inf["nontemporal"] == nt
+                    assert inf["sort_tiles"] == int(srt)
+                    s = torch.full((m,), 1.0 / m, dtype=torch.float32, device=device)
+                    st, errs = plan.run(s, tol=1e-5, max_iter=50)
+                    results.append((ipt, s.cpu().numpy(), st["iterations"]))
+                    plan.close()
     # the tile size does not change any row's summation order either
     for r in results[1:]:
         np.testing.assert_array_equal(r[1], results[0][1])
         assert r[2] == results[0][2]
+
+
+@pytest.mark.parametrize("relabel", [False, True])
+def test_pr_sorted_tiles_bit_identical(device, relabel):
+    """Tiles gathered in source order give the CSR-order results bit for bit,
+    with and without the relabelled private copy (sorted in place)."""
+    out_csr, in_csr = graph.rmat(15, 16, seed=12)
+    a, sa, ea, ia = _gpu_pr(device, in_csr, out_csr.degree, 1e-6, 80, relabel=relabel)
+    import torch
+
+    off = torch.from_numpy(in_csr.row_offsets).to(device)
+    col = torch.from_numpy(in_csr.column_indices).to(device)
+    col_before = col.clone()
+    deg = torch.from_numpy(out_csr.degree).to(device)
+    for srt in (False, True):
+        plan = PageRankPlan(off, col, deg, relabel=relabel, sort_tiles=srt)
+        s = torch.full((in_csr.m,), float(np.float32(1.0) / np.float32(in_csr.m)),
+                       dtype=torch.float32, device=device)
+        st, errs = plan.run(s, tol=1e-6, max_iter=80)
+        np.testing.assert_array_equal(s.cpu().numpy(), a)
+        assert st["iterations"] == sa["iterations"]
+        plan.close()
+    assert torch.equal(col, col_before)  # the caller's column array is never reordered
 
 
 def test_pr_persistent_tiny_and_empty_tail(device):
