@@ -351,6 +351,7 @@ def main():
     ap.add_argument("--ipt", type=int, default=0, help="edges per thread per tile (8/16, 0=auto)")
     ap.add_argument("--nt", type=int, default=-1, help="non-temporal level 0/1/2 (-1 auto)")
     ap.add_argument("--persistent", choices=["auto", "on", "off"], default="auto")
+    ap.add_argument("--sort-tiles", choices=["auto", "on", "off"], default="auto")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the tol-1e-6 run, SpMV (ER 2^22 x 32), batched-PPR (RMAT-22, "
                          "k=256), multi-graph and adjoint figures")
@@ -398,7 +399,8 @@ def main():
     relabel = {"auto": None, "on": True, "off": False}[args.relabel]
     plan = PageRankPlan(off, col, deg, stream=stream, relabel=relabel, items_per_thread=args.ipt,
                         nontemporal=None if args.nt < 0 else args.nt,
-                        persistent={"auto": None, "on": True, "off": False}[args.persistent])
+                        persistent={"auto": None, "on": True, "off": False}[args.persistent],
+                        sort_tiles={"auto": None, "on": True, "off": False}[args.sort_tiles])
     info = plan.info()
     log(f"plan: {info}")
     init = float(np.float32(1.0) / np.float32(V))
@@ -429,7 +431,10 @@ def main():
     roof = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(args.scale),
-        "kernel": "pr_pull_kernel", "avg_launch_ms": round(avg_pull_ms, 5),
+        "kernel": (f"{'pull_persistent_kernel' if info['persistent'] else 'pull_tile_kernel'}"
+                   f"<PrPolicy, {info['items_per_thread']}, {info['nontemporal']}, "
+                   f"{'true' if info['sort_tiles'] else 'false'}>"),
+        "avg_launch_ms": round(avg_pull_ms, 5),
         "algorithmic_bytes_per_launch": bytes_iter,
     }
     extras = {}
@@ -465,7 +470,7 @@ def main():
                    "abcd": [0.57, 0.19, 0.19, 0.05], "parallelism": "single-gpu",
                    "relabel": bool(info["relabel"]), "items_per_thread": info["items_per_thread"],
                    "nontemporal": info["nontemporal"], "persistent": bool(info["persistent"]),
-                   "pull_grid": info["pull_grid"],
+                   "pull_grid": info["pull_grid"], "sort_tiles": bool(info["sort_tiles"]),
                    "plan_ms": round(info["plan_ms"], 1)},
         "roofline": roof, "cpu_baseline": cpu,
     }

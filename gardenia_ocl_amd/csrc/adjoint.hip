@@ -382,8 +382,13 @@ int gd_wpr_backward(gd_wpr_plan* p, const float* d_x, const float* d_w, float* d
     hipLaunchKernelGGL(pack_cq_kernel, dim3(grid_n(m)), dim3(256), 0, p->stream, m, d_x,
                        p->s.as<float>(), p->atl.as<float>(), p->cq.as<float2>());
     const WprGradOp op{p->cq.as<float2>(), d_lambda, d_grad, o.damping};
-    hipLaunchKernelGGL(edge_map_kernel<WprGradOp>, dim3(p->fwd.n_tiles), dim3(kBlock), 0,
-                       p->stream, p->fwd.tiles.as<Tile>(), p->off, p->col, op);
+    // every edge once: the forward's row tiles, then its whole-row hub tiles
+    if (p->fwd.n_tiles)
+      hipLaunchKernelGGL(edge_map_kernel<WprGradOp>, dim3(p->fwd.n_tiles), dim3(kBlock), 0,
+                         p->stream, p->fwd.tiles.as<Tile>(), p->off, p->col, op);
+    if (p->fwd.n_hub_tiles)
+      hipLaunchKernelGGL(edge_map_kernel<WprGradOp>, dim3(p->fwd.n_hub_tiles), dim3(kBlock), 0,
+                         p->stream, p->fwd.hub_tiles.as<Tile>(), p->off, p->col, op);
     GD_HIP_TRY(hipGetLastError());
   }
   GD_HIP_TRY(hipEventRecord(t1, p->stream));

@@ -33,13 +33,20 @@ struct IterEngine {
   int e_max = 0;
   DeviceBuffer dummy_col, col_sorted, pos;
   Schedule sched;
-  int n_tiles = 0, n_hubs = 0, fin_blocks = 1;
-  DeviceBuffer tiles, hub_row, hub_slot, hub_nseg, hub_part, err_part, fin_part, state, errors;
+  int n_tiles = 0, n_hubs = 0, n_hub_tiles = 0, fin_blocks = 1;
+  DeviceBuffer tiles, hub_tiles, hub_row, hub_slot, hub_nseg, hub_part, err_part, fin_part, state,
+      errors;
   int errors_cap = 0;
   std::vector<hipEvent_t> ev;
+  // exact hubs run on a second stream, forked from / joined to `stream`
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 
   ~IterEngine() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
+    if (side) (void)hipStreamDestroy(side);
   }
 
   // Schedule and scratch over a validated CSR already in HBM (offsets are
@@ -64,7 +71,6 @@ struct IterEngine {
     GD_HIP_TRY(hipStreamSynchronize(stream));
     rc = build_schedule(m, nnz, h_off.data(), kBlock * ipt, kTileRows, &sched, exact_len);
     if (rc) return rc;
-    n_tiles = (int)sched.tiles.size();
     n_hubs = (int)sched.hub_row.size();
     rc = pos.alloc(sort_tiles ? 2 * (size_t)std::max<int64_t>(nnz, 1) : 2);
     if (rc) return rc;
@@ -79,12 +85,30 @@ struct IterEngine {
       if (rc) return rc;
       col = dst;
     }
-    rc = tiles.alloc(sizeof(Tile) * n_tiles);
+    // exact hubs: their own tile list (longest first) and kernel
+    std::vector<Tile> row_tiles, hubs;
+    for (const Tile& t : sched.tiles) ((exact_len > 0 && t.y < 0) ? hubs : row_tiles).push_back(t);
+    std::stable_sort(hubs.begin(), hubs.end(),
+                     [](const Tile& a, const Tile& b) { return a.w - a.z > b.w - b.z; });
+    sched.tiles = row_tiles;
+    n_tiles = (int)row_tiles.size();
+    n_hub_tiles = (int)hubs.size();
+    if (n_hub_tiles) {
+      rc = hub_tiles.alloc(sizeof(Tile) * n_hub_tiles);
+      if (rc) return rc;
+      GD_HIP_TRY(hipMemcpyAsync(hub_tiles.ptr, hubs.data(), sizeof(Tile) * n_hub_tiles,
+                                hipMemcpyHostToDevice, stream));
+      GD_HIP_TRY(hipStreamSynchronize(stream));  // host vectors go out of scope
+      GD_HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+      GD_HIP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+      GD_HIP_TRY(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+    }
+    rc = tiles.alloc(sizeof(Tile) * std::max(1, n_tiles));
     if (!rc) rc = hub_row.alloc(4 * (size_t)std::max(1, n_hubs));
     if (!rc) rc = hub_slot.alloc(4 * (size_t)std::max(1, n_hubs));
     if (!rc) rc = hub_nseg.alloc(4 * (size_t)std::max(1, n_hubs));
     if (!rc) rc = hub_part.alloc(4 * (size_t)std::max(1, sched.n_slots));
-    if (!rc) rc = err_part.alloc(8 * (size_t)std::max(1, n_tiles));
+    if (!rc) rc = err_part.alloc(8 * ((size_t)std::max(1, n_tiles) + n_hub_tiles));
     if (!rc) rc = fin_part.alloc(8 * (size_t)kFinBlocks);
     if (!rc) rc = state.alloc(sizeof(IterState));
     if (rc) return rc;
@@ -113,30 +137,44 @@ struct IterEngine {
     } else {
       pull_grid = n_tiles;
     }
-    fin_blocks = std::max(1, std::min(kFinBlocks, std::max(pull_grid, n_hubs) / kBlock));
+    fin_blocks = std::max(
+        1, std::min(kFinBlocks, std::max(pull_grid + n_hub_tiles, n_hubs) / kBlock));
     return GD_OK;
   }
 
   IterArgs iter_args() {
-    return IterArgs{tiles.as<Tile>(), n_tiles, off, col, e_max, pos.as<uint16_t>(),
+    return IterArgs{tiles.as<Tile>(), n_tiles, hub_tiles.as<Tile>(), n_hub_tiles,
+                    n_tiles > 0 ? pull_grid : 0, off, col, e_max, pos.as<uint16_t>(),
                     hub_part.as<float>(), err_part.as<double>(), state.as<IterState>()};
   }
 
   // exact hubs finish their rows in the pull launch: no hub epilogues here
+  // L1 partials: the row-tile launch's (pull_grid of them) then the exact hubs'
   FinArgs fin_args() {
     return FinArgs{hub_row.as<int32_t>(), hub_slot.as<int32_t>(), hub_nseg.as<int32_t>(),
-                   exact_len > 0 ? 0 : n_hubs, pull_grid, errors.as<double>(),
-                   fin_part.as<double>()};
+                   exact_len > 0 ? 0 : n_hubs, (n_tiles > 0 ? pull_grid : 0) + n_hub_tiles,
+                   errors.as<double>(), fin_part.as<double>()};
   }
 
   template <class P, int IPT, int NT, bool SORTED>
   void launch_pull_s(const typename P::Args& a, const IterArgs& it) {
-    if (persistent)
-      hipLaunchKernelGGL((pull_persistent_kernel<P, IPT, NT, SORTED>), dim3(pull_grid),
-                         dim3(kBlock), 0, stream, a, it);
-    else
-      hipLaunchKernelGGL((pull_tile_kernel<P, IPT, NT, SORTED>), dim3(n_tiles), dim3(kBlock), 0,
-                         stream, a, it);
+    if (n_hub_tiles) {  // fork: exact hubs on the side stream
+      (void)hipEventRecord(fork_ev, stream);
+      (void)hipStreamWaitEvent(side, fork_ev, 0);
+      if constexpr (P::kExactHubs)
+        hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED>), dim3(n_hub_tiles),
+                           dim3(kBlock), 0, side, a, it);
+      (void)hipEventRecord(join_ev, side);
+    }
+    if (n_tiles > 0) {
+      if (persistent)
+        hipLaunchKernelGGL((pull_persistent_kernel<P, IPT, NT, SORTED>), dim3(pull_grid),
+                           dim3(kBlock), 0, stream, a, it);
+      else
+        hipLaunchKernelGGL((pull_tile_kernel<P, IPT, NT, SORTED>), dim3(n_tiles), dim3(kBlock), 0,
+                           stream, a, it);
+    }
+    if (n_hub_tiles) (void)hipStreamWaitEvent(stream, join_ev, 0);  // join
   }
 
   template <class P, int IPT, int NT>

@@ -28,7 +28,7 @@
 namespace gd {
 namespace {
 
-constexpr int kMgIPT = 8;
+constexpr int kMgIPT = 16;  // as the single-graph plan: same tiles, same per-row sums
 
 struct GraphState {
   int iter;
@@ -235,7 +235,7 @@ int gd_multi_pr_plan_create(int n_graphs, const int32_t* m, const int64_t* nnz,
     GD_HIP_TRY(hipStreamSynchronize(p->stream));
     Schedule s;
     rc = build_schedule(m[g], nnz[g], h_off.data(), kBlock * kMgIPT, kTileRows, &s,
-                        kBlock * kMgIPT);  // rows > 2048 edges: whole-row exact hubs
+                        kBlock * kMgIPT);  // rows longer than a tile: whole-row exact hubs
     if (rc) return fail(rc);
     hipLaunchKernelGGL(mg_pack_kernel, dim3(grid_for(std::max<int64_t>(nnz[g], m[g]))),
                        dim3(256), 0, p->stream, d_off[g], d_col[g], d_outdeg[g], m[g], nnz[g], vb,

@@ -58,12 +58,6 @@ struct gd_pr_plan {
 
 namespace {
 
-// Rows longer than this are whole-row hub tiles summed with exact_seq_sum:
-// the reference's sequential FP32 sum drifts on long rows (3e-4 relative on a
-// 238,668-edge RMAT-24 row), and matching it keeps every score within 1e-5
-// of the reference; shorter rows stay tree-summed (<= 3e-6 difference there).
-constexpr int kExactHubLen = 2048;
-
 int build_plan(gd_pr_plan* p, const int32_t* d_off, const int32_t* d_col,
                const int32_t* d_outdeg) {
   const int m = p->m;
@@ -160,8 +154,12 @@ int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t
   p->eng.nt = c.nontemporal < 0 ? 0 : c.nontemporal;
   p->eng.persistent = c.persistent == 1;
   p->eng.sort_tiles = c.sort_tiles != 0;
-  // rows longer than 2048 edges are summed in the reference's order and rounding
-  p->eng.exact_len = kExactHubLen;
+  // Rows longer than a tile (256*ipt edges) are whole-row hubs summed in the
+  // reference's order and rounding: its sequential FP32 sum drifts on long
+  // rows (3e-4 relative on the 238,668-edge RMAT-24 row), and matching it
+  // keeps every score within 1e-5 of the reference; shorter rows stay
+  // tree-summed (<= 4e-6 difference on RMAT-24).
+  p->eng.exact_len = kBlock * p->eng.ipt;
   int rc = GD_OK;
   if (m > 0) {
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -191,7 +189,7 @@ int gd_pr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* d
 
 int gd_pr_plan_info(const gd_pr_plan* p, gd_pr_plan_info_t* info) {
   GD_CHECK_ARG(p != nullptr && info != nullptr, "NULL plan / info");
-  info->n_tiles = p->eng.n_tiles;
+  info->n_tiles = p->eng.n_tiles + p->eng.n_hub_tiles;
   info->n_hubs = p->eng.n_hubs;
   info->n_hub_segments = p->eng.sched.n_slots;
   info->max_row_len = p->eng.sched.max_row_len;

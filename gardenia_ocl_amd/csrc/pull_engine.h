@@ -383,6 +383,90 @@ __device__ __forceinline__ int exact_pos(int i) {
   return i + i / IPT;
 }
 
+// One chunk of a sequential-order sum: nc values already in LDS (val, padded
+// layout exact_pos), added to s in index order with sequential FP32 rounding.
+// `head`: add the first kExactHead values serially first (the start of a row,
+// where s crosses a binade at almost every doubling).  All threads return the
+// same s.  Ends with the workgroup synchronised (val may be rewritten).
+template <int IPT>
+__device__ float exact_scan_chunk(const float* __restrict__ val, int nc, bool head, float s,
+                                  ExactLds& x) {
+  const int tid = threadIdx.x;
+  const int lo = tid * IPT, hi = min(lo + IPT, nc);  // this thread's elements
+  int start = 0;
+  if (head) {
+    const int ns = min(nc, kExactHead);
+    if (tid == 0) {
+      float a = s;
+      for (int i = 0; i < ns; i++) a += val[exact_pos<IPT>(i)];
+      x.s = a;
+    }
+    __syncthreads();
+    s = x.s;
+    __syncthreads();
+    start = ns;
+  }
+  while (start < nc) {  // s and start are uniform across the workgroup
+    if (!(fabsf(s) >= 0x1p-125f) || !isfinite(s)) {  // zero, subnormal, inf/NaN
+      if (s == 0.0f && !signbit(s)) {
+        // +0 + (+-0) == +0: jump to the first nonzero term (x != 0: +0 + x == x)
+        int nz = 0x7fffffff;
+        for (int i = max(lo, start); i < hi; i++)
+          if (val[exact_pos<IPT>(i)] != 0.0f) {
+            nz = i;
+            break;
+          }
+        const int jz = block_min(nz, x);
+        if (jz == 0x7fffffff) {
+          start = nc;
+        } else {
+          s = val[exact_pos<IPT>(jz)];
+          start = jz + 1;
+        }
+        continue;
+      }
+      s = s + val[exact_pos<IPT>(start)];
+      start++;
+      continue;
+    }
+    const int ex = ilogbf(s);
+    // s = P * 2^(ex-23) with 2^23 <= |P| < 2^24
+    const int32_t P = (int32_t)(__float_as_uint(s) & 0x7fffffu) | 0x800000;
+    const int32_t Ps = signbit(s) ? -P : P;
+    uint32_t L = 0;
+    for (int i = max(lo, start); i < hi; i++) {
+      bool irr;
+      L += (uint32_t)ulps_of(val[exact_pos<IPT>(i)], ex, irr);
+    }
+    int32_t total;
+    int32_t run = (int32_t)((uint32_t)Ps + (uint32_t)block_excl_scan((int32_t)L, total, x));
+    int bad = 0x7fffffff;
+    for (int i = max(lo, start); i < hi; i++) {
+      bool irr;
+      const int32_t r = (int32_t)((uint32_t)run + (uint32_t)ulps_of(val[exact_pos<IPT>(i)], ex, irr));
+      if (irr || !(abs(r) > (1 << 23) && abs(r) < (1 << 24))) {
+        bad = i;
+        break;
+      }
+      run = r;
+    }
+    const int jb = block_min(bad, x);
+    if (jb == 0x7fffffff) {
+      // exact: an integer below 2^24 in magnitude times a power of two
+      s = ldexpf((float)(int32_t)((uint32_t)Ps + (uint32_t)total), ex - 23);
+      start = nc;
+    } else {
+      if (bad == jb) x.s = ldexpf((float)run, ex - 23) + val[exact_pos<IPT>(jb)];  // FP32 step
+      __syncthreads();
+      s = x.s;
+      __syncthreads();
+      start = jb + 1;
+    }
+  }
+  __syncthreads();
+  return s;
+}
+
 // Sum of the gathered values of edges [e0, e0+ne) in edge order, rounded as the
 // sequential FP32 loop starting from s.  All threads return the same value.
 // `val` holds kBlock*IPT + kBlock floats.
@@ -407,84 +491,11 @@ __device__ float exact_seq_sum(const int32_t* __restrict__ col, const uint16_t* 
 #pragma unroll
     for (int j = 0; j < IPT; j++)
       v[j] = g.value(e0 + c0 + (SORTED ? p[j] : min(tid + j * kBlock, nc - 1)), idx[j]);
-    __syncthreads();  // the previous chunk's readers are done with val
 #pragma unroll
     for (int j = 0; j < IPT; j++)
       if (tid + j * kBlock < nc) val[exact_pos<IPT>(SORTED ? p[j] : tid + j * kBlock)] = v[j];
     __syncthreads();
-    const int lo = tid * IPT, hi = min(lo + IPT, nc);  // this thread's elements
-    int start = 0;
-    if (c0 == 0) {
-      // the first terms move s across a binade at almost every doubling: one
-      // lane adds them in order (the loop itself, no restarts)
-      const int ns = min(nc, kExactHead);
-      if (tid == 0) {
-        float a = s;
-        for (int i = 0; i < ns; i++) a += val[exact_pos<IPT>(i)];
-        x.s = a;
-      }
-      __syncthreads();
-      s = x.s;
-      __syncthreads();
-      start = ns;
-    }
-    while (start < nc) {  // s and start are uniform across the workgroup
-      if (!(fabsf(s) >= 0x1p-125f) || !isfinite(s)) {  // zero, subnormal, inf/NaN
-        if (s == 0.0f && !signbit(s)) {
-          // +0 + (+-0) == +0: jump to the first nonzero term (x != 0: +0 + x == x)
-          int nz = 0x7fffffff;
-          for (int i = max(lo, start); i < hi; i++)
-            if (val[exact_pos<IPT>(i)] != 0.0f) {
-              nz = i;
-              break;
-            }
-          const int jz = block_min(nz, x);
-          if (jz == 0x7fffffff) {
-            start = nc;
-          } else {
-            s = val[exact_pos<IPT>(jz)];
-            start = jz + 1;
-          }
-          continue;
-        }
-        s = s + val[exact_pos<IPT>(start)];
-        start++;
-        continue;
-      }
-      const int ex = ilogbf(s);
-      // s = P * 2^(ex-23) with 2^23 <= |P| < 2^24
-      const int32_t P = (int32_t)(__float_as_uint(s) & 0x7fffffu) | 0x800000;
-      const int32_t Ps = signbit(s) ? -P : P;
-      uint32_t L = 0;
-      for (int i = max(lo, start); i < hi; i++) {
-        bool irr;
-        L += (uint32_t)ulps_of(val[exact_pos<IPT>(i)], ex, irr);
-      }
-      int32_t total;
-      int32_t run = (int32_t)((uint32_t)Ps + (uint32_t)block_excl_scan((int32_t)L, total, x));
-      int bad = 0x7fffffff;
-      for (int i = max(lo, start); i < hi; i++) {
-        bool irr;
-        const int32_t r = (int32_t)((uint32_t)run + (uint32_t)ulps_of(val[exact_pos<IPT>(i)], ex, irr));
-        if (irr || !(abs(r) > (1 << 23) && abs(r) < (1 << 24))) {
-          bad = i;
-          break;
-        }
-        run = r;
-      }
-      const int jb = block_min(bad, x);
-      if (jb == 0x7fffffff) {
-        // exact: an integer below 2^24 in magnitude times a power of two
-        s = ldexpf((float)(int32_t)((uint32_t)Ps + (uint32_t)total), ex - 23);
-        start = nc;
-      } else {
-        if (bad == jb) x.s = ldexpf((float)run, ex - 23) + val[exact_pos<IPT>(jb)];  // FP32 step
-        __syncthreads();
-        s = x.s;
-        __syncthreads();
-        start = jb + 1;
-      }
-    }
+    s = exact_scan_chunk<IPT>(val, nc, c0 == 0, s, x);
   }
   return s;
 }

@@ -40,8 +40,11 @@ struct IterState {
 };
 
 struct IterArgs {
-  const Tile* __restrict__ tiles;
+  const Tile* __restrict__ tiles;  // row tiles (+ hub segments of tree-summing policies)
   int n_tiles;
+  const Tile* __restrict__ hub_tiles;  // whole-row exact hubs (exact_hub_kernel)
+  int n_hub_tiles;
+  int hub_part_base;  // exact hub h writes its L1 partial to err_part[hub_part_base + h]
   const int32_t* __restrict__ off;
   const int32_t* __restrict__ col;
   int e_max;  // last valid index of col (a 1-element dummy when nnz == 0)
@@ -78,7 +81,6 @@ __global__ __launch_bounds__(kBlock) void pull_tile_kernel(const typename P::Arg
   const int odd = it.st->iter & 1;
   __shared__ RowTileLds<IPT> lds;
   __shared__ double s_err[kWaves];
-  __shared__ ExactLds xs;
   const Tile t = it.tiles[blockIdx.x];
   const auto g = P::template gather<NT>(a, odd);
   double err = 0.0;
@@ -86,13 +88,7 @@ __global__ __launch_bounds__(kBlock) void pull_tile_kernel(const typename P::Arg
     const auto e = P::template epilogue<NT>(a, odd);
     err = row_tile<IPT, (NT > 0), SORTED>(it.off, it.col, it.pos, it.e_max, t.x, t.y, t.z,
                                           t.w - t.z, g, e, lds);
-  } else if constexpr (P::kExactHubs) {
-    const auto e = P::template epilogue<NT>(a, odd);
-    const auto pre = e.pre(t.x);
-    const float sum = exact_seq_sum<IPT, (NT > 0), SORTED>(it.col, it.pos, it.e_max, t.z,
-                                                           t.w - t.z, g, e.init(pre), lds.val, xs);
-    if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
-  } else {
+  } else if constexpr (!P::kExactHubs) {
     const float p = segment_sum<IPT, (NT > 0), SORTED>(it.col, it.pos, t.z, t.w - t.z, g, lds.val);
     if (threadIdx.x == 0) it.hub_part[-t.y - 1] = p;
   }
@@ -113,7 +109,6 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
   const int odd = it.st->iter & 1;
   __shared__ RowTileLds<IPT> lds;
   __shared__ double s_err[kWaves];
-  __shared__ ExactLds xs;
   const auto g = P::template gather<NT>(a, odd);
   const auto e = P::template epilogue<NT>(a, odd);
   using E = decltype(e);
@@ -138,12 +133,7 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
         stage_row_tile<IPT, SORTED>(cur, R, v, lds);
         __syncthreads();
         err += reduce_row_tile<IPT>(cur.x, cur.y - cur.x, e, R.pre, lds);
-      } else if constexpr (P::kExactHubs) {
-        const auto pre = e.pre(cur.x);
-        const float sum = exact_seq_sum<IPT, (NT > 0), SORTED>(it.col, it.pos, it.e_max, cur.z,
-                                                               ne, g, e.init(pre), lds.val, xs);
-        if (threadIdx.x == 0) err += e.finish(cur.x, sum, pre);
-      } else {
+      } else if constexpr (!P::kExactHubs) {
         float sacc = 0.0f;
 #pragma unroll
         for (int j = 0; j < IPT; j++) sacc += ((int)threadIdx.x + j * kBlock < ne) ? v[j] : 0.0f;
@@ -159,6 +149,31 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
   }
   const double tot = block_sum<kWaves>(err, s_err);
   if (threadIdx.x == 0) it.err_part[blockIdx.x] = tot;
+}
+
+// Whole-row hubs of an exact-hub policy: one workgroup per hub row gathers the
+// row chunk by chunk and adds it in edge order with the reference's sequential
+// rounding (exact_seq_sum), then runs the row's epilogue.  A kernel of its own,
+// launched on a second stream concurrently with the row-tile launch: its
+// register demand would otherwise lower the occupancy of every row tile.
+template <class P, int IPT, int NT, bool SORTED>
+__global__ __launch_bounds__(kBlock) void exact_hub_kernel(const typename P::Args a,
+                                                           const IterArgs it) {
+  if (it.st->done) return;
+  const int odd = it.st->iter & 1;
+  __shared__ RowTileLds<IPT> lds;
+  __shared__ double s_err[kWaves];
+  __shared__ ExactLds xs;
+  const Tile t = it.hub_tiles[blockIdx.x];
+  const auto g = P::template gather<NT>(a, odd);
+  const auto e = P::template epilogue<NT>(a, odd);
+  const auto pre = e.pre(t.x);
+  const float sum = exact_seq_sum<IPT, (NT > 0), SORTED>(it.col, it.pos, it.e_max, t.z, t.w - t.z,
+                                                         g, e.init(pre), lds.val, xs);
+  double err = 0.0;
+  if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
+  const double tot = block_sum<kWaves>(err, s_err);
+  if (threadIdx.x == 0) it.err_part[it.hub_part_base + blockIdx.x] = tot;
 }
 
 // Hub-row epilogues (segment partials summed in segment order) and the L1

@@ -83,7 +83,7 @@ typedef struct gd_pr_plan_config {
 void gd_pr_default_plan_config(gd_pr_plan_config *cfg);
 
 typedef struct gd_pr_plan_info_t {
-  int64_t n_tiles;        /* workgroups per pull launch */
+  int64_t n_tiles;        /* workgroups per iteration: row tiles + whole-row hub tiles */
   int64_t n_hubs;         /* rows summed as hubs: longer than 2048 edges (whole-row
                              tiles in the reference's summation order) */
   int64_t n_hub_segments; /* tiles spent on hub rows */

@@ -43,7 +43,7 @@ _SIG = {
     "orc_spmv": (None, [_i, _P, _P, _P, _P, _P]),
     "orc_max_rel_err": (_f, [_P, _P, _i64]),
     "orc_spmv_bwd_fp64": (None, [_i, _i, _P, _P, _P, _P, _P, _P, _P]),
-    "orc_wpr_fp64": (_i, [_i, _P, _P, _P, _P, _d, _d, _i]),
+    "orc_wpr_fp64": (_i, [_i, _P, _P, _P, _P, _d, _d, _i, _i]),
     "orc_wpr_grad_fp64": (_i, [_i, _P, _P, _P, _P, _P, _d, _d, _i, _P, _P]),
     "orc_num_threads": (_i, []),
 }
@@ -185,11 +185,13 @@ def spmv_bwd_fp64(rows, cols, Ap, Aj, Ax, x, g):
     return dx, dA
 
 
-def wpr_fp64(m, in_off, in_col, a, *, damping=0.85, tol=1e-14, max_iter=2000):
+def wpr_fp64(m, in_off, in_col, a, *, damping=0.85, tol=1e-14, max_iter=2000, x0=None):
+    """FP64 weighted PR (Jacobi); x0: start vector (default 1/m)."""
     in_off, in_col = _i32(in_off), _i32(in_col)
     a = np.ascontiguousarray(a, np.float64)
-    x = np.zeros(m, np.float64)
-    it = lib().orc_wpr_fp64(m, _p(in_off), _p(in_col), _p(a), _p(x), damping, tol, max_iter)
+    x = np.zeros(m, np.float64) if x0 is None else np.array(x0, np.float64)
+    it = lib().orc_wpr_fp64(m, _p(in_off), _p(in_col), _p(a), _p(x), damping, tol, max_iter,
+                            0 if x0 is None else 1)
     return x, it
 
 

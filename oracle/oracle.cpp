@@ -369,17 +369,22 @@ void orc_spmv_bwd_fp64(int num_rows, int num_cols, const int* Ap, const int* Aj,
 // Weighted pull PageRank in FP64.  Input is the IN-CSR (row v lists sources u)
 // with raw edge weights a_uv; s_u = sum_v a_uv is the source's out-strength.
 //   x = (1-d)/m + d * P x,   P[v,u] = a_uv / s_u   (dangling mass dropped).
+// warm != 0: iterate from the x passed in (e.g. the unperturbed solution when
+// differencing), else from 1/m.  Jacobi sweeps (c is a snapshot of x).
 int orc_wpr_fp64(int m, const int* row_offsets, const int* column_indices, const double* a,
-                 double* x, double damp, double tol, int max_iter) {
+                 double* x, double damp, double tol, int max_iter, int warm) {
   std::vector<double> s(m, 0.0), c(m);
   long long nnz = row_offsets[m];
   for (long long e = 0; e < nnz; e++) s[column_indices[e]] += a[e];
-  for (int i = 0; i < m; i++) x[i] = 1.0 / m;
+  if (!warm)
+    for (int i = 0; i < m; i++) x[i] = 1.0 / m;
   const double base = (1.0 - damp) / m;
   int iter;
   for (iter = 0; iter < max_iter; iter++) {
     double err = 0;
+#pragma omp parallel for
     for (int u = 0; u < m; u++) c[u] = s[u] > 0 ? x[u] / s[u] : 0.0;
+#pragma omp parallel for reduction(+ : err) schedule(dynamic, 256)
     for (int v = 0; v < m; v++) {
       double acc = 0;
       for (int e = row_offsets[v]; e < row_offsets[v + 1]; e++) acc += a[e] * c[column_indices[e]];
@@ -403,15 +408,34 @@ int orc_wpr_grad_fp64(int m, const int* row_offsets, const int* column_indices, 
   std::vector<double> s(m, 0.0), t(m);
   long long nnz = row_offsets[m];
   for (long long e = 0; e < nnz; e++) s[column_indices[e]] += a[e];
+  // transposed index (by source u, edges in CSR order) so P^T lambda is a
+  // parallel, order-fixed gather: t_u = sum over u's out-edges of a_uv lambda_v
+  std::vector<long long> toff(m + 1, 0);
+  for (long long e = 0; e < nnz; e++) toff[column_indices[e] + 1]++;
+  for (int u = 0; u < m; u++) toff[u + 1] += toff[u];
+  std::vector<long long> fill(toff.begin(), toff.end() - 1), tedge(nnz);
+  std::vector<int> tdst(nnz);
+  for (int v = 0; v < m; v++)
+    for (int e = row_offsets[v]; e < row_offsets[v + 1]; e++) {
+      const long long k = fill[column_indices[e]]++;
+      tedge[k] = e;
+      tdst[k] = v;
+    }
+  auto transpose_apply = [&]() {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int u = 0; u < m; u++) {
+      double acc = 0.0;
+      for (long long k = toff[u]; k < toff[u + 1]; k++) acc += a[tedge[k]] * lambda[tdst[k]];
+      t[u] = acc;
+    }
+  };
   for (int i = 0; i < m; i++) lambda[i] = w[i];
   int iter;
   for (iter = 0; iter < max_iter; iter++) {
     // (P^T lambda)_u = sum_v P[v,u] lambda_v = sum_{v: u in in(v)} a_uv lambda_v / s_u
-    std::fill(t.begin(), t.end(), 0.0);
-    for (int v = 0; v < m; v++)
-      for (int e = row_offsets[v]; e < row_offsets[v + 1]; e++)
-        t[column_indices[e]] += a[e] * lambda[v];
+    transpose_apply();
     double err = 0;
+#pragma omp parallel for reduction(+ : err)
     for (int u = 0; u < m; u++) {
       double nl = w[u] + damp * (s[u] > 0 ? t[u] / s[u] : 0.0);
       err += std::fabs(nl - lambda[u]);
@@ -420,9 +444,8 @@ int orc_wpr_grad_fp64(int m, const int* row_offsets, const int* column_indices, 
     if (err < tol) { iter++; break; }
   }
   // q_u = sum_t P[t,u] lambda_t
-  std::fill(t.begin(), t.end(), 0.0);
-  for (int v = 0; v < m; v++)
-    for (int e = row_offsets[v]; e < row_offsets[v + 1]; e++) t[column_indices[e]] += a[e] * lambda[v];
+  transpose_apply();
+#pragma omp parallel for schedule(dynamic, 256)
   for (int v = 0; v < m; v++)
     for (int e = row_offsets[v]; e < row_offsets[v + 1]; e++) {
       int u = column_indices[e];

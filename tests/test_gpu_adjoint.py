@@ -150,3 +150,69 @@ def test_spmv_backward_matches_fp64(device, kind):
     y_ref = oracle.spmv(m, csr.row_offsets, csr.column_indices, vals, x, np.zeros(m, np.float32))
     np.testing.assert_array_equal(y.detach().cpu().numpy()[csr.degree <= 64],
                                   y_ref[csr.degree <= 64])
+
+
+def test_wpr_gradient_rmat20_fp64_and_fd(device):
+    """Full size: RMAT-20, weights U(0.5,1.5) (row-normalised per source by the
+    model), L = w.x with w ~ N(0,1).  The GPU edge-weight gradient against the
+    FP64 reference gradient on every edge, and against FP64 central
+    differences (eps = 1e-3) on 32 random edges (each perturbed solve starts
+    from the unperturbed FP64 fixed point)."""
+    import torch
+
+    from gardenia_ocl_amd.autograd import WeightedPageRankPlan
+
+    out_csr, in_csr, a, off, col, ta = _weighted(20, 11, device)
+    m, nnz = in_csr.m, in_csr.nnz
+    rng = np.random.default_rng(13)
+    w = rng.standard_normal(m)
+    wp = WeightedPageRankPlan(off, col, ta)
+    x = torch.full((m,), 1.0 / m, dtype=torch.float32, device=device)
+    assert wp.forward(x, tol=1e-9, max_iter=500)["converged"]
+    lam = torch.empty_like(x)
+    grad = torch.empty(nnz, dtype=torch.float32, device=device)
+    tw = torch.from_numpy(w.astype(np.float32)).to(device)
+    assert wp.backward(x, tw, lam, grad, tol=1e-7, max_iter=500)["converged"]
+    g = grad.cpu().numpy().astype(np.float64)
+    a64 = a.astype(np.float64)
+    w64 = w.astype(np.float32).astype(np.float64)
+    x64, _ = oracle.wpr_fp64(m, in_csr.row_offsets, in_csr.column_indices, a64, tol=1e-15)
+    lam64, g64, _ = oracle.wpr_grad_fp64(m, in_csr.row_offsets, in_csr.column_indices, a64, x64,
+                                         w64)
+    scale = np.abs(g64).max()
+    assert np.abs(g - g64).max() <= 2e-4 * scale, np.abs(g - g64).max() / scale
+    eps = 1e-3
+    for e in rng.choice(nnz, 32, replace=False):
+        ap, am = a64.copy(), a64.copy()
+        ap[e] += eps
+        am[e] -= eps
+        xp, _ = oracle.wpr_fp64(m, in_csr.row_offsets, in_csr.column_indices, ap, tol=1e-15,
+                                x0=x64)
+        xm, _ = oracle.wpr_fp64(m, in_csr.row_offsets, in_csr.column_indices, am, tol=1e-15,
+                                x0=x64)
+        fd = (w64 @ (xp - xm)) / (2 * eps)
+        assert abs(g64[e] - fd) <= 1e-4 * abs(fd) + 1e-6 * scale, (e, g64[e], fd)
+        assert abs(g[e] - fd) <= 2e-3 * abs(fd) + 2e-4 * scale, (e, g[e], fd)
+
+
+def test_spmv_backward_er22(device):
+    """SpMV backward at the ER 2^22 x 2^22, 32 nnz/row shape: dA = g[row]*x[col]
+    is one FP32 product per nonzero (bit-exact), dx = A^T g against FP64."""
+    import torch
+
+    from gardenia_ocl_amd.autograd import SpmvBackwardPlan
+
+    csr = graph.er_matrix(22, 32, seed=3)
+    m = csr.m
+    x = graph.uniform_pm1(m, 4)
+    gv = graph.uniform_pm1(m, 5)
+    t = lambda v: torch.from_numpy(np.ascontiguousarray(v)).to(device)  # noqa: E731
+    Ap, Aj, Ax = t(csr.row_offsets), t(csr.column_indices), t(csr.weight)
+    bwd = SpmvBackwardPlan(Ap, Aj)
+    dx = torch.empty(m, dtype=torch.float32, device=device)
+    dA = torch.empty(csr.nnz, dtype=torch.float32, device=device)
+    bwd.run(Ax, t(x), t(gv), dx, dA)
+    rows = np.repeat(np.arange(m), np.diff(csr.row_offsets))
+    np.testing.assert_array_equal(dA.cpu().numpy(), gv[rows] * x[csr.column_indices])
+    dx64, _ = oracle.spmv_bwd_fp64(m, m, csr.row_offsets, csr.column_indices, csr.weight, x, gv)
+    assert np.abs(dx.cpu().numpy() - dx64).max() <= 1e-5 * max(1.0, np.abs(dx64).max())

@@ -20,14 +20,15 @@ def test_autotune_pagerank_and_ppr(device, tmp_path):
     off, col, deg = t(in_csr.row_offsets), t(in_csr.column_indices), t(out_csr.degree)
     cache = str(tmp_path / "tune.json")
     best, timings = autotune.tune_pagerank(off, col, deg, iters=3, repeats=2, cache=cache)
-    assert len(timings) == 16 and all(v > 0 for v in timings.values())
+    n_cfg = int(np.prod([len(v) for v in autotune.PR_SPACE.values()]))
+    assert len(timings) == n_cfg and all(v > 0 for v in timings.values())
     assert json.dumps(best, sort_keys=True) in timings
     # cached: second call does not re-time
     best2, timings2 = autotune.tune_pagerank(off, col, deg, cache=cache)
     assert best2 == best and timings2 == timings
-    # the tuned plan reproduces the default plan's scores (relabel / hints /
-    # persistence never change the arithmetic; the tile size is fixed here)
-    cfg = dict(best, items_per_thread=16)
+    # the tuned plan reproduces the default plan's scores (relabel, tile size,
+    # cache hints, persistence and sorted tiles never change the arithmetic)
+    cfg = dict(best)
     a = torch.full((in_csr.m,), 1.0 / in_csr.m, device=device)
     b = torch.full((in_csr.m,), 1.0 / in_csr.m, device=device)
     PageRankPlan(off, col, deg, **cfg).run(a, tol=1e-5, max_iter=50)

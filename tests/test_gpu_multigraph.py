@@ -36,7 +36,7 @@ def test_multigraph_equals_single_graph_plans(device, use_graph):
     scores = _init(device, host)
     its, errs, ms = mg.run(scores, tol=1e-5, max_iter=100, use_graph=use_graph)
     for (off, col, deg), (out_csr, in_csr), s, it in zip(gs, host, scores, its):
-        pr = PageRankPlan(off, col, deg, relabel=False, items_per_thread=8)
+        pr = PageRankPlan(off, col, deg, relabel=False, items_per_thread=16)
         import torch
 
         ref = torch.full((in_csr.m,), 1.0 / in_csr.m, dtype=torch.float32, device=device)

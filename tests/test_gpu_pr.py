@@ -10,7 +10,12 @@ from gardenia_ocl_amd import PRSolver, PageRankPlan, graph
 pytestmark = pytest.mark.gpu
 
 DATASETS = ["test_pr.mtx", "4.mtx", "4w.mtx", "chesapeake.mtx"]
-EXACT_LEN = 2048  # rows longer than this are summed in CSR order with FP32 sequential rounding
+
+
+def exact_len(info):
+    """Rows longer than one tile (256 threads x items_per_thread edges) are
+    hub rows, summed in CSR order with the reference's FP32 sequential rounding."""
+    return 256 * info["items_per_thread"]
 
 
 def _key(ds):
@@ -85,21 +90,21 @@ def test_pr_rmat_vs_reference(golden, device, scale):
     assert np.max(np.abs(s - ref) / ref) <= 1e-5
     if scale == 12:
         np.testing.assert_allclose(s, arrays["rmat12/scores"], rtol=1e-5, atol=0)
-    # rows longer than 2048 edges are whole-row hubs summed in the reference's order
-    assert info["hubs"] == int(np.sum(in_csr.degree > EXACT_LEN))
+    # rows longer than a tile are whole-row hubs summed in the reference's order
+    assert info["hubs"] == int(np.sum(in_csr.degree > exact_len(info)))
 
 
 @pytest.mark.parametrize("relabel,ipt", [(False, 8), (True, 16), (False, 16)])
 def test_pr_first_iteration_short_and_hub_rows_bit_exact(device, relabel, ipt):
     """After one iteration every row of <= 64 in-edges (owner-thread sum) and
-    every row of > 2048 in-edges (whole-row sequential-order sum) is
+    every row longer than a tile (whole-row sequential-order sum) is
     bit-identical to the reference arithmetic; rows in between are tree-summed."""
     out_csr, in_csr = graph.rmat(16, 16, seed=1)
     s, stats, _, info = _gpu_pr(device, in_csr, out_csr.degree, 0.0, 1, relabel=relabel, ipt=ipt)
     ref, it, _ = oracle.pr(in_csr.m, in_csr.row_offsets, in_csr.column_indices, out_csr.degree,
                            tol=0.0, max_iter=1)
-    exact = (in_csr.degree <= 64) | (in_csr.degree > EXACT_LEN)
-    assert info["hubs"] == int(np.sum(in_csr.degree > EXACT_LEN)) > 0
+    exact = (in_csr.degree <= 64) | (in_csr.degree > exact_len(info))
+    assert info["hubs"] == int(np.sum(in_csr.degree > exact_len(info))) > 0
     np.testing.assert_array_equal(s[exact], ref[exact])
     assert np.max(np.abs(s - ref) / ref) <= 1e-5
 
@@ -172,8 +177,9 @@ def _star(n_leaves, hub=0):
 def test_pr_star_hub_boundaries(device, n_leaves):
     """Hub of n in-edges, all addends equal: the reference's sequential FP32 sum
     drifts (its L1 change plateaus at ~3e-6 for n=65, ~7e-5 for n=2048 and the
-    hub score is 3.5e-5 off the FP64 fixed point).  Hubs of <= 64 or > 2048
-    in-edges are summed in the reference's order: bit-identical results.  In
+    hub score is 3.5e-5 off the FP64 fixed point).  Hubs of <= 64 in-edges or
+    longer than a tile are summed in the reference's order: bit-identical
+    results.  In
     between the tree sum is used, and the bar is the FP64 solution: the GPU
     must be at least as close to it as the reference."""
     out_csr, in_csr = _star(n_leaves)
@@ -186,10 +192,10 @@ def test_pr_star_hub_boundaries(device, n_leaves):
     assert l1_gpu <= l1_ref + 2e-7
     if ref_errs[-1] < 1e-6 and not stats["near_tie"]:
         assert stats["iterations"] == it
-    if n_leaves <= 64 or n_leaves > EXACT_LEN:  # same arithmetic as the reference
+    if n_leaves <= 64 or n_leaves > exact_len(info):  # same arithmetic as the reference
         np.testing.assert_array_equal(s, ref)
         assert stats["iterations"] == it
-    assert info["hubs"] == (1 if n_leaves > EXACT_LEN else 0)
+    assert info["hubs"] == (1 if n_leaves > exact_len(info) else 0)
 
 
 def test_pr_empty_and_edgeless(device):
@@ -269,10 +275,10 @@ def test_pr_relabel_personalised_seed(device):
 
 
 def test_pr_kernel_variants_bit_identical(device):
-    """Persistent/pipelined vs one-workgroup-per-tile, every cache-hint level,
-    both tile sizes and gathering in source order (sorted tiles) run the same
-    arithmetic in the same order (RMAT-16 has whole-row exact hubs, so both
-    kernels' hub paths are covered)."""
+    """Persistent/pipelined vs one-workgroup-per-tile, every cache-hint level
+    and gathering in source order (sorted tiles) run the same arithmetic in the
+    same order, for each tile size (RMAT-16 has rows longer than both tile
+    sizes, so the exact hub kernels run too)."""
     import torch
 
     out_csr, in_csr = graph.rmat(16, 16, seed=11)
@@ -296,10 +302,11 @@ def test_pr_kernel_variants_bit_identical(device):
                     st, errs = plan.run(s, tol=1e-5, max_iter=50)
                     results.append((ipt, s.cpu().numpy(), st["iterations"]))
                     plan.close()
-    # the tile size does not change any row's summation order either
-    for r in results[1:]:
-        np.testing.assert_array_equal(r[1], results[0][1])
-        assert r[2] == results[0][2]
+    for ipt in (8, 16):
+        rs = [r for r in results if r[0] == ipt]
+        for r in rs[1:]:
+            np.testing.assert_array_equal(r[1], rs[0][1])
+            assert r[2] == rs[0][2]
 
 
 @pytest.mark.parametrize("relabel", [False, True])
