@@ -34,9 +34,8 @@ struct IterEngine {
   DeviceBuffer dummy_col, col_sorted, pos;
   Schedule sched;
   int n_tiles = 0, n_hubs = 0, n_hub_tiles = 0, fin_blocks = 1;
-  DeviceBuffer tiles, hub_tiles, hub_chunks, hub_chunk0, staged, chunk_epoch, hub_row, hub_slot,
-      hub_nseg, hub_part, err_part, fin_part, state, errors;
-  int n_hub_chunks = 0;
+  DeviceBuffer tiles, hub_tiles, hub_row, hub_slot, hub_nseg, hub_part, err_part, fin_part, state,
+      errors;
   int errors_cap = 0;
   std::vector<hipEvent_t> ev;
   // exact hubs run on a second stream, forked from / joined to `stream`
@@ -95,34 +94,10 @@ struct IterEngine {
     n_tiles = (int)row_tiles.size();
     n_hub_tiles = (int)hubs.size();
     if (n_hub_tiles) {
-      // staging: hub row i's values at [staged index of its first chunk, + deg);
-      // chunks of 256*ipt edges from each row start (the tile sort's chunks),
-      // the longest rows' first
-      const int C = kBlock * ipt;
-      std::vector<Tile> chunks;
-      std::vector<int32_t> chunk0(n_hub_tiles);
-      int64_t total = 0;
-      for (int i = 0; i < n_hub_tiles; i++) {
-        const Tile& t = hubs[i];
-        chunk0[i] = (int32_t)chunks.size();
-        for (int32_t e = t.z; e < t.w; e += C)
-          chunks.push_back(Tile{(int32_t)(total + (e - t.z)), 0, e, std::min(e + C, t.w)});
-        total += t.w - t.z;
-      }
-      n_hub_chunks = (int)chunks.size();
       rc = hub_tiles.alloc(sizeof(Tile) * n_hub_tiles);
-      if (!rc) rc = hub_chunks.alloc(sizeof(Tile) * n_hub_chunks);
-      if (!rc) rc = hub_chunk0.alloc(4 * (size_t)n_hub_tiles);
-      if (!rc) rc = staged.alloc(4 * (size_t)total);
-      if (!rc) rc = chunk_epoch.alloc(4 * (size_t)n_hub_chunks);
       if (rc) return rc;
       GD_HIP_TRY(hipMemcpyAsync(hub_tiles.ptr, hubs.data(), sizeof(Tile) * n_hub_tiles,
                                 hipMemcpyHostToDevice, stream));
-      GD_HIP_TRY(hipMemcpyAsync(hub_chunks.ptr, chunks.data(), sizeof(Tile) * n_hub_chunks,
-                                hipMemcpyHostToDevice, stream));
-      GD_HIP_TRY(hipMemcpyAsync(hub_chunk0.ptr, chunk0.data(), 4 * (size_t)n_hub_tiles,
-                                hipMemcpyHostToDevice, stream));
-      GD_HIP_TRY(hipMemsetAsync(chunk_epoch.ptr, 0, 4 * (size_t)n_hub_chunks, stream));
       GD_HIP_TRY(hipStreamSynchronize(stream));  // host vectors go out of scope
       GD_HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
       GD_HIP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
@@ -137,7 +112,6 @@ struct IterEngine {
     if (!rc) rc = fin_part.alloc(8 * (size_t)kFinBlocks);
     if (!rc) rc = state.alloc(sizeof(IterState));
     if (rc) return rc;
-    GD_HIP_TRY(hipMemsetAsync(state.ptr, 0, sizeof(IterState), stream));  // epoch starts at 0
     auto up = [&](DeviceBuffer& b, const void* src, size_t n) -> hipError_t {
       return n ? hipMemcpyAsync(b.ptr, src, n, hipMemcpyHostToDevice, stream) : hipSuccess;
     };
@@ -170,9 +144,7 @@ struct IterEngine {
 
   IterArgs iter_args() {
     return IterArgs{tiles.as<Tile>(), n_tiles, hub_tiles.as<Tile>(), n_hub_tiles,
-                    n_tiles > 0 ? pull_grid : 0, hub_chunks.as<Tile>(), n_hub_chunks,
-                    hub_chunk0.as<int32_t>(), staged.as<float>(), chunk_epoch.as<unsigned>(), off,
-                    col, e_max, pos.as<uint16_t>(),
+                    n_tiles > 0 ? pull_grid : 0, off, col, e_max, pos.as<uint16_t>(),
                     hub_part.as<float>(), err_part.as<double>(), state.as<IterState>()};
   }
 
@@ -190,8 +162,8 @@ struct IterEngine {
       (void)hipEventRecord(fork_ev, stream);
       (void)hipStreamWaitEvent(side, fork_ev, 0);
       if constexpr (P::kExactHubs)
-        hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED>),
-                           dim3(n_hub_chunks + n_hub_tiles), dim3(kBlock), 0, side, a, it);
+        hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED>), dim3(n_hub_tiles),
+                           dim3(kBlock), 0, side, a, it);
       (void)hipEventRecord(join_ev, side);
     }
     if (n_tiles > 0) {

@@ -37,8 +37,6 @@ struct IterState {
   int max_iter;
   unsigned ticket;  // finalize workgroups arrived in this iteration
   double tol;
-  unsigned epoch;   // iterations ever run by this plan (never reset): tags staged hub chunks
-  unsigned pad;
 };
 
 struct IterArgs {
@@ -47,11 +45,6 @@ struct IterArgs {
   const Tile* __restrict__ hub_tiles;  // whole-row exact hubs (exact_hub_kernel)
   int n_hub_tiles;
   int hub_part_base;  // exact hub h writes its L1 partial to err_part[hub_part_base + h]
-  const Tile* __restrict__ hub_chunks;  // x = staged index of the chunk, [z, w) its edges
-  int n_hub_chunks;
-  const int32_t* __restrict__ hub_chunk0;  // first chunk (index into hub_chunks) of hub tile i
-  float* __restrict__ staged;              // gathered hub values, each hub row contiguous
-  unsigned* __restrict__ chunk_epoch;      // epoch + 1 once a chunk is staged
   const int32_t* __restrict__ off;
   const int32_t* __restrict__ col;
   int e_max;  // last valid index of col (a 1-element dummy when nnz == 0)
@@ -158,98 +151,29 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
   if (threadIdx.x == 0) it.err_part[blockIdx.x] = tot;
 }
 
-// Whole-row hubs of an exact-hub policy.  One launch with two roles, on a
-// second stream concurrently with the row-tile launch (its register demand
-// would otherwise lower the occupancy of every row tile):
-//   workgroups [0, n_hub_chunks): gather one 256*IPT-edge chunk of a hub row
-//     (in source order when SORTED), write each value to its edge's place in
-//     `staged`, then publish the chunk by writing the iteration's epoch tag;
-//   workgroups [n_hub_chunks, +n_hub_tiles): one per hub row (longest first),
-//     add the row's staged chunks in edge order with the reference's
-//     sequential rounding (exact_scan_chunk) and run the row's epilogue.
-// Workgroups are dispatched in index order, so every chunk a row waits for
-// belongs to a workgroup dispatched before it that never waits; the wait is
-// nevertheless bounded, and a row whose chunk is not published in time
-// gathers that chunk itself (same values, so the same result).
-constexpr int kChunkWaitSpins = 20000;  // ~2 ms of s_sleep polling per chunk
-
+// Whole-row hubs of an exact-hub policy: one workgroup per hub row gathers the
+// row chunk by chunk and adds it in edge order with the reference's sequential
+// rounding (exact_seq_sum), then runs the row's epilogue.  A kernel of its own,
+// launched on a second stream concurrently with the row-tile launch: its
+// register demand would otherwise lower the occupancy of every row tile.
 template <class P, int IPT, int NT, bool SORTED>
 __global__ __launch_bounds__(kBlock) void exact_hub_kernel(const typename P::Args a,
                                                            const IterArgs it) {
   if (it.st->done) return;
   const int odd = it.st->iter & 1;
-  const unsigned tag = it.st->epoch + 1u;
-  const auto g = P::template gather<NT>(a, odd);
-  const int tid = threadIdx.x;
-  if ((int)blockIdx.x < it.n_hub_chunks) {  // gather role
-    const Tile c = it.hub_chunks[blockIdx.x];
-    const int ne = c.w - c.z;
-    int idx[IPT], p[IPT];
-    float v[IPT];
-    load_indices<IPT, (NT > 0)>(it.col, c.z, ne, idx);
-    if constexpr (SORTED) load_positions<IPT, (NT > 0)>(it.pos, c.z, ne, p);
-    gather_values<IPT, SORTED>(g, c.z, ne, idx, p, v);
-#pragma unroll
-    for (int j = 0; j < IPT; j++) {
-      const int i = tid + j * kBlock;
-      if (i < ne) it.staged[c.x + (SORTED ? p[j] : i)] = v[j];
-    }
-    __threadfence();  // release: this thread's values before the tag
-    __syncthreads();
-    if (tid == 0) atomicExch(it.chunk_epoch + blockIdx.x, tag);
-    return;
-  }
-  // scan role
   __shared__ RowTileLds<IPT> lds;
   __shared__ double s_err[kWaves];
   __shared__ ExactLds xs;
-  __shared__ int s_ready;
-  constexpr int C = kBlock * IPT;
-  const int h = blockIdx.x - it.n_hub_chunks;
-  const Tile t = it.hub_tiles[h];
-  const int k0 = it.hub_chunk0[h];
+  const Tile t = it.hub_tiles[blockIdx.x];
+  const auto g = P::template gather<NT>(a, odd);
   const auto e = P::template epilogue<NT>(a, odd);
   const auto pre = e.pre(t.x);
-  float s = e.init(pre);
-  const int ne = t.w - t.z;
-  for (int c0 = 0, k = k0; c0 < ne; c0 += C, k++) {
-    const int nc = min(C, ne - c0);
-    if (tid == 0) {
-      int spins = 0;
-      while (__hip_atomic_load(it.chunk_epoch + k, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) !=
-                 tag &&
-             spins < kChunkWaitSpins) {
-        __builtin_amdgcn_s_sleep(8);
-        spins++;
-      }
-      s_ready = spins < kChunkWaitSpins;
-    }
-    __syncthreads();
-    float v[IPT];
-    if (s_ready) {
-      __threadfence();  // acquire: the chunk's values, not stale cache lines
-      const float* src = it.staged + it.hub_chunks[k].x;
-#pragma unroll
-      for (int j = 0; j < IPT; j++) v[j] = src[min(tid + j * kBlock, nc - 1)];
-#pragma unroll
-      for (int j = 0; j < IPT; j++)
-        if (tid + j * kBlock < nc) lds.val[exact_pos<IPT>(tid + j * kBlock)] = v[j];
-    } else {  // not published in time: gather the chunk here
-      int idx[IPT], p[IPT];
-      load_indices<IPT, (NT > 0)>(it.col, t.z + c0, nc, idx);
-      if constexpr (SORTED) load_positions<IPT, (NT > 0)>(it.pos, t.z + c0, nc, p);
-      gather_values<IPT, SORTED>(g, t.z + c0, nc, idx, p, v);
-#pragma unroll
-      for (int j = 0; j < IPT; j++)
-        if (tid + j * kBlock < nc) lds.val[exact_pos<IPT>(SORTED ? p[j] : tid + j * kBlock)] = v[j];
-    }
-    __syncthreads();
-    s = exact_scan_chunk<IPT>(lds.val, nc, c0 == 0, s, xs);
-  }
+  const float sum = exact_seq_sum<IPT, (NT > 0), SORTED>(it.col, it.pos, it.e_max, t.z, t.w - t.z,
+                                                         g, e.init(pre), lds.val, xs);
   double err = 0.0;
-  if (tid == 0) err = e.finish(t.x, s, pre);
+  if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
   const double tot = block_sum<kWaves>(err, s_err);
-  if (tid == 0) it.err_part[it.hub_part_base + h] = tot;
+  if (threadIdx.x == 0) it.err_part[it.hub_part_base + blockIdx.x] = tot;
 }
 
 // Hub-row epilogues (segment partials summed in segment order) and the L1
@@ -299,7 +223,6 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(const typename P::Args
     const int i = st->iter;
     f.errors[i] = total;
     st->iter = i + 1;
-    st->epoch = st->epoch + 1;
     st->ticket = 0;
     if (total < st->tol || i + 1 >= st->max_iter) st->done = 1;
   }
