@@ -206,6 +206,89 @@ def bench_ppr(scale, k, tol, seed, fp64_seeds):
             "fp64_seeds_checked": fp64_seeds}
 
 
+def bench_csr_build(scale, seed, host_s, E):
+    """fill_data() on the device: the RMAT in-CSR (transposed, sorted,
+    self-loops and duplicates removed -- the graph the bench runs on) from its
+    raw edge list already in HBM, against the host builder's time."""
+    import numpy as np
+    import torch
+
+    from gardenia_ocl_amd import graph
+
+    src, dst = graph.rmat_edges(scale, 16, seed=seed)
+    dev = torch.device("cuda:0")
+    ts, td = torch.from_numpy(src).to(dev), torch.from_numpy(dst).to(dev)
+    del src, dst
+    graph.csr_from_edges_device(1 << scale, ts, td, transpose=True, sorted=True)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d = graph.csr_from_edges_device(1 << scale, ts, td, transpose=True, sorted=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"workload": f"RMAT-{scale} in-CSR from {int(ts.shape[0])} raw edges (fill_data flags)",
+            "device_ms": round(dt * 1e3, 2), "host_rmat_graph_s": round(host_s, 2),
+            "edges_kept": int(d["column_indices"].shape[0]),
+            "same_edge_count_as_host_graph": int(d["column_indices"].shape[0]) == E,
+            "speedup_vs_host": round(host_s / dt, 1)}
+
+
+def bench_symgs(n, rmat_scale, seed, reps=20):
+    """Multicolor SymGS sweeps (SymGSPlan): a 27-point stencil on an n^3 grid
+    (HPCG-shaped, 8 colors) and a symmetrized RMAT with self-loops kept and a
+    diagonal on every third row (first-fit colors).  Algorithmic bytes per
+    symmetric sweep: two passes over every swept row, each reading the
+    off-diagonal column, value and x[column] (12 B per entry) and the row's
+    offset, diagonal, original index and b and writing x (20 B per row)."""
+    import numpy as np
+    import torch
+
+    from gardenia_ocl_amd import SymGSPlan, graph
+
+    dev = torch.device("cuda:0")
+    out = {}
+
+    def run(name, c):
+        colors, nc = graph.color_first_fit(c)
+        idx, co = graph.color_order(colors, nc)
+        t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dt)).to(dev)  # noqa: E731
+        plan = SymGSPlan(t(c.row_offsets, np.int32), t(c.column_indices, np.int32),
+                         t(c.weight, np.float32), t(idx, np.int32), co)
+        info = plan.info()
+        x = torch.zeros(c.m, dtype=torch.float32, device=dev)
+        b = t(graph.uniform_pm1(c.m, seed + 17), np.float32)
+        plan.run(x, b, 2)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        plan.run(x, b, reps)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nbytes = 2 * (12 * info["offdiag_nnz"] + 20 * info["rows"])
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = {"rows": c.m, "nnz": c.nnz, "offdiag_nnz": info["offdiag_nnz"],
+                     "colors": nc, "exact_rows": info["n_exact_rows"],
+                     "launches_per_sweep": 2 * nc, "ms_per_sweep": round(ms, 4),
+                     "algorithmic_bytes_per_sweep": nbytes, "achieved_gbs": round(gbs, 1),
+                     "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                     "plan_ms": round(info["plan_ms"], 1)}
+        plan.close()
+
+    run(f"stencil27_{n}^3", graph.stencil27(n))
+    src, dst = graph.rmat_edges(rmat_scale, 16, seed=seed)
+    m = 1 << rmat_scale
+    d = np.arange(0, m, 3, dtype=np.int32)
+    s_ = np.concatenate([src, d])
+    t_ = np.concatenate([dst, d])
+    w = np.abs(graph.uniform_pm1(len(s_), seed + 3)) + np.float32(0.0625)
+    w[len(src):] = 40.0
+    del src, dst
+    c = graph.csr_from_edges(m, s_, t_, w, symmetrize=True, transpose=False, sorted=True,
+                             remove_selfloops=False, remove_redundents=True)
+    run(f"rmat{rmat_scale}_sym", c)
+    return out
+
+
 def bench_multigraph(n, lo, hi, tol, seed):
     """n RMAT graphs of scales uniform in [lo, hi]: one multi-graph plan (one
     pull + one finalize launch per iteration for all graphs, replayed from a
@@ -390,7 +473,8 @@ def main():
     V, E = in_csr.m, in_csr.nnz
     # out-degree = number of times a vertex appears as a source in the in-CSR
     out_deg = np.bincount(in_csr.column_indices, minlength=V).astype(np.int32)
-    log(f"RMAT-{args.scale}: V={V} E={E} built in {time.perf_counter() - t0:.1f} s")
+    host_build_s = time.perf_counter() - t0
+    log(f"RMAT-{args.scale}: V={V} E={E} built in {host_build_s:.1f} s")
     off = torch.from_numpy(in_csr.row_offsets).to(dev)
     col = torch.from_numpy(in_csr.column_indices).to(dev)
     deg = torch.from_numpy(out_deg).to(dev)
@@ -448,6 +532,12 @@ def main():
         torch.cuda.empty_cache()
         log("batched ppr ...")
         extras["ppr"] = bench_ppr(args.ppr_scale, args.ppr_k, args.ppr_tol, args.seed, 2)
+        torch.cuda.empty_cache()
+        log("device CSR build ...")
+        extras["csr_build"] = bench_csr_build(args.scale, args.seed, host_build_s, E)
+        torch.cuda.empty_cache()
+        log("symgs ...")
+        extras["symgs"] = bench_symgs(128, 20, args.seed)
         torch.cuda.empty_cache()
         log("multi-graph ...")
         extras["multigraph"] = bench_multigraph(64, 12, 18, 1e-4, args.seed)

@@ -1,6 +1,7 @@
 """gardenia_ocl_amd -- MI355X-native pull PageRank and CSR SpMV.
 
-A drop-in for the PageRank (src/pr) and SpMV (src/spmv) solver plug-ins of
+A drop-in for the PageRank (src/pr), SpMV (src/spmv) and SymGS (src/symgs)
+solver plug-ins of
 gardenia_ocl: hand-written HIP kernels for gfx950 behind a C ABI
 (include/gardenia_hip.h, include/gardenia_graph.h), with Python mirrors of the
 reference's ``PRSolver`` / ``SpmvSolver`` and device-resident plans over torch
@@ -8,12 +9,14 @@ tensors.
 """
 from . import graph
 from ._native import GardeniaError, LIB_PATH
-from .pr import PRSolver, PageRankPlan, EPSILON, K_DAMP, MAX_ITER
+from .pr import EPSILON, K_DAMP, MAX_ITER, PageRankPlan, PRSolver, PRVerifier, verify_residual
 from .spmv import SpmvSolver, SpmvPlan
+from .symgs import SymGSPlan, SymGSSolver
 from .ppr import PersonalizedPageRankPlan
 from .multigraph import MultiGraphPageRank
 
 __all__ = [
-    "graph", "GardeniaError", "PersonalizedPageRankPlan", "MultiGraphPageRank", "LIB_PATH", "PRSolver", "PageRankPlan", "SpmvSolver", "SpmvPlan",
+    "graph", "GardeniaError", "PersonalizedPageRankPlan", "MultiGraphPageRank", "LIB_PATH", "PRSolver", "PRVerifier", "verify_residual", "PageRankPlan", "SpmvSolver", "SpmvPlan",
+    "SymGSPlan", "SymGSSolver",
     "EPSILON", "K_DAMP", "MAX_ITER",
 ]

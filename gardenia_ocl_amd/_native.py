@@ -100,6 +100,19 @@ class MultiPrConfig(ctypes.Structure):
     _fields_ = [("use_graph", ctypes.c_int), ("chunk", ctypes.c_int)]
 
 
+class SymgsPlanInfo(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_int32), ("n_colors", ctypes.c_int32),
+                ("offdiag_nnz", ctypes.c_int64), ("n_tiles", ctypes.c_int32),
+                ("n_exact_rows", ctypes.c_int32), ("max_row_len", ctypes.c_int32),
+                ("plan_ms", ctypes.c_double)]
+
+
+class DeviceCsr(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int32), ("nnz", ctypes.c_int64), ("row_offsets", ctypes.c_void_p),
+                ("column_indices", ctypes.c_void_p), ("weight", ctypes.c_void_p),
+                ("degree", ctypes.c_void_p)]
+
+
 class PprPlanConfig(ctypes.Structure):
     _fields_ = [("relabel", ctypes.c_int), ("k_chunk", ctypes.c_int),
                 ("column_stop", ctypes.c_int), ("compact", ctypes.c_int)]
@@ -136,6 +149,11 @@ SIGNATURES = {
     "gd_csr_free": (None, [ctypes.POINTER(HostCsr)]),
     "gd_csr_from_edges": (_i, [_i, _i64, _P, _P, _P, _i, _i, _i, _i, _i, _i, ctypes.POINTER(HostCsr)]),
     "gd_read_mtx": (_i, [ctypes.c_char_p, _i, _i, _i, _i, _i, ctypes.POINTER(HostCsr)]),
+    "gd_read_gr": (_i, [ctypes.c_char_p, _i, _i, _i, _i, _i, ctypes.POINTER(HostCsr)]),
+    "gd_read_dotgraph": (_i, [ctypes.c_char_p, _i, _i, _i, _i, _i, ctypes.POINTER(HostCsr)]),
+    "gd_read_graph": (_i, [ctypes.c_char_p, _i, _i, _i, _i, _i, ctypes.POINTER(HostCsr)]),
+    "gd_color_first_fit": (_i, [_i, _P, _P, _P, ctypes.POINTER(ctypes.c_int32)]),
+    "gd_color_order": (_i, [_i, _P, ctypes.c_int32, _P, _P]),
     "gd_rmat_graph": (_i, [_i, _i, _d, _d, _d, _u64, ctypes.POINTER(HostCsr), ctypes.POINTER(HostCsr)]),
     "gd_rmat_edges": (_i, [_i, _i, _d, _d, _d, _u64, _i64, _i64, _P, _P]),
     "gd_er_matrix": (_i, [_i, _i, _u64, ctypes.POINTER(HostCsr)]),
@@ -159,6 +177,16 @@ SIGNATURES = {
                                 ctypes.POINTER(_P)]),
     "gd_ppr_plan_run": (_i, [_P, _P, _P, ctypes.POINTER(PrOptions), ctypes.POINTER(PrStats), _P,
                              _P]),
+    "gd_csr_from_edges_device": (_i, [_i, _i64, _P, _P, _P, _i, _i, _i, _i, _i, _i, _P,
+                                      ctypes.POINTER(DeviceCsr)]),
+    "gd_device_csr_copy": (_i, [ctypes.POINTER(DeviceCsr), _P, _P, _P, _P, _P]),
+    "gd_device_csr_free": (_i, [ctypes.POINTER(DeviceCsr)]),
+    "gd_pr_verify": (_i, [_i, _i64, _P, _P, _P, _P, ctypes.c_float, _P, _F64P]),
+    "gd_symgs_plan_create": (_i, [_i, _i64, _P, _P, _P, _P, _P, _i, _P, ctypes.POINTER(_P)]),
+    "gd_symgs_plan_run": (_i, [_P, _P, _P, _i, _P]),
+    "gd_symgs_plan_info": (_i, [_P, ctypes.POINTER(SymgsPlanInfo)]),
+    "gd_symgs_plan_destroy": (_i, [_P]),
+    "gd_symgs_solve": (_i, [_i, _i, _P, _P, _P, _P, _P, _P, _P, _i, ctypes.POINTER(ctypes.c_float)]),
     "gd_ppr_plan_info": (_i, [_P, ctypes.POINTER(PprPlanInfo)]),
     "gd_ppr_plan_column_iterations": (_i, [_P, _P]),
     "gd_ppr_plan_destroy": (_i, [_P]),

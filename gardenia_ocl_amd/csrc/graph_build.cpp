@@ -2,7 +2,8 @@
 //
 // Replaces the reference's CSR builder (include/graph_io.h) for the PR/SpMV
 // path:
-//   * gd_csr_from_edges / gd_read_mtx: fill_data()/mtx2csr() semantics
+//   * gd_csr_from_edges / gd_read_mtx / gd_read_gr / gd_read_dotgraph /
+//     gd_read_graph: fill_data()/mtx2csr()/gr2csr()/graph2csr() semantics
 //     (graph_io.h:21-130, :216-261, :280-295), including the unsorted-input
 //     quirk where only ADJACENT duplicates are removed (graph_io.h:48-61).
 //     Built with a stable counting sort by row instead of vector<vector<Edge>>.
@@ -16,6 +17,8 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cerrno>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -27,7 +30,16 @@
 
 #include "gardenia_graph.h"
 
+namespace gd {
+void set_error(const std::string& msg);  // gd_last_error() text (schedule.hip)
+}
+
 namespace {
+
+int fail(int code, const std::string& msg) {
+  gd::set_error(msg);
+  return code;
+}
 
 inline uint64_t mix64(uint64_t x) {  // SplitMix64 finaliser (Steele et al. 2014)
   x += 0x9E3779B97F4A7C15ull;
@@ -298,7 +310,8 @@ int gd_csr_from_edges(int m, int64_t n, const int32_t* src, const int32_t* dst, 
                       int remove_redundents, int mirror_selfloops, gd_host_csr* out) {
   if (m < 0 || n < 0 || !out) return GD_ERR_INVALID;
   for (int64_t i = 0; i < n; i++)
-    if (src[i] < 0 || src[i] >= m || dst[i] < 0 || dst[i] >= m) return GD_ERR_INVALID;
+    if (src[i] < 0 || src[i] >= m || dst[i] < 0 || dst[i] >= m)
+      return fail(GD_ERR_INVALID, "edge " + std::to_string(i) + " has a vertex id outside [0, m)");
   // Placement (graph_io.h:245-257): the row that receives each pushed edge, in
   // push order.  A mirrored edge is pushed before the forward one.
   int64_t cap = symmetrize ? 2 * n : n;
@@ -374,7 +387,7 @@ int gd_read_mtx(const char* path, int symmetrize, int transpose, int sorted, int
                 int remove_redundents, gd_host_csr* out) {
   if (!path || !out) return GD_ERR_INVALID;
   FILE* f = fopen(path, "r");
-  if (!f) return GD_ERR_IO;
+  if (!f) return fail(GD_ERR_IO, std::string("cannot open ") + path);
   std::string line;
   char* buf = nullptr;
   size_t cap = 0;
@@ -390,7 +403,7 @@ int gd_read_mtx(const char* path, int symmetrize, int transpose, int sorted, int
   if (!have_header || m < 0 || nnz < 0) {
     free(buf);
     fclose(f);
-    return GD_ERR_IO;
+    return fail(GD_ERR_IO, "malformed .mtx header");
   }
   std::vector<int32_t> s(nnz), d(nnz);
   std::vector<float> w(nnz);
@@ -399,14 +412,14 @@ int gd_read_mtx(const char* path, int symmetrize, int transpose, int sorted, int
     if (getline(&buf, &cap, f) < 0) {
       free(buf);
       fclose(f);
-      return GD_ERR_IO;
+      return fail(GD_ERR_IO, "truncated .mtx edge list");
     }
     int a = 0, b = 0;
     int num = sscanf(buf, "%d %d %f", &a, &b, &wt);
     if (num < 2) {
       free(buf);
       fclose(f);
-      return GD_ERR_IO;
+      return fail(GD_ERR_IO, "malformed .mtx edge line");
     }
     if (num == 2) wt = 1;
     if (wt < 0) wt = -wt;
@@ -418,6 +431,182 @@ int gd_read_mtx(const char* path, int symmetrize, int transpose, int sorted, int
   fclose(f);
   return gd_csr_from_edges(m, nnz, s.data(), d.data(), w.data(), symmetrize, transpose, sorted,
                            remove_selfloops, remove_redundents, 0, out);
+}
+
+// gr2csr() (graph_io.h:133-174): leading lines starting with 'c' skipped, a
+// "p sp <m> <nnz>" header, then nnz arc lines "a <src> <dst> [w]" with 1-based
+// ids; the weight column is ignored (every weight 1), symmetrize mirrors
+// self-loops too (:160-164).  A malformed arc line or an id outside [1, m] is
+// an error here (the reference reuses the previous line's ids or indexes out of
+// bounds).
+int gd_read_gr(const char* path, int symmetrize, int transpose, int sorted, int remove_selfloops,
+               int remove_redundents, gd_host_csr* out) {
+  if (!path || !out) return GD_ERR_INVALID;
+  FILE* f = fopen(path, "r");
+  if (!f) return fail(GD_ERR_IO, std::string("cannot open ") + path);
+  char* buf = nullptr;
+  size_t cap = 0;
+  int m = -1;
+  long long nnz = -1;
+  bool header = false;
+  while (getline(&buf, &cap, f) >= 0) {
+    if (buf[0] == 'c' || buf[0] == '\n' || buf[0] == '\0') continue;
+    char c = 0, sp[8] = {0};
+    header = sscanf(buf, "%c %7s %d %lld", &c, sp, &m, &nnz) == 4;
+    break;
+  }
+  if (!header || m < 0 || nnz < 0) {
+    free(buf);
+    fclose(f);
+    return fail(GD_ERR_IO, "malformed .gr header (expected p sp <m> <nnz>)");
+  }
+  std::vector<int32_t> s(nnz), d(nnz);
+  for (long long i = 0; i < nnz; i++) {
+    char c = 0;
+    int a = 0, b = 0;
+    if (getline(&buf, &cap, f) < 0 || sscanf(buf, "%c %d %d", &c, &a, &b) != 3 || c != 'a') {
+      free(buf);
+      fclose(f);
+      return fail(GD_ERR_IO, "truncated or malformed .gr arc line (expected a <src> <dst>)");
+    }
+    s[i] = a - 1;
+    d[i] = b - 1;
+  }
+  free(buf);
+  fclose(f);
+  return gd_csr_from_edges(m, nnz, s.data(), d.data(), nullptr, symmetrize, transpose, sorted,
+                           remove_selfloops, remove_redundents, 1, out);
+}
+
+// graph2csr() (graph_io.h:177-213), the METIS-style adjacency format: a
+// "<m> <nnz> ..." header, then line v (v = 1..m) lists v's neighbours, 1-based.
+// A line is read as integers up to its first non-integer token (istream >>
+// int); lines missing at the end of the file are empty rows.  Every weight is
+// 1 and symmetrize mirrors self-loops too.
+int gd_read_dotgraph(const char* path, int symmetrize, int transpose, int sorted,
+                     int remove_selfloops, int remove_redundents, gd_host_csr* out) {
+  if (!path || !out) return GD_ERR_INVALID;
+  FILE* f = fopen(path, "r");
+  if (!f) return fail(GD_ERR_IO, std::string("cannot open ") + path);
+  char* buf = nullptr;
+  size_t cap = 0;
+  int m = -1;
+  long long nnz = 0;
+  if (getline(&buf, &cap, f) < 0 || sscanf(buf, "%d %lld", &m, &nnz) != 2 || m < 0) {
+    free(buf);
+    fclose(f);
+    return fail(GD_ERR_IO, "malformed .graph header (expected <m> <nnz>)");
+  }
+  std::vector<int32_t> s, d;
+  s.reserve(nnz > 0 ? nnz : 0);
+  d.reserve(nnz > 0 ? nnz : 0);
+  for (int v = 0; v < m; v++) {
+    if (getline(&buf, &cap, f) < 0) break;
+    const char* p = buf;
+    for (;;) {
+      char* end = nullptr;
+      errno = 0;
+      const long x = strtol(p, &end, 10);
+      if (end == p) break;
+      if (errno == ERANGE || x < INT32_MIN || x > INT32_MAX) {
+        free(buf);
+        fclose(f);
+        return fail(GD_ERR_IO, "integer out of range in .graph adjacency line");
+      }
+      s.push_back(v);
+      d.push_back((int32_t)(x - 1));
+      p = end;
+    }
+  }
+  free(buf);
+  fclose(f);
+  return gd_csr_from_edges(m, (int64_t)s.size(), s.data(), d.data(), nullptr, symmetrize,
+                           transpose, sorted, remove_selfloops, remove_redundents, 1, out);
+}
+
+// Sequential first-fit coloring in vertex order over the union of the CSR and
+// its transpose (an off-diagonal entry in either direction separates two
+// rows), self-loops ignored.  Deterministic; a proper coloring for the SymGS
+// sweep whatever the symmetry of the input.  The reference colors with its VC
+// benchmark (src/vc/omp_base.cc:9-47), whose parallel first-fit is racy.
+int gd_color_first_fit(int m, const int32_t* row_offsets, const int32_t* column_indices,
+                       int32_t* colors, int32_t* n_colors) {
+  if (m < 0 || (m > 0 && (!row_offsets || !colors)) || !n_colors) return GD_ERR_INVALID;
+  *n_colors = 0;
+  if (m == 0) return GD_OK;
+  const int64_t nnz = row_offsets[m];
+  if (row_offsets[0] != 0 || nnz < 0 || (nnz > 0 && !column_indices))
+    return fail(GD_ERR_INVALID, "bad row offsets");
+  for (int i = 0; i < m; i++)
+    if (row_offsets[i + 1] < row_offsets[i]) return fail(GD_ERR_INVALID, "bad row offsets");
+  for (int64_t k = 0; k < nnz; k++)
+    if (column_indices[k] < 0 || column_indices[k] >= m)
+      return fail(GD_ERR_INVALID, "column index out of range [0, m)");
+  // transpose: tstart/tcol list the rows that reference each column
+  std::vector<int64_t> tstart(m + 1, 0);
+  for (int64_t k = 0; k < nnz; k++) tstart[column_indices[k] + 1]++;
+  for (int i = 0; i < m; i++) tstart[i + 1] += tstart[i];
+  std::vector<int32_t> tcol(nnz);
+  {
+    std::vector<int64_t> at(tstart.begin(), tstart.end() - 1);
+    for (int r = 0; r < m; r++)
+      for (int64_t k = row_offsets[r]; k < row_offsets[r + 1]; k++)
+        tcol[at[column_indices[k]]++] = r;
+  }
+  std::vector<int32_t> mark;  // mark[c] == v: color c is taken by a neighbour of v
+  int32_t nc = 0;
+  for (int v = 0; v < m; v++) colors[v] = -1;
+  for (int v = 0; v < m; v++) {
+    auto forbid = [&](int u) {
+      if (u != v && colors[u] >= 0) mark[colors[u]] = v;
+    };
+    for (int64_t k = row_offsets[v]; k < row_offsets[v + 1]; k++) forbid(column_indices[k]);
+    for (int64_t k = tstart[v]; k < tstart[v + 1]; k++) forbid(tcol[k]);
+    int32_t c = 0;
+    while (c < nc && mark[c] == v) c++;
+    if (c == nc) {
+      nc++;
+      mark.push_back(-1);
+    }
+    colors[v] = c;
+  }
+  *n_colors = nc;
+  return GD_OK;
+}
+
+// The driver's ordering (src/symgs/main.cc:38-49): vertices sorted by color
+// (stable: increasing vertex id inside a color) and the color offsets.
+int gd_color_order(int m, const int32_t* colors, int32_t n_colors, int32_t* indices,
+                   int32_t* color_offsets) {
+  if (m < 0 || n_colors < 0 || (m > 0 && (!colors || !indices)) || !color_offsets)
+    return GD_ERR_INVALID;
+  std::vector<int32_t> cnt(n_colors + 1, 0);
+  for (int v = 0; v < m; v++) {
+    if (colors[v] < 0 || colors[v] >= n_colors)
+      return fail(GD_ERR_INVALID, "color outside [0, n_colors)");
+    cnt[colors[v] + 1]++;
+  }
+  for (int c = 0; c < n_colors; c++) cnt[c + 1] += cnt[c];
+  for (int c = 0; c <= n_colors; c++) color_offsets[c] = cnt[c];
+  for (int v = 0; v < m; v++) indices[cnt[colors[v]]++] = v;
+  return GD_OK;
+}
+
+// read_graph() (graph_io.h:280-295): the format chosen by the first of
+// ".mtx", ".graph", ".gr" found in the path (strstr, in that order).
+int gd_read_graph(const char* path, int symmetrize, int transpose, int sorted, int remove_selfloops,
+                  int remove_redundents, gd_host_csr* out) {
+  if (!path || !out) return GD_ERR_INVALID;
+  if (strstr(path, ".mtx"))
+    return gd_read_mtx(path, symmetrize, transpose, sorted, remove_selfloops, remove_redundents,
+                       out);
+  if (strstr(path, ".graph"))
+    return gd_read_dotgraph(path, symmetrize, transpose, sorted, remove_selfloops,
+                            remove_redundents, out);
+  if (strstr(path, ".gr"))
+    return gd_read_gr(path, symmetrize, transpose, sorted, remove_selfloops, remove_redundents,
+                      out);
+  return fail(GD_ERR_IO, "Unrecognizable input file format (not .mtx, .graph or .gr)");
 }
 
 }  // extern "C"

@@ -1,9 +1,11 @@
-// solver_shim.cpp -- PRSolver / SpmvSolver with the reference's signatures.
+// solver_shim.cpp -- PRSolver / SpmvSolver / SymGSSolver with the reference's
+// signatures.
 //
 // The link-time plug-in boundary of the reference (one solver definition per
 // binary, src/pr/Makefile:17-47, src/spmv/Makefile:16-50): these definitions
 // forward to the C ABI and print the reference's stdout lines
-// (src/pr/omp_base.cc:15,42,46-47; src/spmv/omp_base.cc:14,30).
+// (src/pr/omp_base.cc:15,42,46-47; src/spmv/omp_base.cc:14,30;
+// src/symgs/omp_base.cc:33,40).
 
 #include <cstdio>
 #include <cstdlib>
@@ -44,5 +46,16 @@ void SpmvSolver(int m, int nnz, int* Ap, int* Aj, float* Ax, float* x, float* y)
   float ms = 0.0f;
   int rc = gd_spmv_solve(m, nnz, Ap, Aj, Ax, x, y, &ms);
   if (rc != GD_OK) fail("SpmvSolver", rc);
+  printf("\truntime [%s] = %f ms.\n", "hip_tile", ms);
+}
+
+void SymGSSolver(int num_rows, int nnz, int* Ap, int* Aj, int* indices, float* Ax, float* x,
+                 float* b, std::vector<int> color_offsets) {
+  printf("Launching HIP SymGS solver (gfx950 LDS tiles, exact row order) ...\n");
+  if (color_offsets.empty()) fail("SymGSSolver", GD_ERR_INVALID);
+  float ms = 0.0f;
+  int rc = gd_symgs_solve(num_rows, nnz, Ap, Aj, indices, Ax, x, b, color_offsets.data(),
+                          (int)color_offsets.size() - 1, &ms);
+  if (rc != GD_OK) fail("SymGSSolver", rc);
   printf("\truntime [%s] = %f ms.\n", "hip_tile", ms);
 }

@@ -102,6 +102,89 @@ def csr_from_edges(m: int, src: np.ndarray, dst: np.ndarray, wt: Optional[np.nda
     return _take(h)
 
 
+def csr_from_edges_device(m: int, src, dst, wt=None, *, symmetrize=False, transpose=False,
+                          sorted=True, remove_selfloops=True, remove_redundents=True,
+                          mirror_selfloops=False, stream=None) -> dict:
+    """fill_data() on the GPU for an edge list already in HBM (int32 CUDA
+    tensors src/dst, optional float32 weights): the flags and result of
+    csr_from_edges.  Returns torch CUDA tensors row_offsets, column_indices,
+    weight, degree (the caller's; the library's arrays are released)."""
+    import torch
+
+    for t, dt in ((src, torch.int32), (dst, torch.int32)) + (((wt, torch.float32),) if wt is not None else ()):
+        if not (t.is_cuda and t.dtype == dt and t.is_contiguous()):
+            raise TypeError("edge arrays must be contiguous CUDA tensors (int32 src/dst, float32 wt)")
+    n = int(src.shape[0])
+    if int(dst.shape[0]) != n or (wt is not None and int(wt.shape[0]) != n):
+        raise ValueError("src, dst (and wt) must have the same length")
+    st = stream if stream is not None else torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    c = N.DeviceCsr()
+    N.call("gd_csr_from_edges_device", m, n, src.data_ptr(), dst.data_ptr(),
+           None if wt is None else wt.data_ptr(), int(symmetrize), int(transpose), int(sorted),
+           int(remove_selfloops), int(remove_redundents), int(mirror_selfloops), sp, ctypes.byref(c))
+    try:
+        dev = src.device
+        out = {"row_offsets": torch.empty(m + 1, dtype=torch.int32, device=dev),
+               "column_indices": torch.empty(c.nnz, dtype=torch.int32, device=dev),
+               "weight": torch.empty(c.nnz, dtype=torch.float32, device=dev),
+               "degree": torch.empty(m, dtype=torch.int32, device=dev)}
+        N.call("gd_device_csr_copy", ctypes.byref(c), out["row_offsets"].data_ptr(),
+               out["column_indices"].data_ptr(), out["weight"].data_ptr(),
+               out["degree"].data_ptr(), sp)
+        st.synchronize()
+    finally:
+        N.lib().gd_device_csr_free(ctypes.byref(c))
+    return out
+
+
+def stencil27(n: int) -> Csr:
+    """27-point stencil matrix on an n x n x n grid (rows in x-fastest order,
+    sorted columns): diagonal 26.5, off-diagonal 0.25 + a symmetric function of
+    the two endpoints in [0, 1) -- a symmetric, diagonally dominant SymGS input
+    shaped like HPCG's."""
+    m = n ** 3
+    z, y, x = (a.ravel() for a in np.meshgrid(np.arange(n), np.arange(n), np.arange(n),
+                                               indexing="ij"))
+    keep, delta = [], []
+    for dz in (-1, 0, 1):  # (dz, dy, dx) lexicographic = increasing column
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                keep.append((z + dz >= 0) & (z + dz < n) & (y + dy >= 0) & (y + dy < n)
+                            & (x + dx >= 0) & (x + dx < n))
+                delta.append(dz * n * n + dy * n + dx)
+    keep = np.stack(keep, axis=1)  # (m, 27)
+    r = np.broadcast_to(np.arange(m, dtype=np.int64)[:, None], keep.shape)[keep]
+    c = (np.arange(m, dtype=np.int64)[:, None] + np.asarray(delta, np.int64)[None, :])[keep]
+    lo, hi = np.minimum(r, c), np.maximum(r, c)
+    v = np.where(r == c, 26.5, 0.25 + ((lo * 7919 + hi * 104729) % 1000) / 1000.0)
+    deg = keep.sum(axis=1).astype(np.int32)
+    off = np.zeros(m + 1, np.int32)
+    np.cumsum(deg, out=off[1:])
+    return Csr(m, off, c.astype(np.int32), deg, v.astype(np.float32))
+
+
+def color_first_fit(csr: Csr):
+    """Deterministic first-fit coloring over the CSR and its transpose
+    (gd_color_first_fit).  Returns (colors int32[m], n_colors)."""
+    off = np.ascontiguousarray(csr.row_offsets, np.int32)
+    col = np.ascontiguousarray(csr.column_indices, np.int32)
+    colors = np.empty(csr.m, np.int32)
+    nc = ctypes.c_int32(0)
+    N.call("gd_color_first_fit", csr.m, _ptr(off), _ptr(col), _ptr(colors), ctypes.byref(nc))
+    return colors, int(nc.value)
+
+
+def color_order(colors: np.ndarray, n_colors: int):
+    """The SymGS driver's ordering (src/symgs/main.cc:38-49): (indices,
+    color_offsets) with the vertices sorted by color, stable."""
+    colors = np.ascontiguousarray(colors, np.int32)
+    indices = np.empty(len(colors), np.int32)
+    co = np.empty(n_colors + 1, np.int32)
+    N.call("gd_color_order", len(colors), _ptr(colors), n_colors, _ptr(indices), _ptr(co))
+    return indices, co
+
+
 def read_mtx(path: str, *, symmetrize=False, transpose=False, sorted=True, remove_selfloops=True,
              remove_redundents=True) -> Csr:
     """mtx2csr() + degree, flags as read_graph() (include/graph_io.h:280)."""
@@ -111,20 +194,52 @@ def read_mtx(path: str, *, symmetrize=False, transpose=False, sorted=True, remov
     return _take(h)
 
 
+def _read(fn: str, path: str, symmetrize, transpose, sorted, remove_selfloops,
+          remove_redundents) -> Csr:
+    h = N.HostCsr()
+    N.call(fn, os.fsencode(path), int(symmetrize), int(transpose), int(sorted),
+           int(remove_selfloops), int(remove_redundents), ctypes.byref(h))
+    return _take(h)
+
+
+def read_gr(path: str, *, symmetrize=False, transpose=False, sorted=True, remove_selfloops=True,
+            remove_redundents=True) -> Csr:
+    """gr2csr() + degree (include/graph_io.h:133-174): DIMACS "a src dst"
+    arcs, 1-based, weights ignored."""
+    return _read("gd_read_gr", path, symmetrize, transpose, sorted, remove_selfloops,
+                 remove_redundents)
+
+
+def read_dotgraph(path: str, *, symmetrize=False, transpose=False, sorted=True,
+                  remove_selfloops=True, remove_redundents=True) -> Csr:
+    """graph2csr() + degree (include/graph_io.h:177-213): one line of 1-based
+    neighbours per vertex."""
+    return _read("gd_read_dotgraph", path, symmetrize, transpose, sorted, remove_selfloops,
+                 remove_redundents)
+
+
+def read_graph(path: str, *, symmetrize=False, transpose=False, sorted=True,
+               remove_selfloops=True, remove_redundents=True) -> Csr:
+    """read_graph() (include/graph_io.h:280-295): .mtx, .graph or .gr chosen
+    from the file name, in that order."""
+    return _read("gd_read_graph", path, symmetrize, transpose, sorted, remove_selfloops,
+                 remove_redundents)
+
+
 def read_pr_graph(path: str, is_directed: bool = True):
     """The two read_graph() calls of the PR driver (src/pr/main.cc:23-24):
     out-CSR then in-CSR, unsorted, self-loops and adjacent duplicates removed."""
     sym = not is_directed
-    out_csr = read_mtx(path, symmetrize=sym, transpose=False, sorted=False)
-    in_csr = read_mtx(path, symmetrize=sym, transpose=True, sorted=False)
+    out_csr = read_graph(path, symmetrize=sym, transpose=False, sorted=False)
+    in_csr = read_graph(path, symmetrize=sym, transpose=True, sorted=False)
     return out_csr, in_csr
 
 
 def read_spmv_matrix(path: str, is_directed: bool = True) -> Csr:
     """The SpMV driver's read_graph() flags (src/spmv/main.cc:24): sorted,
     self-loops kept, adjacent duplicates removed."""
-    return read_mtx(path, symmetrize=not is_directed, transpose=False, sorted=True,
-                    remove_selfloops=False, remove_redundents=True)
+    return read_graph(path, symmetrize=not is_directed, transpose=False, sorted=True,
+                      remove_selfloops=False, remove_redundents=True)
 
 
 def write_mtx(path: str, m: int, src: np.ndarray, dst: np.ndarray) -> None:

@@ -54,6 +54,33 @@ int gd_csr_from_edges(int m, int64_t n, const int32_t *src, const int32_t *dst, 
 int gd_read_mtx(const char *path, int symmetrize, int transpose, int sorted, int remove_selfloops,
                 int remove_redundents, gd_host_csr *out);
 
+/* gr2csr() + degree (include/graph_io.h:133-174): DIMACS arcs "a src dst [w]"
+ * (1-based, weight ignored), symmetrize mirrors self-loops as well. */
+int gd_read_gr(const char *path, int symmetrize, int transpose, int sorted, int remove_selfloops,
+               int remove_redundents, gd_host_csr *out);
+
+/* graph2csr() + degree (include/graph_io.h:177-213): "m nnz" header, then one
+ * line of 1-based neighbours per vertex. */
+int gd_read_dotgraph(const char *path, int symmetrize, int transpose, int sorted,
+                     int remove_selfloops, int remove_redundents, gd_host_csr *out);
+
+/* read_graph() dispatch (include/graph_io.h:280-295): ".mtx", ".graph", ".gr"
+ * tested in that order with strstr; anything else is GD_ERR_IO. */
+int gd_read_graph(const char *path, int symmetrize, int transpose, int sorted, int remove_selfloops,
+                  int remove_redundents, gd_host_csr *out);
+
+/* Deterministic sequential first-fit coloring (vertex order) over the CSR and
+ * its transpose, self-loops ignored: colors[v] in [0, *n_colors), no two rows
+ * joined by an off-diagonal entry in either direction share a color.  Stands
+ * in for the SymGS driver's VCSolver call (src/symgs/main.cc:41). */
+int gd_color_first_fit(int m, const int32_t *row_offsets, const int32_t *column_indices,
+                       int32_t *colors, int32_t *n_colors);
+
+/* The SymGS driver's ordering (src/symgs/main.cc:38-49): indices[m] = vertices
+ * sorted by color (increasing id inside a color), color_offsets[n_colors+1]. */
+int gd_color_order(int m, const int32_t *colors, int32_t n_colors, int32_t *indices,
+                   int32_t *color_offsets);
+
 /* Seeded, unpermuted RMAT (a,b,c, d=1-a-b-c), 2^scale vertices,
  * edgefactor*2^scale raw edges; self-loops and duplicates removed.  Fills the
  * out-CSR (rows = sources) and/or the in-CSR (rows = destinations), both with

@@ -120,6 +120,42 @@ int gd_pr_plan_info(const gd_pr_plan *plan, gd_pr_plan_info_t *info);
 
 int gd_pr_plan_destroy(gd_pr_plan *plan);
 
+/* fill_data() on the device (include/graph_io.h:21-130): CSR from an edge list
+ * already in HBM (0-based src/dst, optional weights; file order), the flags of
+ * gd_csr_from_edges.  Same result as the host builder, except that with
+ * `sorted` duplicates of one edge are kept in push order (std::sort leaves
+ * their order unspecified), which can only change the weight kept for them.
+ * The library allocates the output arrays; copy them out with
+ * gd_device_csr_copy and release them with gd_device_csr_free. */
+typedef struct gd_device_csr {
+  int32_t m;
+  int64_t nnz;
+  int32_t *row_offsets;    /* m+1, device */
+  int32_t *column_indices; /* nnz, device */
+  float *weight;           /* nnz, device (1.0 when no weights were given) */
+  int32_t *degree;         /* m, device: row lengths */
+} gd_device_csr;
+
+int gd_csr_from_edges_device(int m, int64_t n, const int32_t *d_src, const int32_t *d_dst,
+                             const float *d_weight, int symmetrize, int transpose, int sorted,
+                             int remove_selfloops, int remove_redundents, int mirror_selfloops,
+                             void *stream, gd_device_csr *out);
+/* Device-to-device copies into caller arrays (any may be NULL), on stream. */
+int gd_device_csr_copy(const gd_device_csr *c, int32_t *d_row_offsets, int32_t *d_column_indices,
+                       float *d_weight, int32_t *d_degree, void *stream);
+int gd_device_csr_free(gd_device_csr *c);
+
+/* The reference PRVerifier's residual (src/pr/verifier.cc:10-69, the push part
+ * :50-68) on the device: one push iteration of the tested scores over the
+ * OUT-CSR (d_degree = out-degree), residual = sum_v |base + d*sums_v -
+ * scores_v| in FP64.  Each per-vertex term is bit-identical to the
+ * reference's (sources added in increasing order with sequential FP32
+ * rounding); only the FP64 sum runs in a different fixed order.  The
+ * reference prints "Correct" when residual < EPSILON.  Synchronises stream. */
+int gd_pr_verify(int m, int64_t nnz, const int32_t *d_out_row_offsets,
+                 const int32_t *d_out_column_indices, const int32_t *d_degree,
+                 const float *d_scores, float damping, void *stream, double *residual);
+
 /* One-shot form of PRSolver with host arrays (src/pr/pr.h:23): copies the
  * in-CSR, degree and scores to HBM, runs, copies scores back, frees.  The
  * out-CSR arguments are accepted and unused, as in every pull variant of the
@@ -308,6 +344,48 @@ int gd_spmv_bwd_run(gd_spmv_bwd_plan *plan, const float *d_Ax, const float *d_x,
                     float *d_dx, float *d_dA);
 
 int gd_spmv_bwd_plan_destroy(gd_spmv_bwd_plan *plan);
+
+/* ---- SymGS: multicolor symmetric Gauss-Seidel sweep ----------------------
+ * Replaces SymGSSolver (src/symgs/symgs.h:28; omp_base.cc:27-42).  One sweep
+ * = colors 0..C-1 then C-1..0; each row r of color c (r = indices[k],
+ * color_offsets[c] <= k < color_offsets[c+1]) gets
+ *   x[r] = (b[r] - sum_{j != r} x[j] * A[r][j]) / A[r][r]   unless A[r][r] == 0,
+ * the sum in CSR order with sequential FP32 rounding and A[r][r] the row's last
+ * diagonal entry: bit-exact with the reference's sequential sweep
+ * (src/symgs/verifier.cc:48-76).  The coloring must be proper (no off-diagonal
+ * entry joins two rows of one color) and list each row at most once; the plan
+ * checks both.  Device arrays: d_Ap[m+1], d_Aj/d_Ax[nnz], d_indices (the
+ * entries color_offsets[0] .. color_offsets[n_colors]-1 are read); host
+ * color_offsets[n_colors+1] within [0, m]. */
+typedef struct gd_symgs_plan gd_symgs_plan;
+
+typedef struct gd_symgs_plan_info_t {
+  int32_t rows;         /* rows swept: color_offsets[C] - color_offsets[0] */
+  int32_t n_colors;
+  int64_t offdiag_nnz;  /* off-diagonal entries of the swept rows */
+  int32_t n_tiles;      /* LDS row tiles over all colors */
+  int32_t n_exact_rows; /* rows of more than 64 off-diagonal entries */
+  int32_t max_row_len;
+  double plan_ms;
+} gd_symgs_plan_info_t;
+
+int gd_symgs_plan_create(int m, int64_t nnz, const int32_t *d_Ap, const int32_t *d_Aj,
+                         const float *d_Ax, const int32_t *d_indices,
+                         const int32_t *color_offsets, int n_colors, void *stream,
+                         gd_symgs_plan **plan);
+
+/* `sweeps` symmetric sweeps in place on d_x (m floats), right-hand side d_b. */
+int gd_symgs_plan_run(gd_symgs_plan *plan, float *d_x, const float *d_b, int sweeps, void *stream);
+
+int gd_symgs_plan_info(const gd_symgs_plan *plan, gd_symgs_plan_info_t *info);
+
+int gd_symgs_plan_destroy(gd_symgs_plan *plan);
+
+/* SymGSSolver semantics on host arrays (one sweep, x updated in place);
+ * kernel_ms (may be NULL) receives the device time of the sweep. */
+int gd_symgs_solve(int m, int nnz, const int32_t *Ap, const int32_t *Aj, const int32_t *indices,
+                   const float *Ax, float *x, const float *b, const int32_t *color_offsets,
+                   int n_colors, float *kernel_ms);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,7 @@
  *
  *   void PRSolver(...)    replaces src/pr/pr.h:23      (e.g. src/pr/omp_base.cc:9)
  *   void SpmvSolver(...)  replaces src/spmv/spmv.h:29  (e.g. src/spmv/omp_base.cc:8)
+ *   void SymGSSolver(...) replaces src/symgs/symgs.h:28 (e.g. src/symgs/omp_base.cc:27)
  *
  * Behaviour mirrors the reference plug-ins: host arrays in, scores / y
  * updated in place, the same stdout lines, EPSILON 1e-3 / MAX_ITER 30 /
@@ -18,9 +19,14 @@
 #ifndef GARDENIA_SOLVER_H_
 #define GARDENIA_SOLVER_H_
 
+#include <vector>
+
 void PRSolver(int m, int nnz, int *in_row_offsets, int *in_column_indices, int *out_row_offsets,
               int *out_column_indices, int *degree, float *scores);
 
 void SpmvSolver(int m, int nnz, int *Ap, int *Aj, float *Ax, float *x, float *y);
+
+void SymGSSolver(int num_rows, int nnz, int *Ap, int *Aj, int *indices, float *Ax, float *x,
+                 float *b, std::vector<int> color_offsets);
 
 #endif /* GARDENIA_SOLVER_H_ */

@@ -45,6 +45,7 @@ _SIG = {
     "orc_spmv_bwd_fp64": (None, [_i, _i, _P, _P, _P, _P, _P, _P, _P]),
     "orc_wpr_fp64": (_i, [_i, _P, _P, _P, _P, _d, _d, _i, _i]),
     "orc_wpr_grad_fp64": (_i, [_i, _P, _P, _P, _P, _P, _d, _d, _i, _P, _P]),
+    "orc_symgs": (None, [_P, _P, _P, _P, _P, _P, _P, _i, _i]),
     "orc_num_threads": (_i, []),
 }
 
@@ -166,6 +167,17 @@ def spmv(m, Ap, Aj, Ax, x, y):
     Ax, x = np.ascontiguousarray(Ax, np.float32), np.ascontiguousarray(x, np.float32)
     out = np.array(y, np.float32, copy=True)
     lib().orc_spmv(m, _p(Ap), _p(Aj), _p(Ax), _p(x), _p(out))
+    return out
+
+
+def symgs(Ap, Aj, indices, Ax, x, b, color_offsets, sweeps=1):
+    """src/symgs/verifier.cc:48-76 (gs_serial over the colors forward, then
+    backward); returns a new x."""
+    Ap, Aj, indices, co = _i32(Ap), _i32(Aj), _i32(indices), _i32(color_offsets)
+    Ax, b = np.ascontiguousarray(Ax, np.float32), np.ascontiguousarray(b, np.float32)
+    out = np.array(x, np.float32, copy=True)
+    lib().orc_symgs(_p(Ap), _p(Aj), _p(indices), _p(Ax), _p(out), _p(b), _p(co), len(co) - 1,
+                    sweeps)
     return out
 
 

@@ -25,6 +25,10 @@
 //   orc_ppr_fp64      linearity + dense solve on small graphs.
 //   orc_wpr_*         central finite differences + dense solve.
 //   orc_spmv_bwd_fp64 dense matrix transpose on small inputs.
+//   orc_symgs         golden x from the reference's SymGSSolver (omp_base.cc)
+//                     and verdicts of its SymGSVerifier, compiled from source;
+//                     hand-computed 3x3 sweeps; many sweeps on an SPD system
+//                     converge to the dense solve.
 
 #include <algorithm>
 #include <cmath>
@@ -351,6 +355,40 @@ float orc_max_rel_err(const float* A, const float* B, long long N) {
     if (error != 0) max_error = std::max(max_error, error / (std::fabs(a) + std::fabs(b) + eps));
   }
   return max_error;
+}
+
+// ---------------------------------------------------------------- SymGS
+
+// src/symgs/verifier.cc:48-64 (gs_serial): rows indices[row_start..row_stop)
+// in order; each row's off-diagonal products summed in CSR order (product
+// rounded before the add), the diagonal the row's LAST entry with column ==
+// row; x[row] = (b[row] - rsum) / diag unless diag == 0.
+static void gs_serial(const int* Ap, const int* Aj, const int* indices, const float* Ax, float* x,
+                      const float* b, int row_start, int row_stop) {
+  for (int i = row_start; i != row_stop; i++) {
+    const int inew = indices[i];
+    float rsum = 0, diag = 0;
+    for (int jj = Ap[inew]; jj < Ap[inew + 1]; jj++) {
+      const int j = Aj[jj];
+      if (inew == j)
+        diag = Ax[jj];
+      else
+        rsum += x[j] * Ax[jj];
+    }
+    if (diag != 0) x[inew] = (b[inew] - rsum) / diag;
+  }
+}
+
+// src/symgs/verifier.cc:70-76 (SymGSVerifier's sweep) = src/symgs/omp_base.cc:
+// 35-38: colors 0..C-1 forward, then C-1..0 backward; `sweeps` repetitions.
+void orc_symgs(const int* Ap, const int* Aj, const int* indices, const float* Ax, float* x,
+               const float* b, const int* color_offsets, int n_colors, int sweeps) {
+  for (int s = 0; s < sweeps; s++) {
+    for (int c = 0; c < n_colors; c++)
+      gs_serial(Ap, Aj, indices, Ax, x, b, color_offsets[c], color_offsets[c + 1]);
+    for (int c = n_colors; c > 0; c--)
+      gs_serial(Ap, Aj, indices, Ax, x, b, color_offsets[c - 1], color_offsets[c]);
+  }
 }
 
 // ---------------------------------------------------------------- adjoints (FP64)

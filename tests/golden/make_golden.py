@@ -19,6 +19,16 @@ oracle/ref_harness.cc and records inputs and outputs:
     sample at 16), iteration count, per-iteration L1, verifier verdict.
   * SpMV on an Erdos-Renyi 2^12 x 2^12 matrix (32 nnz/row) and on RMAT-12.
 
+`python tests/golden/make_golden.py formats` records read_graph() on the
+reference's .graph datasets (graph2csr) and on a 1-based .gr of raw RMAT-8
+edges (gr2csr) under seven flag sets: tests/golden/ref_formats.npz / .json.
+
+`python tests/golden/make_golden.py symgs` records the reference's
+SymGSSolver (src/symgs/omp_base.cc, one symmetric sweep; three in a row) and
+SymGSVerifier verdict on a 27-point stencil, a weighted RMAT-10 with and
+without diagonals, 4w.mtx and chesapeake.mtx, read with the SymGS driver's
+flags: tests/golden/ref_symgs.npz / .json.
+
 The reference's compile-time constants apply: EPSILON 1e-3, MAX_ITER 30
 (src/pr/pr.h:4-5).  Output: tests/golden/ref_golden.npz and ref_golden.json.
 """
@@ -204,5 +214,148 @@ def main():
     print("wrote", len(arrays), "arrays")
 
 
+FORMAT_DATASETS = ["4.graph", "test1.graph", "test2.graph", "test3.graph", "test4.graph",
+                   "test5.graph", "test6.graph", "rmat8_raw.gr"]
+FORMAT_FLAGS = {
+    "pr_out": (0, 0, 0, 1, 1), "pr_in": (0, 1, 0, 1, 1), "spmv": (0, 0, 1, 0, 1),
+    "sym_out": (1, 0, 0, 1, 1), "sym_in": (1, 1, 0, 1, 1), "raw": (0, 0, 0, 0, 0),
+    "sym_raw": (1, 0, 0, 0, 0),
+}
+
+
+def write_raw_gr(path, scale=8, edgefactor=8, seed=5):
+    """A 1-based DIMACS .gr of the raw RMAT edges (self-loops and duplicates
+    kept) from this package's generator, with a comment line and weights."""
+    src, dst = graph.rmat_edges(scale, edgefactor, seed=seed)
+    m = 1 << scale
+    with open(path, "w") as f:
+        f.write(f"c raw RMAT-{scale} edges, edgefactor {edgefactor}, seed {seed}\n")
+        f.write(f"p sp {m} {len(src)}\n")
+        for i, (a, b) in enumerate(zip(src.tolist(), dst.tolist())):
+            f.write(f"a {a + 1} {b + 1} {(i * 37) % 100 + 1}\n")
+
+
+def formats():
+    """read_graph() on the .graph datasets (graph2csr) and a .gr file (gr2csr),
+    every flag set above -> tests/golden/ref_formats.npz / .json."""
+    ref = load_ref()
+    gr = os.path.join(GOLDEN, "datasets", "rmat8_raw.gr")
+    if not os.path.exists(gr):
+        write_raw_gr(gr)
+    arrays, meta = {}, {}
+    with tempfile.TemporaryDirectory() as tmp:
+        log = os.path.join(tmp, "read.log")
+        for ds in FORMAT_DATASETS:
+            path = os.path.join(GOLDEN, "datasets", ds)
+            key = ds.replace(".", "_")
+            for vname, flags in FORMAT_FLAGS.items():
+                m, c = read_graph(ref, path, *flags, log)
+                for f, a in c.items():
+                    arrays[f"{key}/{vname}/{f}"] = a
+            meta[ds] = {"m": m}
+    np.savez_compressed(os.path.join(GOLDEN, "ref_formats.npz"), **arrays)
+    with open(os.path.join(GOLDEN, "ref_formats.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+    print("wrote", len(arrays), "arrays")
+
+
+def write_weighted_mtx(path, m, rows, cols, vals):
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real general\n")
+        f.write(f"{m} {m} {len(rows)}\n")
+        for r, c, v in zip(rows.tolist(), cols.tolist(), vals.tolist()):
+            f.write(f"{r + 1} {c + 1} {v:.9g}\n")
+
+
+def stencil27(n):
+    """graph.stencil27(n) as (m, rows, cols, vals) for the .mtx writer."""
+    c = graph.stencil27(n)
+    rows = np.repeat(np.arange(c.m, dtype=np.int64), c.degree)
+    return c.m, rows, c.column_indices.astype(np.int64), c.weight
+
+
+def rmat_with_diag(scale, seed):
+    """Raw RMAT edges (self-loops kept) with weights, plus a diagonal entry on
+    every even row: rows with and without a diagonal."""
+    src, dst = graph.rmat_edges(scale, 8, seed=seed)
+    m = 1 << scale
+    w = np.abs(graph.uniform_pm1(len(src), seed + 7)) + np.float32(0.125)
+    ev = np.arange(0, m, 2, dtype=np.int64)
+    rows = np.concatenate([src.astype(np.int64), ev])
+    cols = np.concatenate([dst.astype(np.int64), ev])
+    vals = np.concatenate([w, np.full(len(ev), 24.0, np.float32)])
+    return m, rows, cols, vals
+
+
+def load_ref_symgs():
+    path = os.path.join(os.path.dirname(oracle.REF_LIB_PATH), "libref_symgs.so")
+    if not os.path.exists(path):
+        raise SystemExit("oracle/_ref/libref_symgs.so missing: run `make -C oracle ref` first")
+    ref = ctypes.CDLL(path)
+    I = ctypes.c_int
+    ref.ref_symgs_solve.argtypes = [I, I, _P, _P, _P, _P, _P, _P, _P, I, ctypes.c_char_p]
+    ref.ref_symgs_verify.argtypes = [I, _P, _P, _P, _P, _P, _P, _P, _P, I, ctypes.c_char_p]
+    return ref
+
+
+SYMGS_FLAGS = (1, 0, 1, 0, 1)  # src/symgs/main.cc:18: symmetrize, sorted, self-loops kept
+
+
+def symgs():
+    """SymGSSolver (src/symgs/omp_base.cc) on matrices read with the SymGS
+    driver's flags, colored by gd_color_first_fit and ordered as the driver
+    orders them -> tests/golden/ref_symgs.npz / .json."""
+    ref = load_ref()
+    rs = load_ref_symgs()
+    arrays, meta = {}, {}
+    with tempfile.TemporaryDirectory() as tmp:
+        log = os.path.join(tmp, "log")
+        cases = {}
+        for name, (m, r, c, v) in {"stencil6": stencil27(6),
+                                   "rmat10_diag": rmat_with_diag(10, 11)}.items():
+            path = os.path.join(tmp, name + ".mtx")
+            write_weighted_mtx(path, m, r, c, v)
+            cases[name] = path
+        cases["4w"] = os.path.join(GOLDEN, "datasets", "4w.mtx")
+        cases["chesapeake"] = os.path.join(GOLDEN, "datasets", "chesapeake.mtx")
+        for seed, (name, path) in enumerate(cases.items()):
+            m, c = read_graph(ref, path, *SYMGS_FLAGS, log)
+            csr = graph.Csr(m, c["row_offsets"], c["column_indices"], c["degree"], c["weight"])
+            colors, nc = graph.color_first_fit(csr)
+            idx, co = graph.color_order(colors, nc)
+            x0 = graph.uniform_pm1(m, 500 + seed)
+            b = graph.uniform_pm1(m, 600 + seed)
+            x = x0.copy()
+            nnz = int(c["row_offsets"][-1])
+            solve = lambda xx: rs.ref_symgs_solve(  # noqa: E731
+                m, nnz, p(c["row_offsets"]), p(c["column_indices"]), p(idx), p(c["weight"]),
+                p(xx), p(b), p(co), nc, os.fsencode(log))
+            solve(x)
+            x1 = x.copy()
+            solve(x)
+            solve(x)
+            rs.ref_symgs_verify(m, p(c["row_offsets"]), p(c["column_indices"]), p(idx),
+                                p(c["weight"]), p(x1), p(x0), p(b), p(co), nc, os.fsencode(log))
+            verdict = "Correct" if "Correct" in open(log).read() else "POSSIBLE FAILURE"
+            for f in ("row_offsets", "column_indices", "weight"):
+                arrays[f"{name}/{f}"] = c[f]
+            for k_, a in (("indices", idx), ("color_offsets", co), ("x0", x0), ("b", b),
+                          ("x1", x1), ("x3", x)):
+                arrays[f"{name}/{k_}"] = a
+            meta[name] = {"m": m, "nnz": nnz, "n_colors": nc, "verdict_x1": verdict,
+                          "rows_with_diag": int(sum(
+                              np.any(c["column_indices"][c["row_offsets"][i]:c["row_offsets"][i + 1]] == i)
+                              for i in range(m)))}
+    np.savez_compressed(os.path.join(GOLDEN, "ref_symgs.npz"), **arrays)
+    with open(os.path.join(GOLDEN, "ref_symgs.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+    print("wrote", len(arrays), "arrays", meta)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["formats"]:
+        formats()
+    elif sys.argv[1:] == ["symgs"]:
+        symgs()
+    else:
+        main()

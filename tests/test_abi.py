@@ -49,6 +49,8 @@ def test_reference_plugin_symbols_exported():
                          check=True).stdout
     assert "_Z8PRSolveriiPiS_S_S_S_Pf" in out  # PRSolver(int,int,int*,int*,int*,int*,int*,float*)
     assert "_Z10SpmvSolveriiPiS_PfS0_S0_" in out  # SpmvSolver(int,int,int*,int*,float*,float*,float*)
+    # SymGSSolver(int,int,int*,int*,int*,float*,float*,float*,std::vector<int>)
+    assert "_Z11SymGSSolveriiPiS_S_PfS0_S0_St6vectorIiSaIiEE" in out
 
 
 def test_argument_errors_before_device():
@@ -63,6 +65,11 @@ def test_argument_errors_before_device():
     assert lib.gd_pr_plan_run(None, None, None, None, None) == N.GD_ERR_INVALID
     assert lib.gd_spmv_plan_run(None, None, None) == N.GD_ERR_INVALID
     assert lib.gd_pr_plan_destroy(None) == N.GD_OK
+    co = (ctypes.c_int32 * 3)(0, 5, 3)
+    assert lib.gd_symgs_plan_create(8, 0, None, None, None, None, co, 2, None,
+                                    ctypes.byref(h)) == N.GD_ERR_INVALID
+    assert lib.gd_symgs_plan_run(None, None, None, 1, None) == N.GD_ERR_INVALID
+    assert lib.gd_symgs_plan_destroy(None) == N.GD_OK
 
 
 def test_default_options_are_reference_constants():

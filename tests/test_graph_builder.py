@@ -147,3 +147,81 @@ def test_write_then_read_mtx_roundtrip(tmp_path):
     np.testing.assert_array_equal(o2.column_indices, o.column_indices)
     np.testing.assert_array_equal(i2.row_offsets, i.row_offsets)
     np.testing.assert_array_equal(i2.column_indices, i.column_indices)
+
+
+# --- .graph (graph2csr) and .gr (gr2csr) readers --------------------------------
+
+FORMAT_DATASETS = ["4.graph", "test1.graph", "test2.graph", "test3.graph", "test4.graph",
+                   "test5.graph", "test6.graph", "rmat8_raw.gr"]
+FORMAT_FLAGS = {
+    "pr_out": (0, 0, 0, 1, 1), "pr_in": (0, 1, 0, 1, 1), "spmv": (0, 0, 1, 0, 1),
+    "sym_out": (1, 0, 0, 1, 1), "sym_in": (1, 1, 0, 1, 1), "raw": (0, 0, 0, 0, 0),
+    "sym_raw": (1, 0, 0, 0, 0),
+}
+
+
+@pytest.fixture(scope="module")
+def formats_golden():
+    import os
+
+    from conftest import GOLDEN_DIR
+
+    return np.load(os.path.join(GOLDEN_DIR, "ref_formats.npz"), allow_pickle=False)
+
+
+@pytest.mark.parametrize("ds", FORMAT_DATASETS)
+@pytest.mark.parametrize("variant", list(FORMAT_FLAGS))
+def test_read_graph_formats_match_reference(formats_golden, ds, variant):
+    s, t, so, rs, rr = FORMAT_FLAGS[variant]
+    kw = dict(symmetrize=bool(s), transpose=bool(t), sorted=bool(so), remove_selfloops=bool(rs),
+              remove_redundents=bool(rr))
+    direct = graph.read_gr if ds.endswith(".gr") else graph.read_dotgraph
+    k = ds.replace(".", "_") + "/" + variant
+    for c in (graph.read_graph(dataset(ds), **kw), direct(dataset(ds), **kw)):
+        np.testing.assert_array_equal(c.row_offsets, formats_golden[k + "/row_offsets"])
+        np.testing.assert_array_equal(c.column_indices, formats_golden[k + "/column_indices"])
+        np.testing.assert_array_equal(c.weight, formats_golden[k + "/weight"])
+        np.testing.assert_array_equal(c.degree, formats_golden[k + "/degree"])
+
+
+def test_gr_reader_matches_oracle_fill_data():
+    # gr2csr = the raw arcs through fill_data with self-loops mirrored and unit weights
+    src, dst = graph.rmat_edges(8, 8, seed=5)
+    for flags in itertools.product([0, 1], repeat=5):
+        kw = dict(zip(("symmetrize", "transpose", "sorted", "remove_selfloops",
+                       "remove_redundents"), map(bool, flags)))
+        a = graph.read_gr(dataset("rmat8_raw.gr"), **kw)
+        b = oracle.csr_from_edges(256, src, dst, None, mirror_selfloops=True, **kw)
+        np.testing.assert_array_equal(a.row_offsets, b.row_offsets, err_msg=str(kw))
+        np.testing.assert_array_equal(a.column_indices, b.column_indices, err_msg=str(kw))
+        np.testing.assert_array_equal(a.weight, b.weight, err_msg=str(kw))
+
+
+def test_dotgraph_short_file_and_stray_tokens(tmp_path):
+    p = tmp_path / "short.graph"
+    # vertex 2's line stops at the first non-integer token; vertices 4 and 5 have no line
+    p.write_text("5 4\n2 3\n1 x 3\n1 2\n")
+    c = graph.read_graph(str(p), sorted=False)
+    np.testing.assert_array_equal(c.row_offsets, [0, 2, 3, 5, 5, 5])
+    np.testing.assert_array_equal(c.column_indices, [1, 2, 0, 0, 1])
+    np.testing.assert_array_equal(c.weight, np.ones(5, np.float32))
+
+
+def test_readers_reject_what_the_reference_cannot_read(tmp_path):
+    from gardenia_ocl_amd import GardeniaError
+
+    # 4.gr numbers its vertices from 0 and 4w.graph interleaves weights with
+    # neighbours: both give ids outside [1, m], which the reference indexes
+    # out of bounds with; here they are errors.
+    with pytest.raises(GardeniaError, match="outside"):
+        graph.read_graph(dataset("4.gr"))
+    with pytest.raises(GardeniaError, match="outside"):
+        graph.read_graph(dataset("4w.graph"))
+    bad = tmp_path / "bad.gr"
+    bad.write_text("p sp 3 2\na 1 2\nc comment inside the arcs\n")
+    with pytest.raises(GardeniaError, match="arc line"):
+        graph.read_gr(str(bad))
+    with pytest.raises(GardeniaError, match="Unrecognizable"):
+        graph.read_graph(str(tmp_path / "x.edges"))
+    with pytest.raises(GardeniaError, match="GD_ERR_IO"):
+        graph.read_graph("/nonexistent/file.graph")
