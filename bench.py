@@ -445,6 +445,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (scales rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
+    ap.add_argument("--only", choices=["symgs", "csr_build", "spmv"], default=None,
+                    help="run just this secondary figure and print its JSON (for profiling)")
     ap.add_argument("--tol-run", type=float, default=1e-6,
                     help="L1 tolerance of the extra run to convergence (with its FP64 L1)")
     args = ap.parse_args()
@@ -460,6 +462,19 @@ def main():
     from gardenia_ocl_amd import PageRankPlan, graph
 
     dev = torch.device("cuda:0")
+    if args.only == "symgs":
+        print(json.dumps({"symgs": bench_symgs(128, 20, args.seed)}), flush=True)
+        return
+    if args.only == "spmv":
+        print(json.dumps({"spmv": bench_spmv(args.spmv_log2, 32, 50, args.seed)}), flush=True)
+        return
+    if args.only == "csr_build":
+        t0 = time.perf_counter()
+        _, ic = graph.rmat(args.scale, args.edgefactor, seed=args.seed, want_out=False)
+        host_s = time.perf_counter() - t0
+        print(json.dumps({"csr_build": bench_csr_build(args.scale, args.seed, host_s, ic.nnz)}),
+              flush=True)
+        return
     if args.calibrate_copy:
         a = torch.ones(256 * 2**20, dtype=torch.float32, device=dev)  # 1 GiB
         b = torch.empty_like(a)

@@ -8,7 +8,8 @@
 //      pushed the transpose flag stays cleared for the rest of the list (the
 //      reference's loop resets it), so placement is a prefix sum over the
 //      per-edge record counts plus the index of the first mirrored edge;
-//   2. a stable radix sort of the records by row (by (row, column) when
+//   2. a stable radix sort of the records (64-bit keys carrying row and
+//      column, the weight as the value) by row (by (row, column) when
 //      `sorted`), so each row keeps its push order.  std::sort in the
 //      reference is not stable: with `sorted` and duplicate edges of different
 //      weights, the weight kept by step 4 may differ; the structure, and
@@ -16,8 +17,8 @@
 //   3. self-loops dropped (graph_io.h:32-45);
 //   4. ADJACENT duplicates dropped, comparing with the previous kept record of
 //      the row (graph_io.h:48-61);
-//   5. row offsets by a histogram + exclusive scan, degree = row length
-//      (graph_io.h:72-78, :289-293).
+//   5. row offsets from the row boundaries of the sorted records, degree =
+//      row length (graph_io.h:72-78, :289-293).
 // Each step is a hipCUB primitive or a one-pass kernel; the edge list never
 // leaves HBM.
 
@@ -62,63 +63,83 @@ __global__ void count_kernel(int64_t n, const int32_t* __restrict__ src,
   GRID_LOOP(i, n) cnt[i] = 1 + ((sym && (mirror || src[i] != dst[i])) ? 1 : 0);
 }
 
-// records in push order: sort key (row, or row << 32 | column), column, weight
+// Records in push order, as one 64-bit sort key each plus the weight.  The
+// key carries the column with the row: (col << 32 | row) sorted on the row
+// bits only, or (row << 32 | col) sorted on both when `sorted`; the radix sort
+// moves the column along with the row, so no permutation gather is needed.
+__device__ __forceinline__ uint64_t make_key(int32_t r, int32_t c, int sorted) {
+  return sorted ? (((uint64_t)(uint32_t)r << 32) | (uint32_t)c)
+                : (((uint64_t)(uint32_t)c << 32) | (uint32_t)r);
+}
+__device__ __forceinline__ int32_t key_row(uint64_t k, int sorted) {
+  return (int32_t)(sorted ? (k >> 32) : (k & 0xffffffffu));
+}
+__device__ __forceinline__ int32_t key_col(uint64_t k, int sorted) {
+  return (int32_t)(sorted ? (k & 0xffffffffu) : (k >> 32));
+}
+
 __global__ void place_kernel(int64_t n, const int32_t* __restrict__ src,
                              const int32_t* __restrict__ dst, const float* __restrict__ wt,
                              const int64_t* __restrict__ at, int sym, int mirror, int transpose,
                              unsigned long long first_mirror, int sorted,
-                             uint64_t* __restrict__ key, int32_t* __restrict__ rcol,
-                             float* __restrict__ rw) {
+                             uint64_t* __restrict__ key, float* __restrict__ rw) {
   GRID_LOOP(i, n) {
     const int32_t s = src[i], d = dst[i];
     const float w = wt ? wt[i] : 1.0f;
     int64_t p = at[i];
     if (sym && (mirror || s != d)) {  // the mirrored record is pushed first
-      key[p] = sorted ? (((uint64_t)(uint32_t)d << 32) | (uint32_t)s) : (uint64_t)(uint32_t)d;
-      rcol[p] = s;
+      key[p] = make_key(d, s, sorted);
       rw[p] = w;
       p++;
     }
     const bool tr = transpose && (unsigned long long)i < first_mirror;
-    const int32_t r = tr ? d : s, c = tr ? s : d;
-    key[p] = sorted ? (((uint64_t)(uint32_t)r << 32) | (uint32_t)c) : (uint64_t)(uint32_t)r;
-    rcol[p] = c;
+    key[p] = tr ? make_key(d, s, sorted) : make_key(s, d, sorted);
     rw[p] = w;
   }
 }
 
-__global__ void iota_kernel(int64_t n, int32_t* __restrict__ a) { GRID_LOOP(i, n) a[i] = (int32_t)i; }
+// keep[i]: not a self-loop (remove_selfloops)
+__global__ void selfloop_flags_kernel(int64_t n, const uint64_t* __restrict__ key, int sorted,
+                                      int32_t* __restrict__ keep) {
+  GRID_LOOP(i, n) keep[i] = key_row(key[i], sorted) != key_col(key[i], sorted);
+}
 
-// sorted records: row from the key, column / weight gathered by the permutation
-__global__ void gather_records_kernel(int64_t n, const uint64_t* __restrict__ key, int sorted,
-                                      const int32_t* __restrict__ perm,
-                                      const int32_t* __restrict__ rcol,
-                                      const float* __restrict__ rw, int32_t* __restrict__ row,
-                                      int32_t* __restrict__ col, float* __restrict__ w) {
+// keep[i]: not equal to the previous record of its row (same row and column:
+// equal keys)
+__global__ void dup_flags_kernel(int64_t n, const uint64_t* __restrict__ key,
+                                 int32_t* __restrict__ keep) {
+  GRID_LOOP(i, n) keep[i] = !(i > 0 && key[i] == key[i - 1]);
+}
+
+__global__ void scatter_kept_kernel(int64_t n, const int32_t* __restrict__ keep,
+                                    const int32_t* __restrict__ pos,
+                                    const uint64_t* __restrict__ key, const float* __restrict__ w,
+                                    uint64_t* __restrict__ key2, float* __restrict__ w2) {
   GRID_LOOP(i, n) {
-    row[i] = (int32_t)(sorted ? (key[i] >> 32) : key[i]);
-    const int32_t p = perm[i];
-    col[i] = rcol[p];
-    w[i] = rw[p];
+    if (keep[i]) {
+      key2[pos[i]] = key[i];
+      w2[pos[i]] = w[i];
+    }
   }
 }
 
-__global__ void selfloop_flags_kernel(int64_t n, const int32_t* __restrict__ row,
-                                      const int32_t* __restrict__ col, int remove,
-                                      uint8_t* __restrict__ keep) {
-  GRID_LOOP(i, n) keep[i] = !(remove && row[i] == col[i]);
+// Row offsets from records sorted by row: record i sets off[q] = i for every
+// row q in (row of record i-1, row of record i]; the last record also closes
+// rows after it (off[q] = n up to q = m).  Every entry is written once.
+__global__ void offsets_kernel(int64_t n, int m, const uint64_t* __restrict__ key, int sorted,
+                               int32_t* __restrict__ off, int32_t* __restrict__ col) {
+  GRID_LOOP(i, n) {
+    const uint64_t k = key[i];
+    const int32_t r = key_row(k, sorted);
+    col[i] = key_col(k, sorted);
+    const int32_t p = (i == 0) ? -1 : key_row(key[i - 1], sorted);
+    for (int32_t q = p + 1; q <= r; q++) off[q] = (int32_t)i;
+    if (i == n - 1)
+      for (int32_t q = r + 1; q <= m; q++) off[q] = (int32_t)n;
+  }
 }
 
-__global__ void dup_flags_kernel(int64_t n, const int32_t* __restrict__ row,
-                                 const int32_t* __restrict__ col, int remove,
-                                 uint8_t* __restrict__ keep) {
-  GRID_LOOP(i, n) keep[i] = !(remove && i > 0 && row[i] == row[i - 1] && col[i] == col[i - 1]);
-}
-
-__global__ void histogram_kernel(int64_t n, const int32_t* __restrict__ row,
-                                 int32_t* __restrict__ cnt) {
-  GRID_LOOP(i, n) atomicAdd(cnt + row[i], 1);
-}
+__global__ void fill_kernel(int64_t n, int32_t v, int32_t* __restrict__ a) { GRID_LOOP(i, n) a[i] = v; }
 
 __global__ void degree_kernel(int m, const int32_t* __restrict__ off, int32_t* __restrict__ deg) {
   GRID_LOOP(i, m) deg[i] = off[i + 1] - off[i];
@@ -130,40 +151,36 @@ int sync_read(const void* dptr, void* h, size_t bytes, hipStream_t s) {
   return GD_OK;
 }
 
-// Stable compaction of (row, col, w) by keep flags.
-int compact(int64_t n, DeviceBuffer& row, DeviceBuffer& col, DeviceBuffer& w,
-            const DeviceBuffer& keep, hipStream_t s, int64_t* kept) {
+// Stable compaction of (key, w) by 0/1 flags: exclusive scan + scatter.
+// `keep` is filled by the caller; key / w are replaced by the kept records.
+int compact(int64_t n, DeviceBuffer& key, DeviceBuffer& w, const DeviceBuffer& keep,
+            hipStream_t s, int64_t* kept) {
   const size_t cap = (size_t)std::max<int64_t>(n, 1);
-  DeviceBuffer row2, col2, w2, nsel, tmp;
-  int rc = row2.alloc(4 * cap);
-  if (!rc) rc = col2.alloc(4 * cap);
+  DeviceBuffer pos, key2, w2, tmp;
+  int rc = pos.alloc(4 * cap);
+  if (!rc) rc = key2.alloc(8 * cap);
   if (!rc) rc = w2.alloc(4 * cap);
-  if (!rc) rc = nsel.alloc(sizeof(int64_t));
   if (rc) return rc;
-  size_t t1 = 0, t2 = 0;
-  GD_HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, t1, row.as<int32_t>(), keep.as<uint8_t>(),
-                                           row2.as<int32_t>(), nsel.as<int64_t>(), (int)n, s));
-  GD_HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, t2, w.as<float>(), keep.as<uint8_t>(),
-                                           w2.as<float>(), nsel.as<int64_t>(), (int)n, s));
-  size_t tb = std::max(t1, t2);
+  size_t tb = 0;
+  GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, keep.as<int32_t>(), pos.as<int32_t>(),
+                                              (int)n, s));
   rc = tmp.alloc(std::max<size_t>(tb, 1));
   if (rc) return rc;
-  GD_HIP_TRY(hipcub::DeviceSelect::Flagged(tmp.ptr, tb, row.as<int32_t>(), keep.as<uint8_t>(),
-                                           row2.as<int32_t>(), nsel.as<int64_t>(), (int)n, s));
-  GD_HIP_TRY(hipcub::DeviceSelect::Flagged(tmp.ptr, tb, col.as<int32_t>(), keep.as<uint8_t>(),
-                                           col2.as<int32_t>(), nsel.as<int64_t>(), (int)n, s));
-  GD_HIP_TRY(hipcub::DeviceSelect::Flagged(tmp.ptr, tb, w.as<float>(), keep.as<uint8_t>(),
-                                           w2.as<float>(), nsel.as<int64_t>(), (int)n, s));
-  int64_t h = 0;
-  rc = sync_read(nsel.ptr, &h, sizeof(int64_t), s);
+  GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.ptr, tb, keep.as<int32_t>(), pos.as<int32_t>(),
+                                              (int)n, s));
+  hipLaunchKernelGGL(scatter_kept_kernel, dim3(grid_for(n)), dim3(kB), 0, s, n,
+                     keep.as<int32_t>(), pos.as<int32_t>(), key.as<uint64_t>(), w.as<float>(),
+                     key2.as<uint64_t>(), w2.as<float>());
+  GD_HIP_TRY(hipGetLastError());
+  int32_t last[2] = {0, 0};
+  rc = sync_read(pos.as<int32_t>() + (n - 1), &last[0], 4, s);
+  if (!rc) rc = sync_read(keep.as<int32_t>() + (n - 1), &last[1], 4, s);
   if (rc) return rc;
-  std::swap(row.ptr, row2.ptr);
-  std::swap(row.bytes, row2.bytes);
-  std::swap(col.ptr, col2.ptr);
-  std::swap(col.bytes, col2.bytes);
+  std::swap(key.ptr, key2.ptr);
+  std::swap(key.bytes, key2.bytes);
   std::swap(w.ptr, w2.ptr);
   std::swap(w.bytes, w2.bytes);
-  *kept = h;
+  *kept = (int64_t)last[0] + last[1];
   return GD_OK;
 }
 
@@ -217,91 +234,74 @@ int build(int m, int64_t n, const int32_t* d_src, const int32_t* d_dst, const fl
   }
   GD_CHECK_ARG(cap < (1ll << 31), "more than 2^31 - 1 edges after symmetrisation");
   const size_t capb = (size_t)std::max<int64_t>(cap, 1);
-  DeviceBuffer row, col, w;
-  rc = row.alloc(4 * capb);
-  if (!rc) rc = col.alloc(4 * capb);
+  DeviceBuffer key, w;  // sorted records
+  rc = key.alloc(8 * capb);
   if (!rc) rc = w.alloc(4 * capb);
   if (rc) return rc;
   if (cap > 0) {
-    DeviceBuffer key, key2, rcol, rw, idx, idx2, tmp;
-    rc = key.alloc(8 * capb);
-    if (!rc) rc = key2.alloc(8 * capb);
-    if (!rc) rc = rcol.alloc(4 * capb);
-    if (!rc) rc = rw.alloc(4 * capb);
-    if (!rc) rc = idx.alloc(4 * capb);
-    if (!rc) rc = idx2.alloc(4 * capb);
+    DeviceBuffer key0, w0, tmp;
+    rc = key0.alloc(8 * capb);
+    if (!rc) rc = w0.alloc(4 * capb);
     if (rc) return rc;
     hipLaunchKernelGGL(place_kernel, dim3(grid_for(n)), dim3(kB), 0, s, n, d_src, d_dst, d_wt,
                        at.as<int64_t>(), sym, mirror, transpose, first_mirror, sorted,
-                       key.as<uint64_t>(), rcol.as<int32_t>(), rw.as<float>());
-    hipLaunchKernelGGL(iota_kernel, dim3(grid_for(cap)), dim3(kB), 0, s, cap, idx.as<int32_t>());
+                       key0.as<uint64_t>(), w0.as<float>());
     GD_HIP_TRY(hipGetLastError());
-    // 2. stable radix sort of record indices by key (only the bits in use)
+    // 2. stable radix sort by row (by row, then column, when sorted): only the
+    // bits in use
     int bits = 1;
     while (bits < 31 && ((int64_t)1 << bits) < m) bits++;
     const int end_bit = sorted ? 32 + bits : bits;
     size_t tb = 0;
-    GD_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key.as<uint64_t>(),
-                                                  key2.as<uint64_t>(), idx.as<int32_t>(),
-                                                  idx2.as<int32_t>(), (int)cap, 0, end_bit, s));
+    GD_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key0.as<uint64_t>(),
+                                                  key.as<uint64_t>(), w0.as<float>(),
+                                                  w.as<float>(), (int)cap, 0, end_bit, s));
     rc = tmp.alloc(std::max<size_t>(tb, 1));
     if (rc) return rc;
-    GD_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tb, key.as<uint64_t>(),
-                                                  key2.as<uint64_t>(), idx.as<int32_t>(),
-                                                  idx2.as<int32_t>(), (int)cap, 0, end_bit, s));
-    hipLaunchKernelGGL(gather_records_kernel, dim3(grid_for(cap)), dim3(kB), 0, s, cap,
-                       key2.as<uint64_t>(), sorted, idx2.as<int32_t>(), rcol.as<int32_t>(),
-                       rw.as<float>(), row.as<int32_t>(), col.as<int32_t>(), w.as<float>());
-    GD_HIP_TRY(hipGetLastError());
+    GD_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tb, key0.as<uint64_t>(),
+                                                  key.as<uint64_t>(), w0.as<float>(),
+                                                  w.as<float>(), (int)cap, 0, end_bit, s));
     GD_HIP_TRY(hipStreamSynchronize(s));  // scratch is freed on scope exit
   }
   at.release();
-  // 3. self-loops, 4. adjacent duplicates (against the previous kept record)
+  // 3. self-loops, 4. adjacent duplicates (against the previous kept record:
+  // equal keys once the self-loops are gone)
   int64_t nk = cap;
-  if (cap > 0 && rm_self) {
+  if (nk > 0 && (rm_self || rm_dup)) {
     DeviceBuffer keep;
-    rc = keep.alloc(capb);
+    rc = keep.alloc(4 * capb);
     if (rc) return rc;
-    hipLaunchKernelGGL(selfloop_flags_kernel, dim3(grid_for(nk)), dim3(kB), 0, s, nk,
-                       row.as<int32_t>(), col.as<int32_t>(), 1, keep.as<uint8_t>());
-    rc = compact(nk, row, col, w, keep, s, &nk);
-    if (rc) return rc;
+    if (rm_self) {
+      hipLaunchKernelGGL(selfloop_flags_kernel, dim3(grid_for(nk)), dim3(kB), 0, s, nk,
+                         key.as<uint64_t>(), sorted, keep.as<int32_t>());
+      rc = compact(nk, key, w, keep, s, &nk);
+      if (rc) return rc;
+    }
+    if (rm_dup && nk > 0) {
+      hipLaunchKernelGGL(dup_flags_kernel, dim3(grid_for(nk)), dim3(kB), 0, s, nk,
+                         key.as<uint64_t>(), keep.as<int32_t>());
+      rc = compact(nk, key, w, keep, s, &nk);
+      if (rc) return rc;
+    }
   }
-  if (nk > 0 && rm_dup) {
-    DeviceBuffer keep;
-    rc = keep.alloc((size_t)nk);
-    if (rc) return rc;
-    hipLaunchKernelGGL(dup_flags_kernel, dim3(grid_for(nk)), dim3(kB), 0, s, nk,
-                       row.as<int32_t>(), col.as<int32_t>(), 1, keep.as<uint8_t>());
-    rc = compact(nk, row, col, w, keep, s, &nk);
-    if (rc) return rc;
-  }
-  // 5. offsets and degrees
-  DeviceBuffer rcount, off, deg;
-  rc = rcount.alloc(4 * ((size_t)m + 1));
-  if (!rc) rc = off.alloc(4 * ((size_t)m + 1));
+  // 5. column array, offsets and degrees
+  DeviceBuffer off, col, deg, wout;
+  rc = off.alloc(4 * ((size_t)m + 1));
+  if (!rc) rc = col.alloc(4 * (size_t)std::max<int64_t>(nk, 1));
   if (!rc) rc = deg.alloc(4 * (size_t)std::max(m, 1));
   if (rc) return rc;
-  GD_HIP_TRY(hipMemsetAsync(rcount.ptr, 0, 4 * ((size_t)m + 1), s));
   if (nk > 0)
-    hipLaunchKernelGGL(histogram_kernel, dim3(grid_for(nk)), dim3(kB), 0, s, nk,
-                       row.as<int32_t>(), rcount.as<int32_t>());
-  {
-    size_t tb = 0;
-    GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, rcount.as<int32_t>(),
-                                                off.as<int32_t>(), m + 1, s));
-    DeviceBuffer tmp;
-    rc = tmp.alloc(std::max<size_t>(tb, 1));
-    if (rc) return rc;
-    GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.ptr, tb, rcount.as<int32_t>(),
-                                                off.as<int32_t>(), m + 1, s));
-    if (m > 0)
-      hipLaunchKernelGGL(degree_kernel, dim3(grid_for(m)), dim3(kB), 0, s, m, off.as<int32_t>(),
-                         deg.as<int32_t>());
-    GD_HIP_TRY(hipGetLastError());
-    GD_HIP_TRY(hipStreamSynchronize(s));
-  }
-  // hand the buffers to the caller
+    hipLaunchKernelGGL(offsets_kernel, dim3(grid_for(nk)), dim3(kB), 0, s, nk, m,
+                       key.as<uint64_t>(), sorted, off.as<int32_t>(), col.as<int32_t>());
+  else
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for((int64_t)m + 1)), dim3(kB), 0, s,
+                       (int64_t)m + 1, 0, off.as<int32_t>());
+  if (m > 0)
+    hipLaunchKernelGGL(degree_kernel, dim3(grid_for(m)), dim3(kB), 0, s, m, off.as<int32_t>(),
+                       deg.as<int32_t>());
+  GD_HIP_TRY(hipGetLastError());
+  GD_HIP_TRY(hipStreamSynchronize(s));
+  // hand the buffers to the caller (w holds the kept weights, in order)
   out->m = m;
   out->nnz = nk;
   out->row_offsets = off.as<int32_t>();
