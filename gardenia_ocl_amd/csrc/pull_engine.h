@@ -388,17 +388,14 @@ __device__ __forceinline__ int exact_pos(int i) {
 // `head`: add the first kExactHead values serially first (the start of a row,
 // where s crosses a binade at almost every doubling).  All threads return the
 // same s.  Ends with the workgroup synchronised (val may be rewritten).
-// SERIAL: the whole chunk is added by one lane (sums whose running value keeps
-// changing binade, e.g. terms of both signs around a small sum, where the scan
-// would restart after almost every step).
-template <int IPT, bool SERIAL = false>
+template <int IPT>
 __device__ float exact_scan_chunk(const float* __restrict__ val, int nc, bool head, float s,
                                   ExactLds& x) {
   const int tid = threadIdx.x;
   const int lo = tid * IPT, hi = min(lo + IPT, nc);  // this thread's elements
   int start = 0;
-  if (head || SERIAL) {
-    const int ns = SERIAL ? nc : min(nc, kExactHead);
+  if (head) {
+    const int ns = min(nc, kExactHead);
     if (tid == 0) {
       // software-pipelined: the next batch's LDS loads are in flight while the
       // current batch is added (the adds are one dependent chain)
@@ -493,7 +490,7 @@ __device__ float exact_scan_chunk(const float* __restrict__ val, int nc, bool he
 // Sum of the gathered values of edges [e0, e0+ne) in edge order, rounded as the
 // sequential FP32 loop starting from s.  All threads return the same value.
 // `val` holds kBlock*IPT + kBlock floats.
-template <int IPT, bool NT, bool SORTED, class G, bool SERIAL = false>
+template <int IPT, bool NT, bool SORTED, class G>
 __device__ float exact_seq_sum(const int32_t* __restrict__ col, const uint16_t* __restrict__ pos,
                                int e_max, int e0, int ne, const G& g, float s,
                                float* __restrict__ val, ExactLds& x) {
@@ -518,7 +515,7 @@ __device__ float exact_seq_sum(const int32_t* __restrict__ col, const uint16_t* 
     for (int j = 0; j < IPT; j++)
       if (tid + j * kBlock < nc) val[exact_pos<IPT>(SORTED ? p[j] : tid + j * kBlock)] = v[j];
     __syncthreads();
-    s = exact_scan_chunk<IPT, SERIAL>(val, nc, c0 == 0, s, x);
+    s = exact_scan_chunk<IPT>(val, nc, c0 == 0, s, x);
   }
   return s;
 }

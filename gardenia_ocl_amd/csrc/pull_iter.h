@@ -39,17 +39,6 @@ struct IterState {
   double tol;
 };
 
-// P::kSerialExact (optional, default false): exact hub rows are added by one
-// lane instead of the binade scan (terms of both signs).
-template <class P, class = void>
-struct SerialExact {
-  static constexpr bool value = false;
-};
-template <class P>
-struct SerialExact<P, decltype((void)P::kSerialExact)> {
-  static constexpr bool value = P::kSerialExact;
-};
-
 struct IterArgs {
   const Tile* __restrict__ tiles;  // row tiles (+ hub segments of tree-summing policies)
   int n_tiles;
@@ -179,8 +168,8 @@ __global__ __launch_bounds__(kBlock) void exact_hub_kernel(const typename P::Arg
   const auto g = P::template gather<NT>(a, odd);
   const auto e = P::template epilogue<NT>(a, odd);
   const auto pre = e.pre(t.x);
-  const float sum = exact_seq_sum<IPT, (NT > 0), SORTED, decltype(g), SerialExact<P>::value>(
-      it.col, it.pos, it.e_max, t.z, t.w - t.z, g, e.init(pre), lds.val, xs);
+  const float sum = exact_seq_sum<IPT, (NT > 0), SORTED>(it.col, it.pos, it.e_max, t.z, t.w - t.z,
+                                                         g, e.init(pre), lds.val, xs);
   double err = 0.0;
   if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
   const double tot = block_sum<kWaves>(err, s_err);

@@ -18,8 +18,8 @@
 // range of plan rows and of edges, with diag[i] and orig[i] beside it.  Each
 // color is cut into the pull engine's LDS row tiles (rows of <= 64 off-diagonal
 // entries, summed by their owner thread in CSR order) and whole-row tiles
-// (longer rows: a workgroup gathers each 2048-entry chunk into LDS, one lane
-// adds it in order, exact_seq_sum's serial mode), with a SymGS gather
+// (longer rows: a workgroup gathers each 2048-entry chunk into LDS while one
+// lane adds the previous chunk in order), with a SymGS gather
 // (x[col] * val) and epilogue (the row update).  One color = one launch of
 // symgs_color_kernel; the stream orders the colors.
 
@@ -71,10 +71,6 @@ struct SymgsEpilogue {
 };
 
 struct SymgsPolicy {
-  static constexpr bool kExactHubs = true;
-  // off-diagonal products of both signs: the running sum changes binade too
-  // often for the scan, so long rows are added by one lane
-  static constexpr bool kSerialExact = true;
   struct Args {
     float* x;
     const float* b;
@@ -92,22 +88,105 @@ struct SymgsPolicy {
   }
 };
 
+constexpr int kSgChunk = kBlock * kSgIPT;  // entries of a long row per LDS chunk
+
+// s + v[0] + ... + v[n-1] in order with FP32 rounding, by the calling lane,
+// from 16-byte aligned LDS: 4*kQ values per step, the next step's LDS reads in
+// flight while the current step is added (the adds are one dependent chain).
+template <int kQ>  // float4 reads per step
+__device__ __forceinline__ float lane_seq_sum(float s, const float* __restrict__ v, int n) {
+  const float4* __restrict__ v4 = reinterpret_cast<const float4*>(v);
+  const int nb = n / (4 * kQ);
+  float4 A[kQ], B[kQ];
+  auto load = [&](float4(&R)[kQ], int blk) {
+#pragma unroll
+    for (int k = 0; k < kQ; k++) R[k] = v4[blk * kQ + k];
+  };
+  auto add = [&](const float4(&R)[kQ]) {
+#pragma unroll
+    for (int k = 0; k < kQ; k++) {
+      s += R[k].x;
+      s += R[k].y;
+      s += R[k].z;
+      s += R[k].w;
+    }
+  };
+  if (nb > 0) {
+    load(A, 0);
+    int b = 0;
+    for (; b + 2 <= nb; b += 2) {
+      load(B, b + 1);
+      add(A);
+      if (b + 2 < nb) load(A, b + 2);
+      add(B);
+    }
+    if (b < nb) add(A);
+  }
+  for (int i = nb * 4 * kQ; i < n; i++) s += v[i];
+  return s;
+}
+
+// One long row: s0 + its off-diagonal products in CSR order, rounded as the
+// reference's sequential loop.  The workgroup gathers chunk c+1 while lane 0
+// adds chunk c (two LDS buffers).  All threads return the sum.
+__device__ float long_row_sum(const int32_t* __restrict__ col, int e_max, int e0, int ne,
+                              const SymgsGather& g, float* __restrict__ buf0,
+                              float* __restrict__ buf1, float* __restrict__ s_out) {
+  const int tid = threadIdx.x;
+  const int nchunks = (ne + kSgChunk - 1) / kSgChunk;
+  float v[kSgIPT];
+  auto gather = [&](int c) {
+    const int c0 = c * kSgChunk, nc = min(kSgChunk, ne - c0);
+    int idx[kSgIPT];
+#pragma unroll
+    for (int j = 0; j < kSgIPT; j++) idx[j] = col[min(e0 + c0 + min(tid + j * kBlock, nc - 1), e_max)];
+#pragma unroll
+    for (int j = 0; j < kSgIPT; j++) v[j] = g.value(e0 + c0 + min(tid + j * kBlock, nc - 1), idx[j]);
+  };
+  auto store = [&](int c, float* buf) {
+    const int nc = min(kSgChunk, ne - c * kSgChunk);
+#pragma unroll
+    for (int j = 0; j < kSgIPT; j++)
+      if (tid + j * kBlock < nc) buf[tid + j * kBlock] = v[j];
+  };
+  gather(0);
+  store(0, buf0);
+  __syncthreads();
+  float s = 0.0f;
+  for (int c = 0; c < nchunks; c++) {
+    float* cur = (c & 1) ? buf1 : buf0;
+    float* nxt = (c & 1) ? buf0 : buf1;
+    const bool more = c + 1 < nchunks;
+    if (more) gather(c + 1);  // in flight during the serial adds
+    if (tid == 0) s = lane_seq_sum<8>(s, cur, min(kSgChunk, ne - c * kSgChunk));
+    if (more) store(c + 1, nxt);
+    __syncthreads();
+  }
+  if (tid == 0) *s_out = s;
+  __syncthreads();
+  return *s_out;
+}
+
 // One color in one launch: workgroups [0, n_hub_tiles) add one long row each
-// (a 2048-entry chunk at a time gathered into LDS, added by one lane in
-// order), the rest one LDS row tile each (owner thread per row, in order).
-// Long rows first: they take longest.
+// (long_row_sum), the rest one LDS row tile each (owner thread per row, in
+// order).  Long rows first: they take longest.  LONG = false (colors without
+// long rows) leaves the long-row code, and its registers, out of the kernel.
+template <bool LONG>
 __global__ __launch_bounds__(kBlock) void symgs_color_kernel(const SymgsPolicy::Args a,
                                                              const IterArgs it) {
-  __shared__ RowTileLds<kSgIPT> lds;
-  __shared__ ExactLds xs;
+  // a row tile's LDS, or a long row's two chunk buffers + the broadcast sum
+  static_assert(sizeof(RowTileLds<kSgIPT>) >= (2 * kSgChunk + 1) * sizeof(float), "LDS union");
+  __shared__ __align__(16) unsigned char smem[sizeof(RowTileLds<kSgIPT>)];
+  RowTileLds<kSgIPT>& lds = *reinterpret_cast<RowTileLds<kSgIPT>*>(smem);
+  float* const buf = reinterpret_cast<float*>(smem);
   const auto g = SymgsPolicy::gather<0>(a, 0);
   const auto e = SymgsPolicy::epilogue<0>(a, 0);
   const int b = blockIdx.x;
-  if (b < it.n_hub_tiles) {
+  if (LONG && b < it.n_hub_tiles) {
     const Tile t = it.hub_tiles[b];
     const auto pre = e.pre(t.x);
-    const float rsum = exact_seq_sum<kSgIPT, false, false, SymgsGather, true>(
-        it.col, nullptr, it.e_max, t.z, t.w - t.z, g, e.init(pre), lds.val, xs);
+    const float rsum = long_row_sum(it.col, it.e_max, t.z, t.w - t.z, g, buf, buf + kSgChunk,
+                                    buf + 2 * kSgChunk);
     if (threadIdx.x == 0) (void)e.finish(t.x, rsum, pre);
   } else {
     const Tile t = it.tiles[b - it.n_hub_tiles];
@@ -308,7 +387,6 @@ int plan_build(gd_symgs_plan* p, int m, int64_t nnz, const int32_t* d_Ap, const 
     const int r0 = co[c] - co[0], r1 = co[c + 1] - co[0];
     const int nr = r1 - r0;
     const int32_t e0 = h_off[r0];
-    int nt = 0, nh = 0;
     if (nr > 0) {
       reb.resize((size_t)nr + 1);
       for (int i = 0; i <= nr; i++) reb[i] = h_off[r0 + i] - e0;
@@ -316,16 +394,14 @@ int plan_build(gd_symgs_plan* p, int m, int64_t nnz, const int32_t* d_Ap, const 
       rc = build_schedule(nr, reb[nr], reb.data(), kBlock * p->ipt, kTileRows, &sc, kSeqRow);
       if (rc) return rc;
       p->max_row_len = std::max(p->max_row_len, (int)sc.max_row_len);
+      const size_t h0 = all_hubs.size();
       for (const Tile& t : sc.tiles) {
         const Tile u{t.x + r0, t.y >= 0 ? t.y + r0 : t.y, t.z + e0, t.w + e0};
-        if (t.y < 0) {
-          all_hubs.push_back(u);
-          nh++;
-        } else {
-          all_tiles.push_back(u);
-          nt++;
-        }
+        (t.y < 0 ? all_hubs : all_tiles).push_back(u);
       }
+      // the color lasts as long as its longest row: start the longest first
+      std::stable_sort(all_hubs.begin() + h0, all_hubs.end(),
+                       [](const Tile& x, const Tile& y) { return x.w - x.z > y.w - y.z; });
     }
     p->tile_begin.push_back((int)all_tiles.size());
     p->hub_begin.push_back((int)all_hubs.size());
@@ -359,8 +435,10 @@ int sweep_color(gd_symgs_plan* p, int c, const SymgsPolicy::Args& a, hipStream_t
                     nullptr,
                     nullptr,
                     nullptr};
-  if (nt + nh > 0)
-    hipLaunchKernelGGL(symgs_color_kernel, dim3(nt + nh), dim3(kBlock), 0, s, a, it);
+  if (nh > 0)
+    hipLaunchKernelGGL(symgs_color_kernel<true>, dim3(nt + nh), dim3(kBlock), 0, s, a, it);
+  else if (nt > 0)
+    hipLaunchKernelGGL(symgs_color_kernel<false>, dim3(nt), dim3(kBlock), 0, s, a, it);
   GD_HIP_TRY(hipGetLastError());
   return GD_OK;
 }
