@@ -56,6 +56,40 @@ def cpu_baseline(in_csr, out_deg, iters_per_pr, target_s):
     }
 
 
+def cpu_baseline_extras(spmv_log2, seed):
+    """The oracle on the secondary workloads: SpMV y += A x on the ER matrix
+    (OpenMP, src/spmv/omp_base.cc restated) and one SymGS sweep on the 128^3
+    stencil (sequential, src/symgs/verifier.cc restated)."""
+    import numpy as np
+
+    import oracle
+    from gardenia_ocl_amd import graph
+
+    out = {}
+    er = graph.er_matrix(spmv_log2, 32, seed=seed)
+    x = graph.uniform_pm1(er.m, seed + 1)
+    y = np.zeros(er.m, np.float32)
+    oracle.spmv(er.m, er.row_offsets, er.column_indices, er.weight, x, y)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        oracle.spmv(er.m, er.row_offsets, er.column_indices, er.weight, x, y)
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    out["spmv"] = {"value": round(ms, 3), "unit": "ms per y += A x", "cores": oracle.num_threads(),
+                   "kind": "port", "sample": f"ER 2^{spmv_log2} x 32, {reps} reps"}
+    del er, x, y
+    c = graph.stencil27(128)
+    colors, nc = graph.color_first_fit(c)
+    idx, co = graph.color_order(colors, nc)
+    b = graph.uniform_pm1(c.m, seed + 17)
+    t0 = time.perf_counter()
+    oracle.symgs(c.row_offsets, c.column_indices, idx, c.weight, np.zeros(c.m, np.float32), b, co)
+    ms = (time.perf_counter() - t0) * 1e3
+    out["symgs"] = {"value": round(ms, 1), "unit": "ms per symmetric sweep", "cores": 1,
+                    "kind": "port", "sample": "stencil27 128^3, 8 colors, one sweep"}
+    return out
+
+
 def bench_spmv(log2_rows, nnz_per_row, reps, seed):
     """ER 2^n x 2^n, nnz_per_row uniform columns, y += A x repeated `reps` times
     (HIP events around the reps on the plan's stream)."""
@@ -564,6 +598,9 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         cpu = cpu_baseline(in_csr, out_deg, args.iters, args.cpu_seconds)
+        if not args.no_extras:
+            del in_csr
+            cpu["secondary"] = cpu_baseline_extras(args.spmv_log2, args.seed)
     rec = {
         "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),

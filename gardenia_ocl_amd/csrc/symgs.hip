@@ -114,11 +114,17 @@ __device__ __forceinline__ float lane_seq_sum(float s, const float* __restrict__
   if (nb > 0) {
     load(A, 0);
     int b = 0;
+    // scheduling barriers keep each block's reads issued before the previous
+    // block's adds, so the adds wait only for reads issued one block earlier
     for (; b + 2 <= nb; b += 2) {
       load(B, b + 1);
+      __builtin_amdgcn_sched_barrier(0);
       add(A);
-      if (b + 2 < nb) load(A, b + 2);
+      __builtin_amdgcn_sched_barrier(0);
+      load(A, min(b + 2, nb - 1));  // unconditional (clamped): A and B keep their registers
+      __builtin_amdgcn_sched_barrier(0);
       add(B);
+      __builtin_amdgcn_sched_barrier(0);
     }
     if (b < nb) add(A);
   }
@@ -126,22 +132,28 @@ __device__ __forceinline__ float lane_seq_sum(float s, const float* __restrict__
   return s;
 }
 
-// One long row: s0 + its off-diagonal products in CSR order, rounded as the
-// reference's sequential loop.  The workgroup gathers chunk c+1 while lane 0
-// adds chunk c (two LDS buffers).  All threads return the sum.
+// One long row: its off-diagonal products summed in CSR order from 0,
+// rounded as the reference's sequential loop.  Lane 0 adds chunk c (2048
+// entries, LDS) while the workgroup's gathers of chunk c+1 and column loads of
+// chunk c+2 are in flight (two LDS buffers).  All threads return the sum.
 __device__ float long_row_sum(const int32_t* __restrict__ col, int e_max, int e0, int ne,
                               const SymgsGather& g, float* __restrict__ buf0,
                               float* __restrict__ buf1, float* __restrict__ s_out) {
   const int tid = threadIdx.x;
   const int nchunks = (ne + kSgChunk - 1) / kSgChunk;
-  float v[kSgIPT];
-  auto gather = [&](int c) {
+  auto edge = [&](int c, int j) {  // edge of lane tid, slot j, chunk c (clamped into the row)
     const int c0 = c * kSgChunk, nc = min(kSgChunk, ne - c0);
-    int idx[kSgIPT];
+    return e0 + c0 + min(tid + j * kBlock, nc - 1);
+  };
+  int idx[kSgIPT];
+  float v[kSgIPT];
+  auto load_idx = [&](int c) {
 #pragma unroll
-    for (int j = 0; j < kSgIPT; j++) idx[j] = col[min(e0 + c0 + min(tid + j * kBlock, nc - 1), e_max)];
+    for (int j = 0; j < kSgIPT; j++) idx[j] = col[min(edge(c, j), e_max)];
+  };
+  auto gather = [&](int c) {
 #pragma unroll
-    for (int j = 0; j < kSgIPT; j++) v[j] = g.value(e0 + c0 + min(tid + j * kBlock, nc - 1), idx[j]);
+    for (int j = 0; j < kSgIPT; j++) v[j] = g.value(edge(c, j), idx[j]);
   };
   auto store = [&](int c, float* buf) {
     const int nc = min(kSgChunk, ne - c * kSgChunk);
@@ -149,7 +161,9 @@ __device__ float long_row_sum(const int32_t* __restrict__ col, int e_max, int e0
     for (int j = 0; j < kSgIPT; j++)
       if (tid + j * kBlock < nc) buf[tid + j * kBlock] = v[j];
   };
+  load_idx(0);
   gather(0);
+  if (nchunks > 1) load_idx(1);
   store(0, buf0);
   __syncthreads();
   float s = 0.0f;
@@ -157,7 +171,8 @@ __device__ float long_row_sum(const int32_t* __restrict__ col, int e_max, int e0
     float* cur = (c & 1) ? buf1 : buf0;
     float* nxt = (c & 1) ? buf0 : buf1;
     const bool more = c + 1 < nchunks;
-    if (more) gather(c + 1);  // in flight during the serial adds
+    if (more) gather(c + 1);                  // indices arrived during chunk c-1's adds
+    if (c + 2 < nchunks) load_idx(c + 2);     // in flight during this chunk's adds
     if (tid == 0) s = lane_seq_sum<8>(s, cur, min(kSgChunk, ne - c * kSgChunk));
     if (more) store(c + 1, nxt);
     __syncthreads();
