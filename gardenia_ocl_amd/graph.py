@@ -164,6 +164,37 @@ def stencil27(n: int) -> Csr:
     return Csr(m, off, c.astype(np.int32), deg, v.astype(np.float32))
 
 
+def with_dominant_diagonal(csr: Csr, margin: float = 1.0) -> Csr:
+    """The matrix with every row's diagonal set to margin + the sum of the
+    row's off-diagonal |weights| (existing diagonal entries replaced, missing
+    ones inserted in column order): a strictly diagonally dominant SymGS input,
+    whose sweeps converge instead of overflowing."""
+    m = csr.m
+    rows = np.repeat(np.arange(m, dtype=np.int64), np.diff(csr.row_offsets))
+    col = csr.column_indices.astype(np.int64)
+    w = (csr.weight if csr.weight is not None else np.ones(len(col), np.float32)).astype(np.float32)
+    off_diag = col != rows
+    rows, col, w = rows[off_diag], col[off_diag], w[off_diag]
+    dval = (np.bincount(rows, weights=np.abs(w).astype(np.float64), minlength=m) + margin)
+    rows = np.concatenate([rows, np.arange(m, dtype=np.int64)])
+    col = np.concatenate([col, np.arange(m, dtype=np.int64)])
+    w = np.concatenate([w, dval.astype(np.float32)])
+    # stable: each row keeps its column order, the diagonal lands in place when
+    # the columns are sorted, after the row's entries otherwise
+    order = np.lexsort((col, rows)) if _sorted_rows(csr) else np.argsort(rows, kind="stable")
+    rows, col, w = rows[order], col[order], w[order]
+    deg = np.bincount(rows, minlength=m).astype(np.int32)
+    off = np.zeros(m + 1, np.int32)
+    np.cumsum(deg, out=off[1:])
+    return Csr(m, off, col.astype(np.int32), deg, w)
+
+
+def _sorted_rows(csr: Csr) -> bool:
+    rows = np.repeat(np.arange(csr.m, dtype=np.int64), np.diff(csr.row_offsets))
+    key = rows * csr.m + csr.column_indices
+    return bool(np.all(np.diff(key) >= 0))
+
+
 def color_first_fit(csr: Csr):
     """Deterministic first-fit coloring over the CSR and its transpose
     (gd_color_first_fit).  Returns (colors int32[m], n_colors)."""

@@ -92,6 +92,26 @@ def test_symgs_rmat_hubs_bit_exact(device, scale, seed):
         np.testing.assert_array_equal(_sweeps(device, p, x0, b, sweeps), want)
 
 
+def test_symgs_dominant_rmat_many_sweeps(device):
+    """Weights of both signs, a dominant diagonal: sweeps converge, every
+    sweep bit-exact (long rows included)."""
+    src, dst = graph.rmat_edges(15, 16, seed=8)
+    w = graph.uniform_pm1(len(src), 9)
+    c = graph.csr_from_edges(1 << 15, src, dst, w, symmetrize=True, transpose=False, sorted=True,
+                             remove_selfloops=False, remove_redundents=True)
+    c = graph.with_dominant_diagonal(c)
+    colors, nc = graph.color_first_fit(c)
+    idx, co = graph.color_order(colors, nc)
+    b = graph.uniform_pm1(c.m, 10)
+    x0 = np.zeros(c.m, np.float32)
+    p = _plan(device, c.row_offsets, c.column_indices, c.weight, idx, co)
+    assert p.info()["n_exact_rows"] > 0
+    want = oracle.symgs(c.row_offsets, c.column_indices, idx, c.weight, x0, b, co, 5)
+    got = _sweeps(device, p, x0, b, 5)
+    np.testing.assert_array_equal(got, want)
+    assert np.all(np.isfinite(got)) and np.abs(got).max() < 2.0
+
+
 def test_symgs_stencil_bit_exact_and_converges(device):
     c = graph.stencil27(32)
     colors, nc = graph.color_first_fit(c)

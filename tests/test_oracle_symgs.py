@@ -87,3 +87,28 @@ def test_first_fit_coloring_is_proper_and_ordered(sg):
         np.testing.assert_array_equal(idx, c["indices"])
         np.testing.assert_array_equal(co, c["color_offsets"])
         assert np.all(np.diff(colors[idx]) >= 0)
+
+
+def test_with_dominant_diagonal():
+    # the stencil is already dominant with a diagonal in place: same structure
+    c = graph.stencil27(5)
+    d = graph.with_dominant_diagonal(c, margin=1.0)
+    np.testing.assert_array_equal(d.row_offsets, c.row_offsets)
+    np.testing.assert_array_equal(d.column_indices, c.column_indices)
+    rows = np.repeat(np.arange(d.m), d.degree)
+    diag = d.column_indices == rows
+    off = np.bincount(rows[~diag], weights=np.abs(d.weight[~diag]).astype(np.float64), minlength=d.m)
+    np.testing.assert_allclose(d.weight[diag], off + 1.0, rtol=1e-6)
+    np.testing.assert_array_equal(d.weight[~diag], c.weight[c.column_indices != rows])
+    # rows without a diagonal get one, in column order; self-loops are replaced
+    src = np.array([0, 1, 2, 2, 3], np.int32)
+    dst = np.array([1, 2, 2, 0, 1], np.int32)
+    w = np.array([0.5, -2.0, 7.0, 1.5, -0.25], np.float32)
+    e = graph.csr_from_edges(4, src, dst, w, symmetrize=True, sorted=True, remove_selfloops=False)
+    f = graph.with_dominant_diagonal(e, margin=0.5)
+    rows = np.repeat(np.arange(4), f.degree)
+    key = rows * 4 + f.column_indices
+    assert np.all(np.diff(key) > 0)
+    assert np.all(np.bincount(rows[f.column_indices == rows], minlength=4) == 1)
+    # row 2: |-2| (from 1) + |1.5| (to 0) + 0.5; the self-loop 7.0 is gone
+    assert f.weight[(rows == 2) & (f.column_indices == 2)][0] == np.float32(4.0)
