@@ -185,6 +185,42 @@ def bench_tol_run(plan, off, col, deg, tol, max_iter):
             "max_rel_vs_fp64": rel, "fp64_iterations": it64}
 
 
+def bench_small(scale, tol, max_iter, seed, reps=7):
+    """A whole solve on a small graph (RMAT-16, tol 1e-4, max 100: the
+    reference's CPU-runnable case) with the plan reused, scores reset between
+    solves; median over reps of the HIP-event solve time and of the wall time
+    of the call."""
+    import numpy as np
+    import torch
+
+    from gardenia_ocl_amd import PageRankPlan, graph
+
+    dev = torch.device("cuda:0")
+    _, ic = graph.rmat(scale, 16, seed=seed, want_out=False)
+    V, E = ic.m, ic.nnz
+    od = np.bincount(ic.column_indices, minlength=V).astype(np.int32)
+    off = torch.from_numpy(ic.row_offsets).to(dev)
+    col = torch.from_numpy(ic.column_indices).to(dev)
+    deg = torch.from_numpy(od).to(dev)
+    plan = PageRankPlan(off, col, deg)
+    s = torch.empty(V, dtype=torch.float32, device=dev)
+    init = float(np.float32(1.0) / np.float32(V))
+    ev, wall = [], []
+    for _ in range(reps):
+        s.fill_(init)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st, _ = plan.run(s, tol=tol, max_iter=max_iter)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        ev.append(st["solve_ms"])
+    its = st["iterations"]
+    ms = float(np.median(ev))
+    return {"workload": f"RMAT-{scale} ({E} edges) pull PageRank to L1 tol {tol}, max {max_iter}",
+            "iterations": its, "solve_ms": round(ms, 4), "wall_ms": round(float(np.median(wall)), 4),
+            "us_per_iteration": round(ms / max(its, 1) * 1e3, 2),
+            "gteps": round(E * its / (ms * 1e-3) / 1e9, 2)}
+
+
 def bench_ppr(scale, k, tol, seed, fp64_seeds):
     """Batched PPR (k seeds in one SpMM-style pull) vs k looped single-seed
     runs of the plain pull kernel, both to the same per-seed L1 tolerance."""
@@ -476,7 +512,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (scales rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
-    ap.add_argument("--only", choices=["symgs", "csr_build", "spmv"], default=None,
+    ap.add_argument("--only", choices=["symgs", "csr_build", "spmv", "small"], default=None,
                     help="run just this secondary figure and print its JSON (for profiling)")
     ap.add_argument("--tol-run", type=float, default=1e-6,
                     help="L1 tolerance of the extra run to convergence (with its FP64 L1)")
@@ -495,6 +531,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.only == "symgs":
         print(json.dumps({"symgs": bench_symgs(128, 20, args.seed)}), flush=True)
+        return
+    if args.only == "small":
+        print(json.dumps({"small": bench_small(16, 1e-4, 100, args.seed)}), flush=True)
         return
     if args.only == "spmv":
         print(json.dumps({"spmv": bench_spmv(args.spmv_log2, 32, 50, args.seed)}), flush=True)
@@ -573,6 +612,8 @@ def main():
         extras["tol_run"] = bench_tol_run(plan, off, col, deg, args.tol_run, 200)
         del plan, off, col, deg, scores
         torch.cuda.empty_cache()
+        log("small graph ...")
+        extras["small"] = bench_small(16, 1e-4, 100, args.seed)
         log("spmv ...")
         extras["spmv"] = bench_spmv(args.spmv_log2, 32, 50, args.seed)
         torch.cuda.empty_cache()
