@@ -41,8 +41,32 @@ struct IterEngine {
   // exact hubs run on a second stream, forked from / joined to `stream`
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // A chunk of iterations (identical launches) captured once into a hipGraph
+  // and replayed: one graph launch per chunk instead of 3-4 launches and a
+  // fork/join per iteration.  Keyed by the launch arguments and the chunk
+  // length (a run has at most two lengths: the chunk and the tail).
+  struct ChunkGraph {
+    std::vector<unsigned char> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<ChunkGraph> graphs;
+  bool use_graphs = true;  // cleared for good if a capture ever fails
+  // Capture runs on a private stream (the caller's may be the legacy default
+  // stream, which cannot be captured); the graph is launched on the caller's.
+  hipStream_t cap_stream = nullptr;
+
+  void drop_graphs() {
+    for (ChunkGraph& g : graphs) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    graphs.clear();
+  }
 
   ~IterEngine() {
+    drop_graphs();
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (join_ev) (void)hipEventDestroy(join_ev);
@@ -201,6 +225,52 @@ struct IterEngine {
   // `make_args(odd)` is not needed: the kernels pick buffers by the device
   // iteration parity.  Fills iterations / converged / final_error / pull
   // timing / near_tie of *s (solve_ms is the caller's).
+  // The executable graph of n iterations with these arguments, captured on
+  // first use; nullptr if capture is unavailable (then launches are direct).
+  template <class P>
+  hipGraphExec_t chunk_graph(const typename P::Args& a, const IterArgs& it, const FinArgs& f,
+                             int n) {
+    if (!use_graphs) return nullptr;
+    std::vector<unsigned char> key(sizeof(a) + sizeof(it) + sizeof(f) + sizeof(n));
+    unsigned char* k = key.data();
+    memcpy(k, &a, sizeof(a));
+    memcpy(k + sizeof(a), &it, sizeof(it));
+    memcpy(k + sizeof(a) + sizeof(it), &f, sizeof(f));
+    memcpy(k + sizeof(a) + sizeof(it) + sizeof(f), &n, sizeof(n));
+    for (ChunkGraph& g : graphs)
+      if (g.key == key) return g.exec;
+    if (graphs.size() >= 4) drop_graphs();  // arguments changed (e.g. another score array)
+    ChunkGraph g;
+    g.key = std::move(key);
+    if (!cap_stream && hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
+      use_graphs = false;
+      return nullptr;
+    }
+    if (hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipGetLastError();
+      use_graphs = false;
+      return nullptr;
+    }
+    const hipStream_t user = stream;
+    stream = cap_stream;  // launch_pull forks from / joins to `stream`
+    for (int i = 0; i < n; i++) {
+      launch_pull<P>(a, it);
+      hipLaunchKernelGGL(finalize_kernel<P>, dim3(fin_blocks), dim3(kBlock), 0, stream, a, it, f);
+    }
+    stream = user;
+    hipError_t e = hipStreamEndCapture(cap_stream, &g.graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      if (g.graph) (void)hipGraphDestroy(g.graph);
+      use_graphs = false;
+      return nullptr;
+    }
+    graphs.push_back(std::move(g));
+    return graphs.back().exec;
+  }
+
   template <class P>
   int run(const typename P::Args& a, int max_iter, double tol, int chunk_opt, bool profile,
           gd_pr_stats* s, double* errors_out) {
@@ -227,12 +297,17 @@ struct IterEngine {
     int pull_n = 0;
     while (launched < max_iter) {
       const int n = std::min(chunk, max_iter - launched);
-      for (int k = 0; k < n; k++) {
-        if (profile) GD_HIP_TRY(hipEventRecord(ev[2 * k], stream));
-        launch_pull<P>(a, it);
-        if (profile) GD_HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
-        hipLaunchKernelGGL(finalize_kernel<P>, dim3(fin_blocks), dim3(kBlock), 0, stream, a, it,
-                           f);
+      hipGraphExec_t exec = profile ? nullptr : chunk_graph<P>(a, it, f, n);
+      if (exec) {
+        GD_HIP_TRY(hipGraphLaunch(exec, stream));
+      } else {
+        for (int k = 0; k < n; k++) {
+          if (profile) GD_HIP_TRY(hipEventRecord(ev[2 * k], stream));
+          launch_pull<P>(a, it);
+          if (profile) GD_HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
+          hipLaunchKernelGGL(finalize_kernel<P>, dim3(fin_blocks), dim3(kBlock), 0, stream, a,
+                             it, f);
+        }
       }
       GD_HIP_TRY(hipGetLastError());
       launched += n;

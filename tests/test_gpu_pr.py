@@ -145,6 +145,46 @@ def test_pr_deterministic_and_chunk_invariant(device):
     assert sa["iterations"] == sb["iterations"] == sc["iterations"]
 
 
+@pytest.mark.parametrize("relabel", [False, True])
+def test_pr_graph_replays_match_direct_launches(device, relabel):
+    """Chunks of iterations are replayed from captured hipGraphs (two chunk
+    lengths here: 8 and a tail of 5); the profiled run launches directly.
+    Every replay, with eager work and synchronisations between, gives the
+    direct launches' scores, L1 values and iteration count, bit for bit."""
+    import torch
+
+    out_csr, in_csr = graph.rmat(14, 16, seed=7)
+    m = in_csr.m
+    off = torch.from_numpy(in_csr.row_offsets).to(device)
+    col = torch.from_numpy(in_csr.column_indices).to(device)
+    deg = torch.from_numpy(out_csr.degree).to(device)
+    plan = PageRankPlan(off, col, deg, relabel=relabel)
+    init = float(np.float32(1.0) / np.float32(m))
+    results = []
+    for rep in range(4):
+        s = torch.full((m,), init, dtype=torch.float32, device=device)
+        junk = torch.randn(1 << 16, device=device).sum()  # eager work between replays
+        torch.cuda.synchronize()
+        st, errs = plan.run(s, tol=0.0, max_iter=21, chunk=8, profile=(rep == 0))
+        results.append((s.cpu().numpy(), errs, st["iterations"]))
+        assert float(junk) == float(junk)
+    for r in results[1:]:
+        np.testing.assert_array_equal(r[0], results[0][0])
+        np.testing.assert_array_equal(r[1], results[0][1])
+        assert r[2] == results[0][2] == 21
+    # converging runs (tol > 0): the replayed chunks after convergence return at once
+    for rep in range(3):
+        s = torch.full((m,), init, dtype=torch.float32, device=device)
+        st, errs = plan.run(s, tol=1e-4, max_iter=100, profile=(rep == 0))
+        if rep == 0:
+            ref = (s.cpu().numpy(), errs, st["iterations"])
+        else:
+            np.testing.assert_array_equal(s.cpu().numpy(), ref[0])
+            np.testing.assert_array_equal(errs, ref[1])
+            assert st["iterations"] == ref[2]
+    plan.close()
+
+
 def test_pr_fixed_iterations_not_converged(device):
     out_csr, in_csr = graph.rmat(14, 16, seed=2)
     s, stats, errs, _ = _gpu_pr(device, in_csr, out_csr.degree, 0.0, 20)
