@@ -297,21 +297,28 @@ __device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
 //   While the running sum s lies in one binade, 2^k <= |s| < 2^(k+1) with
 //   ulp u = 2^(k-23), and the exact s + v[i] stays strictly inside it, the
 //   rounded step is s + q_i*u with q_i = v[i]/u rounded to the nearest
-//   integer: s advances by an integer number of ulps.  The workgroup scans the
-//   q_i (32-bit integers) and advances s over every step up to the
-//   first one that leaves the binade, rounds a tie (fraction exactly 1/2,
-//   whose rounding depends on the running parity) or starts from a subnormal /
-//   non-finite s; that step is one ordinary FP32 addition, and the scan
-//   restarts after it.  From s == +0 the sum jumps to the first nonzero term.  For sums of positive terms (PageRank) the
-//   restarts are the ~log2(n) binade crossings plus rare ties.
+//   integer: s advances by an integer number of ulps.  A positive term whose
+//   fraction is exactly 1/2 (a tie) rounds s to even: that step is the map
+//   x -> (x + floor(q)) rounded up to even, and the workgroup scans the steps
+//   as maps (TieMap: additions and round-to-even steps compose into maps of
+//   the same form), so runs of equal terms that tie at every step -- many
+//   sources with the same score and degree -- do not stop the scan.  s
+//   advances over every step up to the first one that leaves the binade, is a
+//   negative tie or inf / NaN, or starts from a subnormal / non-finite s;
+//   that step is one ordinary FP32 addition, and the scan restarts after it.
+//   From s == +0 the sum jumps to the first nonzero term.  For sums of
+//   positive terms (PageRank) the restarts are the ~log2(n) binade crossings.
 // ---------------------------------------------------------------------------
 
 constexpr int kExactHead = 256;  // terms of a row added serially before the scan
 
 struct ExactLds {
   int32_t wsum[kWaves];
+  int32_t wtie[kWaves];
+  int32_t wb[kWaves];
   int wmin[kWaves];
   float s;
+  int tie;  // the step that stopped the scan was a positive tie
 };
 
 // Exclusive prefix over the workgroup (lanes, then waves) and the total, in
@@ -339,6 +346,67 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t& total, Ex
   return (int32_t)(before + (inc - (uint32_t)v));
 }
 
+// A run of steps in integer ulps as one map of the running value x:
+//   t == 0:  x -> x + a
+//   t == 1:  x -> y + (y & 1) + b,  y = x + a
+// A regular step q is (0, q, 0).  A positive term that is exactly half an ulp
+// above an integer f rounds to even: x -> (x + f) rounded up to even, the map
+// (1, f, 0).  After such a step the value is even, so what follows it only
+// depends on constants: maps compose into maps of the same form, and a scan
+// over them gives every prefix exactly (wrapping 32-bit arithmetic keeps the
+// parity).  Negative ties stay irregular.
+struct TieMap {
+  uint32_t t, a, b;
+};
+
+__device__ __forceinline__ TieMap tie_compose(const TieMap& p, const TieMap& q) {  // p, then q
+  if (!p.t) return TieMap{q.t, p.a + q.a, q.b};
+  if (!q.t) return TieMap{1u, p.a, p.b + q.a};
+  const uint32_t c = p.b + q.a;
+  return TieMap{1u, p.a, c + (c & 1u) + q.b};
+}
+
+__device__ __forceinline__ uint32_t tie_apply(const TieMap& m, uint32_t x) {
+  const uint32_t y = x + m.a;
+  return m.t ? y + (y & 1u) + m.b : y;
+}
+
+// Exclusive prefix (lanes, then waves, in order) of the maps and their total.
+__device__ __forceinline__ TieMap block_excl_scan_map(TieMap v, TieMap& total, ExactLds& x) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  TieMap inc = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    TieMap left;
+    left.t = (uint32_t)__shfl_up((int)inc.t, o, kWave);
+    left.a = (uint32_t)__shfl_up((int)inc.a, o, kWave);
+    left.b = (uint32_t)__shfl_up((int)inc.b, o, kWave);
+    if (lane >= o) inc = tie_compose(left, inc);
+  }
+  // exclusive within the wave: the inclusive value of the previous lane
+  TieMap ex;
+  ex.t = (uint32_t)__shfl_up((int)inc.t, 1, kWave);
+  ex.a = (uint32_t)__shfl_up((int)inc.a, 1, kWave);
+  ex.b = (uint32_t)__shfl_up((int)inc.b, 1, kWave);
+  if (lane == 0) ex = TieMap{0u, 0u, 0u};
+  if (lane == kWave - 1) {
+    x.wtie[wid] = (int32_t)inc.t;
+    x.wsum[wid] = (int32_t)inc.a;
+    x.wb[wid] = (int32_t)inc.b;
+  }
+  __syncthreads();
+  TieMap before{0u, 0u, 0u}, tot{0u, 0u, 0u};
+#pragma unroll
+  for (int w = 0; w < kWaves; w++) {
+    const TieMap wm{(uint32_t)x.wtie[w], (uint32_t)x.wsum[w], (uint32_t)x.wb[w]};
+    if (w < wid) before = tie_compose(before, wm);
+    tot = tie_compose(tot, wm);
+  }
+  __syncthreads();
+  total = tot;
+  return tie_compose(before, ex);
+}
+
 __device__ __forceinline__ int block_min(int v, ExactLds& x) {
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
 #pragma unroll
@@ -353,27 +421,30 @@ __device__ __forceinline__ int block_min(int v, ExactLds& x) {
 }
 
 // v in units of u = 2^(ex-23), rounded to the nearest integer, from the bits of
-// v (|q| clamped to 2^26: such a step leaves the binade anyway).  tie: the
-// fraction is exactly 1/2 (the rounding then depends on the running sum's
-// parity); odd: v is inf or NaN.  Both make the step irregular.
-__device__ __forceinline__ int32_t ulps_of(float v, int ex, bool& irregular) {
+// v (|q| clamped to 2^26: such a step leaves the binade anyway).  tie: v is
+// positive and its fraction is exactly 1/2 (the result holds the floor; the
+// step rounds to even, map (1, q, 0)).  irregular: a negative tie, or v is inf
+// or NaN.
+__device__ __forceinline__ int32_t ulps_of_tie(float v, int ex, bool& tie, bool& irregular) {
   const uint32_t b = __float_as_uint(v);
   const int eb = (int)((b >> 23) & 0xffu);
   const uint32_t mv = eb ? ((b & 0x7fffffu) | 0x800000u) : (b & 0x7fffffu);  // v = mv*2^(ev-23)
   const int d = ex - (eb ? eb - 127 : -126);                                  // q = mv / 2^d
   int32_t q;
-  bool tie = false;
+  bool half = false;
   if (d <= 0) {
     q = (d <= -3) ? (1 << 26) : (int32_t)min((uint32_t)1 << 26, mv << (-d));
   } else if (d >= 26) {
     q = 0;
   } else {
-    const uint32_t half = 1u << (d - 1), rem = mv & ((1u << d) - 1u);
-    tie = (rem == half);
-    q = (int32_t)((mv >> d) + (rem > half ? 1u : 0u));
+    const uint32_t hf = 1u << (d - 1), rem = mv & ((1u << d) - 1u);
+    half = (rem == hf);
+    q = (int32_t)((mv >> d) + (rem > hf ? 1u : 0u));
   }
-  irregular = tie || eb == 0xff;
-  return (b >> 31) ? -q : q;
+  const bool neg = (b >> 31) != 0;
+  tie = half && !neg;
+  irregular = (half && neg) || eb == 0xff;
+  return neg ? -q : q;
 }
 
 // Position of chunk element i in the padded LDS buffer: thread t reads its IPT
@@ -388,9 +459,12 @@ __device__ __forceinline__ int exact_pos(int i) {
 // `head`: add the first kExactHead values serially first (the start of a row,
 // where s crosses a binade at almost every doubling).  All threads return the
 // same s.  Ends with the workgroup synchronised (val may be rewritten).
+// tie_mode (uniform, kept across the chunks of a row): once a positive tie has
+// stopped the scan, the row is scanned as maps (TieMap), which go through
+// ties; before that, the plain integer scan (cheaper) treats a tie as a stop.
 template <int IPT>
 __device__ float exact_scan_chunk(const float* __restrict__ val, int nc, bool head, float s,
-                                  ExactLds& x) {
+                                  ExactLds& x, bool& tie_mode) {
   const int tid = threadIdx.x;
   const int lo = tid * IPT, hi = min(lo + IPT, nc);  // this thread's elements
   int start = 0;
@@ -453,19 +527,37 @@ __device__ float exact_scan_chunk(const float* __restrict__ val, int nc, bool he
     // s = P * 2^(ex-23) with 2^23 <= |P| < 2^24
     const int32_t P = (int32_t)(__float_as_uint(s) & 0x7fffffu) | 0x800000;
     const int32_t Ps = signbit(s) ? -P : P;
-    uint32_t L = 0;
-    for (int i = max(lo, start); i < hi; i++) {
-      bool irr;
-      L += (uint32_t)ulps_of(val[exact_pos<IPT>(i)], ex, irr);
+    int32_t run, end;  // the value before this thread's first step; after the last step
+    if (tie_mode) {
+      TieMap L{0u, 0u, 0u};
+      for (int i = max(lo, start); i < hi; i++) {
+        bool tie, irr;
+        const uint32_t q = (uint32_t)ulps_of_tie(val[exact_pos<IPT>(i)], ex, tie, irr);
+        L = tie_compose(L, TieMap{tie ? 1u : 0u, q, 0u});
+      }
+      TieMap total;
+      run = (int32_t)tie_apply(block_excl_scan_map(L, total, x), (uint32_t)Ps);
+      end = (int32_t)tie_apply(total, (uint32_t)Ps);
+    } else {
+      uint32_t L = 0;
+      for (int i = max(lo, start); i < hi; i++) {
+        bool tie, irr;
+        L += (uint32_t)ulps_of_tie(val[exact_pos<IPT>(i)], ex, tie, irr);
+      }
+      int32_t total;
+      run = (int32_t)((uint32_t)Ps + (uint32_t)block_excl_scan((int32_t)L, total, x));
+      end = (int32_t)((uint32_t)Ps + (uint32_t)total);
     }
-    int32_t total;
-    int32_t run = (int32_t)((uint32_t)Ps + (uint32_t)block_excl_scan((int32_t)L, total, x));
     int bad = 0x7fffffff;
+    bool bad_tie = false;
     for (int i = max(lo, start); i < hi; i++) {
-      bool irr;
-      const int32_t r = (int32_t)((uint32_t)run + (uint32_t)ulps_of(val[exact_pos<IPT>(i)], ex, irr));
-      if (irr || !(abs(r) > (1 << 23) && abs(r) < (1 << 24))) {
+      bool tie, irr;
+      const uint32_t q = (uint32_t)ulps_of_tie(val[exact_pos<IPT>(i)], ex, tie, irr);
+      const uint32_t y = (uint32_t)run + q;
+      const int32_t r = (int32_t)(tie ? y + (y & 1u) : y);
+      if (irr || (tie && !tie_mode) || !(abs(r) > (1 << 23) && abs(r) < (1 << 24))) {
         bad = i;
+        bad_tie = tie && !tie_mode;
         break;
       }
       run = r;
@@ -473,12 +565,16 @@ __device__ float exact_scan_chunk(const float* __restrict__ val, int nc, bool he
     const int jb = block_min(bad, x);
     if (jb == 0x7fffffff) {
       // exact: an integer below 2^24 in magnitude times a power of two
-      s = ldexpf((float)(int32_t)((uint32_t)Ps + (uint32_t)total), ex - 23);
+      s = ldexpf((float)end, ex - 23);
       start = nc;
     } else {
-      if (bad == jb) x.s = ldexpf((float)run, ex - 23) + val[exact_pos<IPT>(jb)];  // FP32 step
+      if (bad == jb) {
+        x.s = ldexpf((float)run, ex - 23) + val[exact_pos<IPT>(jb)];  // FP32 step
+        x.tie = bad_tie ? 1 : 0;
+      }
       __syncthreads();
       s = x.s;
+      tie_mode = tie_mode || x.tie != 0;
       __syncthreads();
       start = jb + 1;
     }
@@ -496,6 +592,7 @@ __device__ float exact_seq_sum(const int32_t* __restrict__ col, const uint16_t* 
                                float* __restrict__ val, ExactLds& x) {
   constexpr int C = kBlock * IPT;
   const int tid = threadIdx.x;
+  bool tie_mode = false;
   for (int c0 = 0; c0 < ne; c0 += C) {
     const int nc = min(C, ne - c0);
     int idx[IPT], p[IPT];
@@ -515,7 +612,7 @@ __device__ float exact_seq_sum(const int32_t* __restrict__ col, const uint16_t* 
     for (int j = 0; j < IPT; j++)
       if (tid + j * kBlock < nc) val[exact_pos<IPT>(SORTED ? p[j] : tid + j * kBlock)] = v[j];
     __syncthreads();
-    s = exact_scan_chunk<IPT>(val, nc, c0 == 0, s, x);
+    s = exact_scan_chunk<IPT>(val, nc, c0 == 0, s, x, tie_mode);
   }
   return s;
 }

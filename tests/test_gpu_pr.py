@@ -238,6 +238,37 @@ def test_pr_star_hub_boundaries(device, n_leaves):
     assert info["hubs"] == (1 if n_leaves > exact_len(info) else 0)
 
 
+@pytest.mark.parametrize("mix", ["ties", "ties_and_crossings"])
+def test_pr_hub_row_of_ties_bit_exact(device, mix):
+    """A hub of 20,000 in-edges whose addends are mostly exact ties (odd
+    multiples of half an ulp of the running sum): each rounds to even, so the
+    sum depends on every step's parity.  The exact hub path resolves them in
+    its scan; the first iteration must equal the reference's sequential sum."""
+    rng = np.random.default_rng(11 if mix == "ties" else 12)
+    n = 20000
+    m = n + 1
+    src = np.arange(1, m, dtype=np.int32)
+    dst = np.zeros(n, np.int32)
+    out_csr = graph.csr_from_edges(m, src, dst)
+    in_csr = graph.csr_from_edges(m, src, dst, transpose=True)
+    u = np.float32(2.0 ** -23)  # ulp of [1, 2)
+    init = np.zeros(m, np.float32)
+    init[0] = 1.0
+    init[1] = 1.0  # the first addend puts the sum in [1, 2)
+    k = rng.integers(0, 4, n - 1).astype(np.float32)
+    init[2:] = (k + np.float32(0.5)) * u
+    if mix == "ties_and_crossings":
+        big = rng.random(n - 1) < 0.01  # push the sum across binades now and then
+        init[2:][big] = np.float32(2.0 ** -6)
+        regular = rng.random(n - 1) < 0.2
+        init[2:][regular] = (k[regular] + np.float32(0.25)) * u
+    s, stats, _, info = _gpu_pr(device, in_csr, out_csr.degree, 0.0, 1, init=init)
+    ref, it, _ = oracle.pr(m, in_csr.row_offsets, in_csr.column_indices, out_csr.degree,
+                           init, tol=0.0, max_iter=1)
+    assert info["hubs"] == 1
+    np.testing.assert_array_equal(s, ref)
+
+
 def test_pr_empty_and_edgeless(device):
     import torch
 
