@@ -41,16 +41,19 @@ struct IterEngine {
   // exact hubs run on a second stream, forked from / joined to `stream`
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
-  // A chunk of iterations (identical launches) captured once into a hipGraph
-  // and replayed: one graph launch per chunk instead of 3-4 launches and a
-  // fork/join per iteration.  Keyed by the launch arguments and the chunk
-  // length (a run has at most two lengths: the chunk and the tail).
+  // A chunk of iterations (identical launches) captured into a hipGraph once
+  // the same arguments come back in a later run, then replayed: one graph launch per chunk
+  // instead of 3-4 launches and a fork/join per iteration.  Keyed by the launch
+  // arguments and the chunk length (a run has at most two lengths: the chunk
+  // and the tail).
   struct ChunkGraph {
     std::vector<unsigned char> key;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   };
   std::vector<ChunkGraph> graphs;
+  std::vector<std::vector<unsigned char>> seen;      // keys of earlier runs
+  std::vector<std::vector<unsigned char>> this_run;  // keys of the current run
   bool use_graphs = true;  // cleared for good if a capture ever fails
   // Capture runs on a private stream (the caller's may be the legacy default
   // stream, which cannot be captured); the graph is launched on the caller's.
@@ -239,7 +242,18 @@ struct IterEngine {
     memcpy(k + sizeof(a) + sizeof(it) + sizeof(f), &n, sizeof(n));
     for (ChunkGraph& g : graphs)
       if (g.key == key) return g.exec;
-    if (graphs.size() >= 4) drop_graphs();  // arguments changed (e.g. another score array)
+    // capture once the arguments come back in a later run: arguments that
+    // change every run (a personalised seed, another score array) keep
+    // launching directly instead of paying a capture per run
+    bool again = false;
+    for (const auto& k2 : seen) again = again || k2 == key;
+    if (!again) {
+      bool noted = false;
+      for (const auto& k2 : this_run) noted = noted || k2 == key;
+      if (!noted) this_run.push_back(key);
+      return nullptr;
+    }
+    if (graphs.size() >= 4) drop_graphs();
     ChunkGraph g;
     g.key = std::move(key);
     if (!cap_stream && hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking) != hipSuccess) {
@@ -289,6 +303,11 @@ struct IterEngine {
     }
     const IterArgs it = iter_args();
     const FinArgs f = fin_args();
+    for (auto& k2 : this_run) {  // keys of the previous run become capturable
+      if (seen.size() >= 8) seen.erase(seen.begin());
+      seen.push_back(std::move(k2));
+    }
+    this_run.clear();
     hipLaunchKernelGGL(iter_state_init_kernel, dim3(1), dim3(1), 0, stream, it.st, max_iter, tol);
     GD_HIP_TRY(hipGetLastError());
     int launched = 0;
