@@ -41,8 +41,9 @@ struct IterEngine {
   // exact hubs run on a second stream, forked from / joined to `stream`
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
-  // A chunk of iterations (identical launches) captured into a hipGraph once
-  // the same arguments come back in a later run, then replayed: one graph launch per chunk
+  // A chunk of iterations (identical launches) captured into a hipGraph when it
+  // will run again (later in the run, or in a later run with the same
+  // arguments), then replayed: one graph launch per chunk
   // instead of 3-4 launches and a fork/join per iteration.  Keyed by the launch
   // arguments and the chunk length (a run has at most two lengths: the chunk
   // and the tail).
@@ -232,7 +233,7 @@ struct IterEngine {
   // first use; nullptr if capture is unavailable (then launches are direct).
   template <class P>
   hipGraphExec_t chunk_graph(const typename P::Args& a, const IterArgs& it, const FinArgs& f,
-                             int n) {
+                             int n, bool repeats) {
     if (!use_graphs) return nullptr;
     std::vector<unsigned char> key(sizeof(a) + sizeof(it) + sizeof(f) + sizeof(n));
     unsigned char* k = key.data();
@@ -242,10 +243,10 @@ struct IterEngine {
     memcpy(k + sizeof(a) + sizeof(it) + sizeof(f), &n, sizeof(n));
     for (ChunkGraph& g : graphs)
       if (g.key == key) return g.exec;
-    // capture once the arguments come back in a later run: arguments that
-    // change every run (a personalised seed, another score array) keep
-    // launching directly instead of paying a capture per run
-    bool again = false;
+    // capture when the chunk will be launched again in this run, or once the
+    // arguments come back in a later run: single-chunk runs whose arguments
+    // change every time launch directly instead of paying a capture per run
+    bool again = repeats;
     for (const auto& k2 : seen) again = again || k2 == key;
     if (!again) {
       bool noted = false;
@@ -316,7 +317,8 @@ struct IterEngine {
     int pull_n = 0;
     while (launched < max_iter) {
       const int n = std::min(chunk, max_iter - launched);
-      hipGraphExec_t exec = profile ? nullptr : chunk_graph<P>(a, it, f, n);
+      hipGraphExec_t exec =
+          profile ? nullptr : chunk_graph<P>(a, it, f, n, max_iter - launched >= 2 * n);
       if (exec) {
         GD_HIP_TRY(hipGraphLaunch(exec, stream));
       } else {
