@@ -191,7 +191,9 @@ __device__ __forceinline__ void issue_tile_loads(const Tile& t, const int32_t* _
 
 // Phases C-E of a row tile whose gathered values are already in lds.val and
 // rebased offsets in lds.off (after a barrier): reduce rows, epilogue.
-template <int IPT, class E>
+// SEQ_ALL: every row, whatever its length, is summed by its owner thread in
+// CSR order (no wavefront tree): bit-identical sums for any row the tile holds.
+template <int IPT, class E, bool SEQ_ALL = false>
 __device__ __forceinline__ double reduce_row_tile(int r0, int nr, const E& e,
                                                   const typename E::Pre (&pre)[kRowsPerThread],
                                                   RowTileLds<IPT>& lds) {
@@ -204,13 +206,13 @@ __device__ __forceinline__ double reduce_row_tile(int r0, int nr, const E& e,
     sum[q] = 0.0f;
     if (i < nr) {
       const int b = lds.off[i], en = lds.off[i + 1];
-      if (en - b <= kSeqRow)
+      if (SEQ_ALL || en - b <= kSeqRow)
         sum[q] = seq_sum(e.init(pre[q]), lds.val, b, en);
       else
         have_long = true;
     }
   }
-  if (__syncthreads_or(have_long)) {
+  if (!SEQ_ALL && __syncthreads_or(have_long)) {
     const int lane = tid & (kWave - 1), wid = tid / kWave;
     for (int c = wid * kWave; c < nr; c += kBlock) {
       const int i = c + lane;
@@ -272,7 +274,7 @@ __device__ __forceinline__ void stage_row_tile(const Tile& t, const TileRegs<IPT
 //   float init(const Pre&)   value the row sum starts from
 //   double finish(int row, float sum, const Pre&)   writes the row, returns
 //                            its contribution to the tile's double accumulator
-template <int IPT, bool NT, bool SORTED, class G, class E>
+template <int IPT, bool NT, bool SORTED, class G, class E, bool SEQ_ALL = false>
 __device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
                                            const int32_t* __restrict__ col,
                                            const uint16_t* __restrict__ pos, int e_max, int r0,
@@ -285,7 +287,7 @@ __device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
   if (ne > 0) gather_values<IPT, SORTED>(g, e0, ne, R.idx, R.pos, v);
   stage_row_tile<IPT, SORTED>(t, R, v, lds);
   __syncthreads();
-  return reduce_row_tile<IPT>(r0, r1 - r0, e, R.pre, lds);
+  return reduce_row_tile<IPT, E, SEQ_ALL>(r0, r1 - r0, e, R.pre, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -615,6 +617,111 @@ __device__ float exact_seq_sum(const int32_t* __restrict__ col, const uint16_t* 
     s = exact_scan_chunk<IPT>(val, nc, c0 == 0, s, x, tie_mode);
   }
   return s;
+}
+
+// ---------------------------------------------------------------------------
+// Long rows summed by one lane (terms of both signs).
+//
+// When the running sum keeps changing binade (products of both signs around a
+// small sum) the exact scan restarts after almost every step; the fastest
+// exact order is then the sequential chain itself, on one lane, with the
+// workgroup keeping the chain fed.
+
+// s + v[0] + ... + v[n-1] in order with FP32 rounding, by the calling lane,
+// from 16-byte aligned LDS: 4*kQ values per step, the next step's LDS reads in
+// flight while the current step is added (the adds are one dependent chain).
+template <int kQ>  // float4 reads per step
+__device__ __forceinline__ float lane_seq_sum(float s, const float* __restrict__ v, int n) {
+  const float4* __restrict__ v4 = reinterpret_cast<const float4*>(v);
+  const int nb = n / (4 * kQ);
+  float4 A[kQ], B[kQ];
+  auto load = [&](float4(&R)[kQ], int blk) {
+#pragma unroll
+    for (int k = 0; k < kQ; k++) R[k] = v4[blk * kQ + k];
+  };
+  auto add = [&](const float4(&R)[kQ]) {
+#pragma unroll
+    for (int k = 0; k < kQ; k++) {
+      s += R[k].x;
+      s += R[k].y;
+      s += R[k].z;
+      s += R[k].w;
+    }
+  };
+  if (nb > 0) {
+    load(A, 0);
+    int b = 0;
+    // scheduling barriers keep each block's reads issued before the previous
+    // block's adds, so the adds wait only for reads issued one block earlier
+    for (; b + 2 <= nb; b += 2) {
+      load(B, b + 1);
+      __builtin_amdgcn_sched_barrier(0);
+      add(A);
+      __builtin_amdgcn_sched_barrier(0);
+      load(A, min(b + 2, nb - 1));  // unconditional (clamped): A and B keep their registers
+      __builtin_amdgcn_sched_barrier(0);
+      add(B);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (b < nb) add(A);
+  }
+  for (int i = nb * 4 * kQ; i < n; i++) s += v[i];
+  return s;
+}
+
+// s0 + the gathered values of edges [e0, e0+ne) in edge order, rounded as the
+// sequential FP32 loop.  Lane 0 adds chunk c (kBlock*IPT entries, LDS) while
+// the workgroup's gathers of chunk c+1 and column loads of chunk c+2 are in
+// flight (two 16-byte aligned LDS buffers of kBlock*IPT floats, and one float
+// for the broadcast).  All threads return the sum.
+template <int IPT, class G>
+__device__ float long_row_sum(const int32_t* __restrict__ col, int e_max, int e0, int ne,
+                              const G& g, float s0, float* __restrict__ buf0,
+                              float* __restrict__ buf1, float* __restrict__ s_out) {
+  constexpr int C = kBlock * IPT;
+  const int tid = threadIdx.x;
+  const int nchunks = (ne + C - 1) / C;
+  auto edge = [&](int c, int j) {  // edge of lane tid, slot j, chunk c (clamped into the row)
+    const int c0 = c * C, nc = min(C, ne - c0);
+    return e0 + c0 + min(tid + j * kBlock, nc - 1);
+  };
+  int idx[IPT];
+  float v[IPT];
+  auto load_idx = [&](int c) {
+#pragma unroll
+    for (int j = 0; j < IPT; j++) idx[j] = col[min(edge(c, j), e_max)];
+  };
+  auto gather = [&](int c) {
+#pragma unroll
+    for (int j = 0; j < IPT; j++) v[j] = g.value(edge(c, j), idx[j]);
+  };
+  auto store = [&](int c, float* buf) {
+    const int nc = min(C, ne - c * C);
+#pragma unroll
+    for (int j = 0; j < IPT; j++)
+      if (tid + j * kBlock < nc) buf[tid + j * kBlock] = v[j];
+  };
+  float s = s0;
+  if (nchunks > 0) {
+    load_idx(0);
+    gather(0);
+    if (nchunks > 1) load_idx(1);
+    store(0, buf0);
+    __syncthreads();
+  }
+  for (int c = 0; c < nchunks; c++) {
+    float* cur = (c & 1) ? buf1 : buf0;
+    float* nxt = (c & 1) ? buf0 : buf1;
+    const bool more = c + 1 < nchunks;
+    if (more) gather(c + 1);               // indices arrived during chunk c-1's adds
+    if (c + 2 < nchunks) load_idx(c + 2);  // in flight during this chunk's adds
+    if (tid == 0) s = lane_seq_sum<8>(s, cur, min(C, ne - c * C));
+    if (more) store(c + 1, nxt);
+    __syncthreads();
+  }
+  if (tid == 0) *s_out = s;
+  __syncthreads();
+  return *s_out;
 }
 
 }  // namespace gd

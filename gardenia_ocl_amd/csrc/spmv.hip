@@ -2,10 +2,12 @@
 //
 // Same tile engine as PageRank: products x[Aj[k]] * Ax[k] are formed during
 // the gather (rounded, this file is built with -ffp-contract=off) and staged
-// in LDS; rows of <= 64 nonzeros are then summed by one thread starting from
-// y[row] in CSR order, which is exactly the reference's arithmetic.  Longer
-// rows use a wavefront tree; rows longer than a tile are split into segments
-// whose partials are added to y[row] in segment order by a second kernel.
+// in LDS; rows are then summed by one thread starting from
+// y[row] in CSR order, which is exactly the reference's arithmetic; so are
+// the longer rows of a tile (the owner thread's chain just runs longer).
+// Rows longer than a tile get a workgroup each that feeds one lane's
+// sequential chain in CSR order (long_row_sum), so every row is the
+// reference's sum bit for bit.
 
 #include <hip/hip_runtime.h>
 
@@ -40,34 +42,34 @@ struct SpEpilogue {
   }
 };
 
+// Tile mode, one launch: workgroups [0, n_long) each add one row longer than
+// a tile (2048 nonzeros) in CSR order on one lane (long_row_sum: products of
+// both signs), the rest one LDS row tile each (every row summed by its owner
+// thread in CSR order).  Every row is the reference's sum bit for bit.  LONG =
+// false (no long rows) leaves the long-row code, and its registers, out.
+template <bool LONG>
 __global__ __launch_bounds__(kBlock) void spmv_pull_kernel(
-    const Tile* __restrict__ tiles, const int32_t* __restrict__ Ap,
-    const int32_t* __restrict__ Aj, int e_max, const float* __restrict__ Ax,
-    const float* __restrict__ x, float* __restrict__ y, float* __restrict__ hub_part) {
-  __shared__ RowTileLds<kSpIPT> lds;
-  const Tile t = tiles[blockIdx.x];
+    const Tile* __restrict__ tiles, const Tile* __restrict__ long_tiles, int n_long,
+    const int32_t* __restrict__ Ap, const int32_t* __restrict__ Aj, int e_max,
+    const float* __restrict__ Ax, const float* __restrict__ x, float* __restrict__ y) {
+  // a row tile's LDS, or a long row's two chunk buffers + the broadcast sum
+  static_assert(sizeof(RowTileLds<kSpIPT>) >= (2 * kSpTileNnz + 1) * sizeof(float), "LDS union");
+  __shared__ __align__(16) unsigned char smem[sizeof(RowTileLds<kSpIPT>)];
+  RowTileLds<kSpIPT>& lds = *reinterpret_cast<RowTileLds<kSpIPT>*>(smem);
   const SpGather g{x, Ax};
-  if (t.y >= 0) {
-    const SpEpilogue e{y};
-    (void)row_tile<kSpIPT, false, false>(Ap, Aj, nullptr, e_max, t.x, t.y, t.z, t.w - t.z, g, e, lds);
+  const int b = blockIdx.x;
+  if (LONG && b < n_long) {
+    float* const buf = reinterpret_cast<float*>(smem);
+    const Tile t = long_tiles[b];
+    const float s = long_row_sum<kSpIPT>(Aj, e_max, t.z, t.w - t.z, g, y[t.x], buf,
+                                         buf + kSpTileNnz, buf + 2 * kSpTileNnz);
+    if (threadIdx.x == 0) y[t.x] = s;
   } else {
-    const float p = segment_sum<kSpIPT, false, false>(Aj, nullptr, t.z, t.w - t.z, g, lds.val);
-    if (threadIdx.x == 0) hub_part[-t.y - 1] = p;
+    const Tile t = tiles[b - n_long];
+    const SpEpilogue e{y};
+    (void)row_tile<kSpIPT, false, false, SpGather, SpEpilogue, true>(
+        Ap, Aj, nullptr, e_max, t.x, t.y, t.z, t.w - t.z, g, e, lds);
   }
-}
-
-__global__ __launch_bounds__(kBlock) void spmv_hub_kernel(const int32_t* __restrict__ hub_row,
-                                                          const int32_t* __restrict__ hub_slot,
-                                                          const int32_t* __restrict__ hub_nseg,
-                                                          int n_hubs,
-                                                          const float* __restrict__ hub_part,
-                                                          float* __restrict__ y) {
-  const int h = blockIdx.x * kBlock + threadIdx.x;
-  if (h >= n_hubs) return;
-  const float* p = hub_part + hub_slot[h];
-  float s = y[hub_row[h]];
-  for (int k = 0; k < hub_nseg[h]; k++) s += p[k];
-  y[hub_row[h]] = s;
 }
 
 // ---------------------------------------------------------------- column slices
@@ -197,8 +199,8 @@ struct gd_spmv_plan {
   const float* Ax = nullptr;
   hipStream_t stream = nullptr;
   Schedule sched;
-  int n_tiles = 0, n_hubs = 0;
-  DeviceBuffer tiles, hub_row, hub_slot, hub_nseg, hub_part;
+  int n_tiles = 0, n_hubs = 0;  // row tiles; rows longer than a tile
+  DeviceBuffer tiles, long_tiles;
   DeviceBuffer dummy_col;  // 1 element, used as Aj when nnz == 0
   int e_max = 0;
   // column-sliced mode
@@ -311,7 +313,8 @@ int gd_spmv_plan_create_ex(int rows, int cols, int64_t nnz, const int32_t* d_Ap,
       set_error(std::string("copying row offsets: ") + hipGetErrorString(e));
       rc = GD_ERR_DEVICE;
     }
-    if (!rc) rc = build_schedule(rows, nnz, h_off.data(), kSpTileNnz, kTileRows, &p->sched);
+    if (!rc)
+      rc = build_schedule(rows, nnz, h_off.data(), kSpTileNnz, kTileRows, &p->sched, kSpTileNnz);
     if (!rc) rc = check_columns(d_Aj, nnz, cols, p->stream);
     if (!rc && nnz > 0 && mode != GD_SPMV_TILES) {
       // auto: slice when x spans more than one XCD's L2 and rows are short
@@ -330,29 +333,23 @@ int gd_spmv_plan_create_ex(int rows, int cols, int64_t nnz, const int32_t* d_Ap,
     }
     p->e_max = (int)std::max<int64_t>(nnz - 1, 0);
     if (!rc) {
-      p->n_tiles = (int)p->sched.tiles.size();
-      p->n_hubs = (int)p->sched.hub_row.size();
-      rc = p->tiles.alloc(sizeof(Tile) * p->n_tiles);
-      if (!rc) rc = p->hub_row.alloc(4 * std::max(1, p->n_hubs));
-      if (!rc) rc = p->hub_slot.alloc(4 * std::max(1, p->n_hubs));
-      if (!rc) rc = p->hub_nseg.alloc(4 * std::max(1, p->n_hubs));
-      if (!rc) rc = p->hub_part.alloc(sizeof(float) * std::max(1, p->sched.n_slots));
-    }
-    if (!rc) {
-      hipError_t e2 = hipMemcpyAsync(p->tiles.ptr, p->sched.tiles.data(),
-                                     sizeof(Tile) * p->n_tiles, hipMemcpyHostToDevice, p->stream);
-      if (e2 == hipSuccess && p->n_hubs) {
-        e2 = hipMemcpyAsync(p->hub_row.ptr, p->sched.hub_row.data(), 4 * p->n_hubs,
+      std::vector<Tile> row_tiles, longs;
+      for (const Tile& t : p->sched.tiles) (t.y < 0 ? longs : row_tiles).push_back(t);
+      std::stable_sort(longs.begin(), longs.end(),  // the longest rows start first
+                       [](const Tile& a, const Tile& b) { return a.w - a.z > b.w - b.z; });
+      p->n_tiles = (int)row_tiles.size();
+      p->n_hubs = (int)longs.size();
+      rc = p->tiles.alloc(sizeof(Tile) * std::max(1, p->n_tiles));
+      if (!rc) rc = p->long_tiles.alloc(sizeof(Tile) * std::max(1, p->n_hubs));
+      hipError_t e2 = hipSuccess;
+      if (!rc && p->n_tiles)
+        e2 = hipMemcpyAsync(p->tiles.ptr, row_tiles.data(), sizeof(Tile) * p->n_tiles,
                             hipMemcpyHostToDevice, p->stream);
-        if (e2 == hipSuccess)
-          e2 = hipMemcpyAsync(p->hub_slot.ptr, p->sched.hub_slot.data(), 4 * p->n_hubs,
-                              hipMemcpyHostToDevice, p->stream);
-        if (e2 == hipSuccess)
-          e2 = hipMemcpyAsync(p->hub_nseg.ptr, p->sched.hub_nseg.data(), 4 * p->n_hubs,
-                              hipMemcpyHostToDevice, p->stream);
-      }
-      if (e2 == hipSuccess) e2 = hipStreamSynchronize(p->stream);
-      if (e2 != hipSuccess) {
+      if (!rc && e2 == hipSuccess && p->n_hubs)
+        e2 = hipMemcpyAsync(p->long_tiles.ptr, longs.data(), sizeof(Tile) * p->n_hubs,
+                            hipMemcpyHostToDevice, p->stream);
+      if (!rc && e2 == hipSuccess) e2 = hipStreamSynchronize(p->stream);
+      if (!rc && e2 != hipSuccess) {
         set_error(std::string("uploading schedule: ") + hipGetErrorString(e2));
         rc = GD_ERR_DEVICE;
       }
@@ -380,13 +377,14 @@ int gd_spmv_plan_run(gd_spmv_plan* p, const float* d_x, float* d_y) {
     GD_HIP_TRY(hipGetLastError());
     return GD_OK;
   }
-  hipLaunchKernelGGL(spmv_pull_kernel, dim3(p->n_tiles), dim3(kBlock), 0, p->stream,
-                     p->tiles.as<Tile>(), p->Ap, p->Aj, p->e_max, p->Ax, d_x, d_y,
-                     p->hub_part.as<float>());
   if (p->n_hubs)
-    hipLaunchKernelGGL(spmv_hub_kernel, dim3((p->n_hubs + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                       p->stream, p->hub_row.as<int32_t>(), p->hub_slot.as<int32_t>(),
-                       p->hub_nseg.as<int32_t>(), p->n_hubs, p->hub_part.as<float>(), d_y);
+    hipLaunchKernelGGL(spmv_pull_kernel<true>, dim3(p->n_hubs + p->n_tiles), dim3(kBlock), 0,
+                       p->stream, p->tiles.as<Tile>(), p->long_tiles.as<Tile>(), p->n_hubs, p->Ap,
+                       p->Aj, p->e_max, p->Ax, d_x, d_y);
+  else if (p->n_tiles)
+    hipLaunchKernelGGL(spmv_pull_kernel<false>, dim3(p->n_tiles), dim3(kBlock), 0, p->stream,
+                       p->tiles.as<Tile>(), p->long_tiles.as<Tile>(), 0, p->Ap, p->Aj, p->e_max,
+                       p->Ax, d_x, d_y);
   GD_HIP_TRY(hipGetLastError());
   return GD_OK;
 }

@@ -90,98 +90,6 @@ struct SymgsPolicy {
 
 constexpr int kSgChunk = kBlock * kSgIPT;  // entries of a long row per LDS chunk
 
-// s + v[0] + ... + v[n-1] in order with FP32 rounding, by the calling lane,
-// from 16-byte aligned LDS: 4*kQ values per step, the next step's LDS reads in
-// flight while the current step is added (the adds are one dependent chain).
-template <int kQ>  // float4 reads per step
-__device__ __forceinline__ float lane_seq_sum(float s, const float* __restrict__ v, int n) {
-  const float4* __restrict__ v4 = reinterpret_cast<const float4*>(v);
-  const int nb = n / (4 * kQ);
-  float4 A[kQ], B[kQ];
-  auto load = [&](float4(&R)[kQ], int blk) {
-#pragma unroll
-    for (int k = 0; k < kQ; k++) R[k] = v4[blk * kQ + k];
-  };
-  auto add = [&](const float4(&R)[kQ]) {
-#pragma unroll
-    for (int k = 0; k < kQ; k++) {
-      s += R[k].x;
-      s += R[k].y;
-      s += R[k].z;
-      s += R[k].w;
-    }
-  };
-  if (nb > 0) {
-    load(A, 0);
-    int b = 0;
-    // scheduling barriers keep each block's reads issued before the previous
-    // block's adds, so the adds wait only for reads issued one block earlier
-    for (; b + 2 <= nb; b += 2) {
-      load(B, b + 1);
-      __builtin_amdgcn_sched_barrier(0);
-      add(A);
-      __builtin_amdgcn_sched_barrier(0);
-      load(A, min(b + 2, nb - 1));  // unconditional (clamped): A and B keep their registers
-      __builtin_amdgcn_sched_barrier(0);
-      add(B);
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    if (b < nb) add(A);
-  }
-  for (int i = nb * 4 * kQ; i < n; i++) s += v[i];
-  return s;
-}
-
-// One long row: its off-diagonal products summed in CSR order from 0,
-// rounded as the reference's sequential loop.  Lane 0 adds chunk c (2048
-// entries, LDS) while the workgroup's gathers of chunk c+1 and column loads of
-// chunk c+2 are in flight (two LDS buffers).  All threads return the sum.
-__device__ float long_row_sum(const int32_t* __restrict__ col, int e_max, int e0, int ne,
-                              const SymgsGather& g, float* __restrict__ buf0,
-                              float* __restrict__ buf1, float* __restrict__ s_out) {
-  const int tid = threadIdx.x;
-  const int nchunks = (ne + kSgChunk - 1) / kSgChunk;
-  auto edge = [&](int c, int j) {  // edge of lane tid, slot j, chunk c (clamped into the row)
-    const int c0 = c * kSgChunk, nc = min(kSgChunk, ne - c0);
-    return e0 + c0 + min(tid + j * kBlock, nc - 1);
-  };
-  int idx[kSgIPT];
-  float v[kSgIPT];
-  auto load_idx = [&](int c) {
-#pragma unroll
-    for (int j = 0; j < kSgIPT; j++) idx[j] = col[min(edge(c, j), e_max)];
-  };
-  auto gather = [&](int c) {
-#pragma unroll
-    for (int j = 0; j < kSgIPT; j++) v[j] = g.value(edge(c, j), idx[j]);
-  };
-  auto store = [&](int c, float* buf) {
-    const int nc = min(kSgChunk, ne - c * kSgChunk);
-#pragma unroll
-    for (int j = 0; j < kSgIPT; j++)
-      if (tid + j * kBlock < nc) buf[tid + j * kBlock] = v[j];
-  };
-  load_idx(0);
-  gather(0);
-  if (nchunks > 1) load_idx(1);
-  store(0, buf0);
-  __syncthreads();
-  float s = 0.0f;
-  for (int c = 0; c < nchunks; c++) {
-    float* cur = (c & 1) ? buf1 : buf0;
-    float* nxt = (c & 1) ? buf0 : buf1;
-    const bool more = c + 1 < nchunks;
-    if (more) gather(c + 1);                  // indices arrived during chunk c-1's adds
-    if (c + 2 < nchunks) load_idx(c + 2);     // in flight during this chunk's adds
-    if (tid == 0) s = lane_seq_sum<8>(s, cur, min(kSgChunk, ne - c * kSgChunk));
-    if (more) store(c + 1, nxt);
-    __syncthreads();
-  }
-  if (tid == 0) *s_out = s;
-  __syncthreads();
-  return *s_out;
-}
-
 // One color in one launch: workgroups [0, n_hub_tiles) add one long row each
 // (long_row_sum), the rest one LDS row tile each (owner thread per row, in
 // order).  Long rows first: they take longest.  LONG = false (colors without
@@ -200,8 +108,8 @@ __global__ __launch_bounds__(kBlock) void symgs_color_kernel(const SymgsPolicy::
   if (LONG && b < it.n_hub_tiles) {
     const Tile t = it.hub_tiles[b];
     const auto pre = e.pre(t.x);
-    const float rsum = long_row_sum(it.col, it.e_max, t.z, t.w - t.z, g, buf, buf + kSgChunk,
-                                    buf + 2 * kSgChunk);
+    const float rsum = long_row_sum<kSgIPT>(it.col, it.e_max, t.z, t.w - t.z, g, 0.0f, buf,
+                                            buf + kSgChunk, buf + 2 * kSgChunk);
     if (threadIdx.x == 0) (void)e.finish(t.x, rsum, pre);
   } else {
     const Tile t = it.tiles[b - it.n_hub_tiles];

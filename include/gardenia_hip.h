@@ -230,18 +230,22 @@ typedef struct gd_spmv_plan gd_spmv_plan;
 int gd_spmv_plan_create(int num_rows, int num_cols, int64_t nnz, const int32_t *d_Ap,
                         const int32_t *d_Aj, const float *d_Ax, void *stream, gd_spmv_plan **plan);
 
-/* y <- y + A x (src/spmv/omp_base.cc:17-28).  Rows up to 64 nonzeros are
- * summed sequentially in CSR order (bit-identical to the reference); longer
- * rows use a wavefront tree. */
+/* y <- y + A x (src/spmv/omp_base.cc:17-28).  Every row is summed
+ * sequentially in CSR order from y[row], bit-identical to the reference: rows
+ * of up to 2048 nonzeros by one thread of an LDS row tile, longer rows by one
+ * lane of a workgroup that gathers the row chunk by chunk. */
 int gd_spmv_plan_run(gd_spmv_plan *plan, const float *d_x, float *d_y);
 
+/* n_tiles: LDS row tiles; n_hubs: rows of more than 2048 nonzeros (one
+ * workgroup each). */
 int gd_spmv_plan_info(const gd_spmv_plan *plan, int64_t *n_tiles, int64_t *n_hubs,
                       int32_t *max_row_len);
 
 int gd_spmv_plan_destroy(gd_spmv_plan *plan);
 
 /* SpMV execution modes.
- *   GD_SPMV_TILES   the tile engine over the caller's CSR (any matrix).
+ *   GD_SPMV_TILES   the tile engine over the caller's CSR (any matrix, any
+ *                   row order; bit-identical to the reference).
  *   GD_SPMV_SLICED  columns split into 8 slices, one per XCD: pass 1 forms the
  *                   products over a slice-major copy of the matrix (each XCD's
  *                   L2 holds one slice of x), pass 2 adds each row's products
