@@ -109,7 +109,9 @@ struct IterEngine {
         if (rc) return rc;
         dst = col_sorted.as<int32_t>();
       }
-      rc = build_tile_sort(sched, kBlock * ipt, col, dst, pos.as<uint16_t>(), stream);
+      // exact hub rows stay in edge order: one lane adds them as they are read
+      rc = build_tile_sort(sched, kBlock * ipt, col, dst, pos.as<uint16_t>(), stream,
+                           exact_len > 0);
       if (rc) return rc;
       col = dst;
     }
@@ -186,23 +188,34 @@ struct IterEngine {
 
   template <class P, int IPT, int NT, bool SORTED>
   void launch_pull_s(const typename P::Args& a, const IterArgs& it) {
-    if (n_hub_tiles) {  // fork: exact hubs on the side stream
+    // Exact hubs go first, on the plan's stream, so that their workgroups
+    // (longest row first) are placed before the row tiles fill the CUs; the
+    // row tiles fork onto the side stream (after the same prior work) and
+    // join back.
+    const bool fork = n_hub_tiles > 0 && n_tiles > 0;
+    hipStream_t ts = stream;
+    if (fork) {
       (void)hipEventRecord(fork_ev, stream);
       (void)hipStreamWaitEvent(side, fork_ev, 0);
+      ts = side;
+    }
+    if (n_hub_tiles) {
       if constexpr (P::kExactHubs)
-        hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED>), dim3(n_hub_tiles),
-                           dim3(kBlock), 0, side, a, it);
-      (void)hipEventRecord(join_ev, side);
+        hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT>), dim3(n_hub_tiles), dim3(kBlock), 0,
+                           stream, a, it);
     }
     if (n_tiles > 0) {
       if (persistent)
         hipLaunchKernelGGL((pull_persistent_kernel<P, IPT, NT, SORTED>), dim3(pull_grid),
-                           dim3(kBlock), 0, stream, a, it);
+                           dim3(kBlock), 0, ts, a, it);
       else
         hipLaunchKernelGGL((pull_tile_kernel<P, IPT, NT, SORTED>), dim3(n_tiles), dim3(kBlock), 0,
-                           stream, a, it);
+                           ts, a, it);
     }
-    if (n_hub_tiles) (void)hipStreamWaitEvent(stream, join_ev, 0);  // join
+    if (fork) {
+      (void)hipEventRecord(join_ev, side);
+      (void)hipStreamWaitEvent(stream, join_ev, 0);  // join
+    }
   }
 
   template <class P, int IPT, int NT>

@@ -29,6 +29,8 @@ namespace gd {
 namespace {
 
 constexpr int kMgIPT = 16;  // as the single-graph plan: same tiles, same per-row sums
+constexpr int kMgHubIPT = 8;  // hub rows: LDS chunks of kBlock*8 edges, two in flight
+constexpr int kMgChunk = kBlock * kMgHubIPT;
 
 struct GraphState {
   int iter;
@@ -65,9 +67,11 @@ __global__ __launch_bounds__(kBlock) void mg_pull_kernel(const MgArgs a) {
   const int g = a.tile_graph[blockIdx.x];
   if (a.gs[g].done) return;
   const int odd = a.gs[g].iter & 1;
-  __shared__ RowTileLds<kMgIPT> lds;
+  // a row tile's LDS, or a hub row's two chunk buffers + the broadcast sum
+  static_assert(sizeof(RowTileLds<kMgIPT>) >= (2 * kMgChunk + 1) * sizeof(float), "LDS union");
+  __shared__ __align__(16) unsigned char smem[sizeof(RowTileLds<kMgIPT>)];
+  RowTileLds<kMgIPT>& lds = *reinterpret_cast<RowTileLds<kMgIPT>*>(smem);
   __shared__ double s_err[kWaves];
-  __shared__ ExactLds xs;
   const Tile t = a.tiles[blockIdx.x];
   const PrGather gat{odd ? a.c1 : a.c0};
   double err = 0.0;
@@ -76,8 +80,9 @@ __global__ __launch_bounds__(kBlock) void mg_pull_kernel(const MgArgs a) {
     err = row_tile<kMgIPT, false, false>(a.off, a.col, nullptr, a.e_max, t.x, t.y, t.z, t.w - t.z, gat, e, lds);
   } else {  // whole hub row in the reference's order, as the single-graph plan
     const auto pre = e.pre(t.x);
-    const float sum = exact_seq_sum<kMgIPT, false, false>(a.col, nullptr, a.e_max, t.z, t.w - t.z, gat,
-                                                   e.init(pre), lds.val, xs);
+    float* const buf = reinterpret_cast<float*>(smem);
+    const float sum = long_row_sum<kMgHubIPT>(a.col, nullptr, a.e_max, t.z, t.w - t.z, gat,
+                                              e.init(pre), buf, buf + kMgChunk, buf + 2 * kMgChunk);
     if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
   }
   const double tot = block_sum<kWaves>(err, s_err);

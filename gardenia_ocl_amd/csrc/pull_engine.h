@@ -144,7 +144,7 @@ __device__ __forceinline__ float segment_sum(const int32_t* __restrict__ col,
 // LDS of one row tile.
 template <int IPT>
 struct RowTileLds {
-  float val[kBlock * IPT + kBlock];  // gathered per-edge values (+ padding, exact_seq_sum)
+  float val[kBlock * IPT];    // gathered per-edge values
   int off[kTileRows + 1];     // row offsets, rebased to the tile on use
   float longsum[kTileRows];   // wavefront sums of rows longer than kSeqRow
 };
@@ -587,7 +587,7 @@ __device__ float exact_scan_chunk(const float* __restrict__ val, int nc, bool he
 
 // Sum of the gathered values of edges [e0, e0+ne) in edge order, rounded as the
 // sequential FP32 loop starting from s.  All threads return the same value.
-// `val` holds kBlock*IPT + kBlock floats.
+// `val` holds kBlock*IPT + kBlock floats (the padded layout exact_pos).
 template <int IPT, bool NT, bool SORTED, class G>
 __device__ float exact_seq_sum(const int32_t* __restrict__ col, const uint16_t* __restrict__ pos,
                                int e_max, int e0, int ne, const G& g, float s,
@@ -620,12 +620,17 @@ __device__ float exact_seq_sum(const int32_t* __restrict__ col, const uint16_t* 
 }
 
 // ---------------------------------------------------------------------------
-// Long rows summed by one lane (terms of both signs).
+// Long rows summed in edge order by one lane.
 //
-// When the running sum keeps changing binade (products of both signs around a
-// small sum) the exact scan restarts after almost every step; the fastest
-// exact order is then the sequential chain itself, on one lane, with the
-// workgroup keeping the chain fed.
+// The reference adds a row's terms one after the other in FP32
+// (src/pr/omp_base.cc:34-37, src/spmv/omp_base.cc:21-24); for a row longer
+// than a tile the fastest way to the same bits is that dependent chain
+// itself, on one lane, with the rest of the workgroup keeping it fed: the
+// gathers of chunk c+1 and the index loads of chunk c+2 are in flight while
+// lane 0 adds chunk c from LDS.  Measured on the MI355X the chain costs
+// 2.8-3.4 ns per term (a dependent v_add_f32 every 4 cycles when the terms
+// sit in registers; the LDS reads feeding it take the rest), independent of
+// the terms' signs, ties or magnitudes.
 
 // s + v[0] + ... + v[n-1] in order with FP32 rounding, by the calling lane,
 // from 16-byte aligned LDS: 4*kQ values per step, the next step's LDS reads in
@@ -674,33 +679,49 @@ __device__ __forceinline__ float lane_seq_sum(float s, const float* __restrict__
 // the workgroup's gathers of chunk c+1 and column loads of chunk c+2 are in
 // flight (two 16-byte aligned LDS buffers of kBlock*IPT floats, and one float
 // for the broadcast).  All threads return the sum.
-template <int IPT, class G>
-__device__ float long_row_sum(const int32_t* __restrict__ col, int e_max, int e0, int ne,
-                              const G& g, float s0, float* __restrict__ buf0,
-                              float* __restrict__ buf1, float* __restrict__ s_out) {
+// SORTED: each chunk of the row is stored sorted by source (tile_sort.h) and
+// pos[e] is the edge's offset within its chunk; values go back to edge order
+// in LDS, and g.value sees the edge's own (unsorted) index.
+template <int IPT, int kQ = 8, bool SORTED = false, class G>
+__device__ float long_row_sum(const int32_t* __restrict__ col, const uint16_t* __restrict__ pos,
+                              int e_max, int e0, int ne, const G& g, float s0,
+                              float* __restrict__ buf0, float* __restrict__ buf1,
+                              float* __restrict__ s_out) {
   constexpr int C = kBlock * IPT;
   const int tid = threadIdx.x;
   const int nchunks = (ne + C - 1) / C;
-  auto edge = [&](int c, int j) {  // edge of lane tid, slot j, chunk c (clamped into the row)
-    const int c0 = c * C, nc = min(C, ne - c0);
-    return e0 + c0 + min(tid + j * kBlock, nc - 1);
+  auto slot = [&](int c, int j) {  // offset of lane tid, slot j in chunk c (clamped into the row)
+    return min(tid + j * kBlock, min(C, ne - c * C) - 1);
   };
-  int idx[IPT];
+  int idx[IPT], p[IPT], ps[IPT];
   float v[IPT];
   auto load_idx = [&](int c) {
 #pragma unroll
-    for (int j = 0; j < IPT; j++) idx[j] = col[min(edge(c, j), e_max)];
+    for (int j = 0; j < IPT; j++) idx[j] = col[min(e0 + c * C + slot(c, j), e_max)];
+    if constexpr (SORTED) {
+#pragma unroll
+      for (int j = 0; j < IPT; j++) p[j] = (int)pos[min(e0 + c * C + slot(c, j), e_max)];
+    }
   };
   auto gather = [&](int c) {
 #pragma unroll
-    for (int j = 0; j < IPT; j++) v[j] = g.value(edge(c, j), idx[j]);
+    for (int j = 0; j < IPT; j++) {
+      const int o = SORTED ? p[j] : slot(c, j);
+      v[j] = g.value(e0 + c * C + o, idx[j]);
+      ps[j] = o;
+    }
   };
   auto store = [&](int c, float* buf) {
     const int nc = min(C, ne - c * C);
 #pragma unroll
     for (int j = 0; j < IPT; j++)
-      if (tid + j * kBlock < nc) buf[tid + j * kBlock] = v[j];
+      if (tid + j * kBlock < nc) buf[ps[j]] = v[j];
   };
+  // The chain's wavefront issues first on its SIMD: the row tiles that share
+  // the CU would otherwise take most of its issue slots and stretch every
+  // dependent add.
+  const bool chain_wave = tid < kWave;
+  if (chain_wave) __builtin_amdgcn_s_setprio(3);
   float s = s0;
   if (nchunks > 0) {
     load_idx(0);
@@ -715,14 +736,16 @@ __device__ float long_row_sum(const int32_t* __restrict__ col, int e_max, int e0
     const bool more = c + 1 < nchunks;
     if (more) gather(c + 1);               // indices arrived during chunk c-1's adds
     if (c + 2 < nchunks) load_idx(c + 2);  // in flight during this chunk's adds
-    if (tid == 0) s = lane_seq_sum<8>(s, cur, min(C, ne - c * C));
+    if (tid == 0) s = lane_seq_sum<kQ>(s, cur, min(C, ne - c * C));
     if (more) store(c + 1, nxt);
     __syncthreads();
   }
+  if (chain_wave) __builtin_amdgcn_s_setprio(0);
   if (tid == 0) *s_out = s;
   __syncthreads();
   return *s_out;
 }
+
 
 }  // namespace gd
 

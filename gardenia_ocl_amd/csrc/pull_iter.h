@@ -11,7 +11,7 @@
 //   struct P::Args                      kernel arguments (pointers, scalars)
 //   static constexpr bool kExactHubs    hub rows (whole-row tiles) summed in
 //                                       edge order with sequential FP32
-//                                       rounding (exact_seq_sum), else hub
+//                                       rounding (exact_hub_kernel), else hub
 //                                       segments tree-summed + finalize
 //   template <int NT> static G gather(const Args&, int odd)
 //   template <int NT> static E epilogue(const Args&, int odd)
@@ -151,25 +151,41 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
   if (threadIdx.x == 0) it.err_part[blockIdx.x] = tot;
 }
 
-// Whole-row hubs of an exact-hub policy: one workgroup per hub row gathers the
-// row chunk by chunk and adds it in edge order with the reference's sequential
-// rounding (exact_seq_sum), then runs the row's epilogue.  A kernel of its own,
-// launched on a second stream concurrently with the row-tile launch: its
-// register demand would otherwise lower the occupancy of every row tile.
-template <class P, int IPT, int NT, bool SORTED>
+// Whole-row hubs of an exact-hub policy: one workgroup per hub row adds the
+// row in edge order with the reference's sequential rounding, then runs the
+// row's epilogue.  Rows of up to kScanRow edges: one lane's dependent chain
+// (long_row_sum), the shortest path to the bits for a row alone on the GPU;
+// longer rows: the workgroup's binade scan (exact_seq_sum), whose cost per
+// term holds up beside the row tiles where a chain's doubles (their waves
+// take the CU's issue slots and LDS).  A kernel of its own, launched on a
+// second stream concurrently with the row-tile launch; its registers are kept
+// near a row tile's so that a hub workgroup fits on a CU beside four of them.
+// Hub rows are stored in edge order (not source-sorted).
+constexpr int kHubIPT = 8;         // chain chunks: kBlock*8 edges, two in LDS
+constexpr int kScanRow = 1 << 17;  // longer hub rows: binade scan
+
+template <class P, int IPT, int NT>
 __global__ __launch_bounds__(kBlock) void exact_hub_kernel(const typename P::Args a,
                                                            const IterArgs it) {
   if (it.st->done) return;
   const int odd = it.st->iter & 1;
-  __shared__ RowTileLds<IPT> lds;
+  constexpr int C = kBlock * kHubIPT;
+  constexpr int kBuf = (2 * C > kBlock * IPT + kBlock ? 2 * C : kBlock * IPT + kBlock) + 4;
+  __shared__ __align__(16) float buf[kBuf];
   __shared__ double s_err[kWaves];
   __shared__ ExactLds xs;
   const Tile t = it.hub_tiles[blockIdx.x];
   const auto g = P::template gather<NT>(a, odd);
   const auto e = P::template epilogue<NT>(a, odd);
   const auto pre = e.pre(t.x);
-  const float sum = exact_seq_sum<IPT, (NT > 0), SORTED>(it.col, it.pos, it.e_max, t.z, t.w - t.z,
-                                                         g, e.init(pre), lds.val, xs);
+  const int ne = t.w - t.z;
+  float sum;
+  if (ne > kScanRow)
+    sum = exact_seq_sum<IPT, (NT > 0), false>(it.col, nullptr, it.e_max, t.z, ne, g, e.init(pre),
+                                              buf, xs);
+  else
+    sum = long_row_sum<kHubIPT>(it.col, nullptr, it.e_max, t.z, ne, g, e.init(pre), buf, buf + C,
+                                buf + 2 * C);
   double err = 0.0;
   if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
   const double tot = block_sum<kWaves>(err, s_err);

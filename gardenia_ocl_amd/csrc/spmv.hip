@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void spmv_pull_kernel(
   if (LONG && b < n_long) {
     float* const buf = reinterpret_cast<float*>(smem);
     const Tile t = long_tiles[b];
-    const float s = long_row_sum<kSpIPT>(Aj, e_max, t.z, t.w - t.z, g, y[t.x], buf,
+    const float s = long_row_sum<kSpIPT>(Aj, nullptr, e_max, t.z, t.w - t.z, g, y[t.x], buf,
                                          buf + kSpTileNnz, buf + 2 * kSpTileNnz);
     if (threadIdx.x == 0) y[t.x] = s;
   } else {

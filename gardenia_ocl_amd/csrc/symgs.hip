@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void symgs_color_kernel(const SymgsPolicy::
   if (LONG && b < it.n_hub_tiles) {
     const Tile t = it.hub_tiles[b];
     const auto pre = e.pre(t.x);
-    const float rsum = long_row_sum<kSgIPT>(it.col, it.e_max, t.z, t.w - t.z, g, 0.0f, buf,
+    const float rsum = long_row_sum<kSgIPT>(it.col, nullptr, it.e_max, t.z, t.w - t.z, g, 0.0f, buf,
                                             buf + kSgChunk, buf + 2 * kSgChunk);
     if (threadIdx.x == 0) (void)e.finish(t.x, rsum, pre);
   } else {

@@ -13,6 +13,7 @@ constexpr int kSortBlock = 256;
 // One workgroup per segment [seg.x, seg.x + seg.y): keys (source << 12 |
 // offset) sorted by an LDS bitonic network, padded to a power of two with
 // UINT64_MAX.  Sorting the 64-bit key orders by source, then by offset.
+// seg.y < 0: the -seg.y edges stay in edge order.
 __global__ __launch_bounds__(kSortBlock) void segment_sort_kernel(const int2* __restrict__ segs,
                                                                   const int32_t* __restrict__ col,
                                                                   int32_t* __restrict__ col_sorted,
@@ -20,6 +21,13 @@ __global__ __launch_bounds__(kSortBlock) void segment_sort_kernel(const int2* __
   __shared__ uint64_t key[kSortMax];
   const int2 sg = segs[blockIdx.x];
   const int e0 = sg.x, n = sg.y;
+  if (n < 0) {
+    for (int i = threadIdx.x; i < -n; i += kSortBlock) {
+      col_sorted[e0 + i] = col[e0 + i];
+      pos[e0 + i] = (uint16_t)i;
+    }
+    return;
+  }
   int np = 1;
   while (np < n) np <<= 1;
   for (int i = threadIdx.x; i < np; i += kSortBlock)
@@ -52,7 +60,7 @@ __global__ __launch_bounds__(kSortBlock) void segment_sort_kernel(const int2* __
 }  // namespace
 
 int build_tile_sort(const Schedule& s, int chunk, const int32_t* d_col, int32_t* d_col_sorted,
-                    uint16_t* d_pos, hipStream_t stream) {
+                    uint16_t* d_pos, hipStream_t stream, bool hubs_in_edge_order) {
   if (chunk <= 0 || chunk > kSortMax) {
     set_error("build_tile_sort: chunk must be in [1, 4096]");
     return GD_ERR_INVALID;
@@ -60,13 +68,14 @@ int build_tile_sort(const Schedule& s, int chunk, const int32_t* d_col, int32_t*
   std::vector<int2> segs;
   segs.reserve(s.tiles.size() + 16);
   for (const Tile& t : s.tiles) {
+    const bool keep = hubs_in_edge_order && t.y < 0;
     for (int32_t e = t.z; e < t.w; e += chunk) {
       const int32_t n = std::min(chunk, t.w - e);
       if (n > kSortMax) {
         set_error("build_tile_sort: a tile is longer than 4096 edges");
         return GD_ERR_INVALID;
       }
-      segs.push_back(make_int2(e, n));
+      segs.push_back(make_int2(e, keep ? -n : n));
     }
   }
   if (segs.empty()) return GD_OK;

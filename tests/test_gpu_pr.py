@@ -213,7 +213,8 @@ def _star(n_leaves, hub=0):
     return out_csr, in_csr
 
 
-@pytest.mark.parametrize("n_leaves", [1, 63, 64, 65, 2047, 2048, 2049, 4096, 4097, 100_000])
+@pytest.mark.parametrize("n_leaves", [1, 63, 64, 65, 2047, 2048, 2049, 4096, 4097, 100_000,
+                                      140_000])
 def test_pr_star_hub_boundaries(device, n_leaves):
     """Hub of n in-edges, all addends equal: the reference's sequential FP32 sum
     drifts (its L1 change plateaus at ~3e-6 for n=65, ~7e-5 for n=2048 and the
@@ -239,13 +240,14 @@ def test_pr_star_hub_boundaries(device, n_leaves):
 
 
 @pytest.mark.parametrize("mix", ["ties", "ties_and_crossings"])
-def test_pr_hub_row_of_ties_bit_exact(device, mix):
-    """A hub of 20,000 in-edges whose addends are mostly exact ties (odd
-    multiples of half an ulp of the running sum): each rounds to even, so the
-    sum depends on every step's parity.  The exact hub path resolves them in
-    its scan; the first iteration must equal the reference's sequential sum."""
-    rng = np.random.default_rng(11 if mix == "ties" else 12)
-    n = 20000
+@pytest.mark.parametrize("n", [20_000, 140_000])
+def test_pr_hub_row_of_ties_bit_exact(device, mix, n):
+    """A hub of n in-edges whose addends are mostly exact ties (odd multiples
+    of half an ulp of the running sum): each rounds to even, so the sum depends
+    on every step's parity.  20,000 in-edges take the one-lane chain, 140,000
+    (more than 2^17) the binade scan, which resolves the ties as maps; the
+    first iteration must equal the reference's sequential sum either way."""
+    rng = np.random.default_rng((11 if mix == "ties" else 12) + n)
     m = n + 1
     src = np.arange(1, m, dtype=np.int32)
     dst = np.zeros(n, np.int32)

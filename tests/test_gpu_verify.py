@@ -48,9 +48,10 @@ def test_verifier_datasets_match_reference_verdicts(golden, device, ds, mode, ca
 
 
 def test_verifier_long_rows_exact(device):
-    """A destination with 100,000 sources (and one with 3,000): the push sum is
-    sequential over 1e5 equal-order terms, where a tree sum would differ."""
-    m = 100_002
+    """A destination with 140,000 sources (more than 2^17: the binade scan) and
+    one with 3,000 (the one-lane chain): each push sum is sequential over
+    equal-order terms, where a tree sum would differ."""
+    m = 140_002
     leaves = np.arange(2, m, dtype=np.int32)
     src = np.concatenate([leaves, leaves[:3000], np.zeros(m - 2, np.int32)])
     dst = np.concatenate([np.zeros(m - 2, np.int32), np.ones(3000, np.int32), leaves])
