@@ -48,6 +48,13 @@ struct Tile {
 // than exact_len is instead one whole-row hub tile (summed in edge order by one
 // workgroup); the longest of them come first, the rest are spread evenly
 // among the row tiles.
+// Whole-row exact hubs (exact_hub_kernel): rows of up to kScanRow edges are
+// added by one lane from LDS chunks of kHubChunk edges, each chunk stored
+// sorted by source (the plan's tile sort); longer rows by the workgroup's
+// binade scan, stored in edge order.
+constexpr int kHubChunk = 1536;    // 192 gathering lanes x 8 edges
+constexpr int kScanRow = 1 << 17;
+
 struct Schedule {
   std::vector<Tile> tiles;
   std::vector<int32_t> hub_row, hub_slot, hub_nseg;

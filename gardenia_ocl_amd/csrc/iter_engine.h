@@ -109,7 +109,7 @@ struct IterEngine {
         if (rc) return rc;
         dst = col_sorted.as<int32_t>();
       }
-      // exact hub rows stay in edge order: one lane adds them as they are read
+      // exact hub rows: sorted in the chunks their kernel adds, or kept in order
       rc = build_tile_sort(sched, kBlock * ipt, col, dst, pos.as<uint16_t>(), stream,
                            exact_len > 0);
       if (rc) return rc;
@@ -201,7 +201,7 @@ struct IterEngine {
     }
     if (n_hub_tiles) {
       if constexpr (P::kExactHubs)
-        hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT>), dim3(n_hub_tiles), dim3(kBlock), 0,
+        hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED>), dim3(n_hub_tiles), dim3(kBlock), 0,
                            stream, a, it);
     }
     if (n_tiles > 0) {

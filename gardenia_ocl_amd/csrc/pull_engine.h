@@ -589,7 +589,7 @@ __device__ float exact_scan_chunk(const float* __restrict__ val, int nc, bool he
 // sequential FP32 loop starting from s.  All threads return the same value.
 // `val` holds kBlock*IPT + kBlock floats (the padded layout exact_pos).
 template <int IPT, bool NT, bool SORTED, class G>
-__device__ float exact_seq_sum(const int32_t* __restrict__ col, const uint16_t* __restrict__ pos,
+__device__ __forceinline__ float exact_seq_sum(const int32_t* __restrict__ col, const uint16_t* __restrict__ pos,
                                int e_max, int e0, int ne, const G& g, float s,
                                float* __restrict__ val, ExactLds& x) {
   constexpr int C = kBlock * IPT;
@@ -746,6 +746,77 @@ __device__ float long_row_sum(const int32_t* __restrict__ col, const uint16_t* _
   return *s_out;
 }
 
+
+// s0 + the gathered values of edges [e0, e0+ne) in edge order, rounded as the
+// sequential FP32 loop, with the workgroup split by role: lane 0 of wave 0
+// adds chunk c (kHubChunk entries, LDS) while waves 1..3 gather chunk c+1
+// into the other buffer (their column loads of chunk c+2 already in flight),
+// so no gather waits on the chain and the chain wave issues nothing else.
+// SORTED: each chunk is stored sorted by source and pos[e] is the edge's
+// offset within its chunk; values go back to edge order in LDS.  Buffers:
+// two 16-byte aligned arrays of kHubChunk floats, one float for the sum.
+// All threads return the sum.
+template <int kQ, bool SORTED, class G>
+__device__ float hub_chain_sum(const int32_t* __restrict__ col, const uint16_t* __restrict__ pos,
+                               int e_max, int e0, int ne, const G& g, float s0,
+                               float* __restrict__ buf0, float* __restrict__ buf1,
+                               float* __restrict__ s_out) {
+  constexpr int kProd = kBlock - kWave;  // gathering lanes
+  constexpr int PPT = kHubChunk / kProd;
+  static_assert(PPT * kProd == kHubChunk && kHubChunk % (4 * kQ) == 0, "hub chunk shape");
+  const int tid = threadIdx.x;
+  const bool chain_wave = tid < kWave;
+  const int pt = tid - kWave;
+  const int nchunks = (ne + kHubChunk - 1) / kHubChunk;
+  int idx[PPT], p[PPT];
+  auto slot = [&](int c, int j) {  // clamped into the chunk
+    return min(pt + j * kProd, min(kHubChunk, ne - c * kHubChunk) - 1);
+  };
+  auto load_idx = [&](int c) {
+#pragma unroll
+    for (int j = 0; j < PPT; j++) idx[j] = col[min(e0 + c * kHubChunk + slot(c, j), e_max)];
+    if constexpr (SORTED) {
+#pragma unroll
+      for (int j = 0; j < PPT; j++) p[j] = (int)pos[min(e0 + c * kHubChunk + slot(c, j), e_max)];
+    }
+  };
+  auto produce = [&](int c, float* buf) {  // gathers chunk c (indices loaded), stores it
+    const int nc = min(kHubChunk, ne - c * kHubChunk);
+    float v[PPT];
+    int o[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+      o[j] = SORTED ? p[j] : slot(c, j);
+      v[j] = g.value(e0 + c * kHubChunk + o[j], idx[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; j++)
+      if (pt + j * kProd < nc) buf[o[j]] = v[j];
+  };
+  if (chain_wave) __builtin_amdgcn_s_setprio(3);
+  if (!chain_wave && nchunks > 0) {
+    load_idx(0);
+    produce(0, buf0);
+    if (nchunks > 1) load_idx(1);
+  }
+  __syncthreads();
+  float s = s0;
+  for (int c = 0; c < nchunks; c++) {
+    float* cur = (c & 1) ? buf1 : buf0;
+    float* nxt = (c & 1) ? buf0 : buf1;
+    if (chain_wave) {
+      if (tid == 0) s = lane_seq_sum<kQ>(s, cur, min(kHubChunk, ne - c * kHubChunk));
+    } else if (c + 1 < nchunks) {
+      produce(c + 1, nxt);
+      if (c + 2 < nchunks) load_idx(c + 2);  // in flight across the barrier
+    }
+    __syncthreads();
+  }
+  if (chain_wave) __builtin_amdgcn_s_setprio(0);
+  if (tid == 0) *s_out = s;
+  __syncthreads();
+  return *s_out;
+}
 
 }  // namespace gd
 

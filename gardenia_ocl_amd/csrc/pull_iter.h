@@ -154,23 +154,23 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
 // Whole-row hubs of an exact-hub policy: one workgroup per hub row adds the
 // row in edge order with the reference's sequential rounding, then runs the
 // row's epilogue.  Rows of up to kScanRow edges: one lane's dependent chain
-// (long_row_sum), the shortest path to the bits for a row alone on the GPU;
-// longer rows: the workgroup's binade scan (exact_seq_sum), whose cost per
-// term holds up beside the row tiles where a chain's doubles (their waves
-// take the CU's issue slots and LDS).  A kernel of its own, launched on a
-// second stream concurrently with the row-tile launch; its registers are kept
-// near a row tile's so that a hub workgroup fits on a CU beside four of them.
-// Hub rows are stored in edge order (not source-sorted).
-constexpr int kHubIPT = 8;         // chain chunks: kBlock*8 edges, two in LDS
-constexpr int kScanRow = 1 << 17;  // longer hub rows: binade scan
+// (hub_chain_sum, fed by the other three waves), the shortest path to the
+// bits for a row alone on the GPU; longer rows: the workgroup's binade scan
+// (exact_seq_sum), whose cost per term holds up beside the row tiles where a
+// chain's doubles (their waves take the CU's issue slots and LDS).  A kernel
+// of its own, launched before and beside the row-tile launch; its registers
+// are kept near a row tile's so that a hub workgroup fits on a CU beside four
+// of them.  With sorted tiles the chained rows are stored source-sorted per
+// kHubChunk edges (fewer L2 requests per gather), scanned rows in edge order.
+constexpr int kHubQ = 8;  // float4 LDS reads per step of a hub chain
 
-template <class P, int IPT, int NT>
+template <class P, int IPT, int NT, bool SORTED>
 __global__ __launch_bounds__(kBlock) void exact_hub_kernel(const typename P::Args a,
                                                            const IterArgs it) {
   if (it.st->done) return;
   const int odd = it.st->iter & 1;
-  constexpr int C = kBlock * kHubIPT;
-  constexpr int kBuf = (2 * C > kBlock * IPT + kBlock ? 2 * C : kBlock * IPT + kBlock) + 4;
+  constexpr int kBuf = (2 * kHubChunk > kBlock * IPT + kBlock ? 2 * kHubChunk
+                                                              : kBlock * IPT + kBlock) + 4;
   __shared__ __align__(16) float buf[kBuf];
   __shared__ double s_err[kWaves];
   __shared__ ExactLds xs;
@@ -184,8 +184,8 @@ __global__ __launch_bounds__(kBlock) void exact_hub_kernel(const typename P::Arg
     sum = exact_seq_sum<IPT, (NT > 0), false>(it.col, nullptr, it.e_max, t.z, ne, g, e.init(pre),
                                               buf, xs);
   else
-    sum = long_row_sum<kHubIPT>(it.col, nullptr, it.e_max, t.z, ne, g, e.init(pre), buf, buf + C,
-                                buf + 2 * C);
+    sum = hub_chain_sum<kHubQ, SORTED>(it.col, it.pos, it.e_max, t.z, ne, g, e.init(pre), buf,
+                                       buf + kHubChunk, buf + 2 * kHubChunk);
   double err = 0.0;
   if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
   const double tot = block_sum<kWaves>(err, s_err);

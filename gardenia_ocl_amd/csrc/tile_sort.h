@@ -24,11 +24,12 @@ constexpr int kSortMax = 4096;  // longest sorted segment (one LDS bitonic sort)
 // (hub segments).  col_sorted[e] / pos[e] receive, for the i-th edge of a
 // segment in source order (ties kept in edge order), its source and its
 // offset in the segment.  col and col_sorted may be the same array.
-// hubs_in_edge_order (a schedule built with exact_len > 0, whose hub tiles
-// are whole rows) leaves the hub tiles' edges in edge order -- their kernel
-// reads them unsorted: col_sorted = col and pos = offset there.
+// exact_hubs (a schedule built with exact_len > 0, whose hub tiles are whole
+// rows): hub rows of up to kScanRow edges are sorted in chunks of kHubChunk
+// edges (the chunks their kernel adds), longer ones stay in edge order
+// (col_sorted = col and pos = offset there).
 int build_tile_sort(const Schedule& s, int chunk, const int32_t* d_col, int32_t* d_col_sorted,
-                    uint16_t* d_pos, hipStream_t stream, bool hubs_in_edge_order = false);
+                    uint16_t* d_pos, hipStream_t stream, bool exact_hubs = false);
 
 }  // namespace gd
 

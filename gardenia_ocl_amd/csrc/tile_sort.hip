@@ -60,7 +60,7 @@ __global__ __launch_bounds__(kSortBlock) void segment_sort_kernel(const int2* __
 }  // namespace
 
 int build_tile_sort(const Schedule& s, int chunk, const int32_t* d_col, int32_t* d_col_sorted,
-                    uint16_t* d_pos, hipStream_t stream, bool hubs_in_edge_order) {
+                    uint16_t* d_pos, hipStream_t stream, bool exact_hubs) {
   if (chunk <= 0 || chunk > kSortMax) {
     set_error("build_tile_sort: chunk must be in [1, 4096]");
     return GD_ERR_INVALID;
@@ -68,9 +68,11 @@ int build_tile_sort(const Schedule& s, int chunk, const int32_t* d_col, int32_t*
   std::vector<int2> segs;
   segs.reserve(s.tiles.size() + 16);
   for (const Tile& t : s.tiles) {
-    const bool keep = hubs_in_edge_order && t.y < 0;
-    for (int32_t e = t.z; e < t.w; e += chunk) {
-      const int32_t n = std::min(chunk, t.w - e);
+    const bool hub = exact_hubs && t.y < 0;
+    const bool keep = hub && t.w - t.z > kScanRow;
+    const int seg = (hub && !keep) ? kHubChunk : chunk;
+    for (int32_t e = t.z; e < t.w; e += seg) {
+      const int32_t n = std::min(seg, t.w - e);
       if (n > kSortMax) {
         set_error("build_tile_sort: a tile is longer than 4096 edges");
         return GD_ERR_INVALID;
