@@ -29,8 +29,6 @@ namespace gd {
 namespace {
 
 constexpr int kMgIPT = 16;  // as the single-graph plan: same tiles, same per-row sums
-constexpr int kMgHubIPT = 8;  // hub rows: LDS chunks of kBlock*8 edges, two in flight
-constexpr int kMgChunk = kBlock * kMgHubIPT;
 
 struct GraphState {
   int iter;
@@ -68,7 +66,7 @@ __global__ __launch_bounds__(kBlock) void mg_pull_kernel(const MgArgs a) {
   if (a.gs[g].done) return;
   const int odd = a.gs[g].iter & 1;
   // a row tile's LDS, or a hub row's two chunk buffers + the broadcast sum
-  static_assert(sizeof(RowTileLds<kMgIPT>) >= (2 * kMgChunk + 1) * sizeof(float), "LDS union");
+  static_assert(sizeof(RowTileLds<kMgIPT>) >= (2 * kHubChunk + 1) * sizeof(float), "LDS union");
   __shared__ __align__(16) unsigned char smem[sizeof(RowTileLds<kMgIPT>)];
   RowTileLds<kMgIPT>& lds = *reinterpret_cast<RowTileLds<kMgIPT>*>(smem);
   __shared__ double s_err[kWaves];
@@ -81,8 +79,8 @@ __global__ __launch_bounds__(kBlock) void mg_pull_kernel(const MgArgs a) {
   } else {  // whole hub row in the reference's order, as the single-graph plan
     const auto pre = e.pre(t.x);
     float* const buf = reinterpret_cast<float*>(smem);
-    const float sum = long_row_sum<kMgHubIPT>(a.col, nullptr, a.e_max, t.z, t.w - t.z, gat,
-                                              e.init(pre), buf, buf + kMgChunk, buf + 2 * kMgChunk);
+    const float sum = hub_chain_sum<8, false>(a.col, nullptr, a.e_max, t.z, t.w - t.z, gat,
+                                              e.init(pre), buf, buf + kHubChunk, buf + 2 * kHubChunk);
     if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
   }
   const double tot = block_sum<kWaves>(err, s_err);

@@ -625,12 +625,11 @@ __device__ __forceinline__ float exact_seq_sum(const int32_t* __restrict__ col, 
 // The reference adds a row's terms one after the other in FP32
 // (src/pr/omp_base.cc:34-37, src/spmv/omp_base.cc:21-24); for a row longer
 // than a tile the fastest way to the same bits is that dependent chain
-// itself, on one lane, with the rest of the workgroup keeping it fed: the
-// gathers of chunk c+1 and the index loads of chunk c+2 are in flight while
-// lane 0 adds chunk c from LDS.  Measured on the MI355X the chain costs
-// 2.8-3.4 ns per term (a dependent v_add_f32 every 4 cycles when the terms
-// sit in registers; the LDS reads feeding it take the rest), independent of
-// the terms' signs, ties or magnitudes.
+// itself, on one lane, with the rest of the workgroup keeping it fed
+// (hub_chain_sum).  Measured on the MI355X the chain costs 2.8-3.4 ns per
+// term (a dependent v_add_f32 every ~4.8 cycles when the terms sit in
+// registers; the LDS reads feeding it take the rest), independent of the
+// terms' signs, ties or magnitudes.
 
 // s + v[0] + ... + v[n-1] in order with FP32 rounding, by the calling lane,
 // from 16-byte aligned LDS: 4*kQ values per step, the next step's LDS reads in
@@ -673,79 +672,6 @@ __device__ __forceinline__ float lane_seq_sum(float s, const float* __restrict__
   for (int i = nb * 4 * kQ; i < n; i++) s += v[i];
   return s;
 }
-
-// s0 + the gathered values of edges [e0, e0+ne) in edge order, rounded as the
-// sequential FP32 loop.  Lane 0 adds chunk c (kBlock*IPT entries, LDS) while
-// the workgroup's gathers of chunk c+1 and column loads of chunk c+2 are in
-// flight (two 16-byte aligned LDS buffers of kBlock*IPT floats, and one float
-// for the broadcast).  All threads return the sum.
-// SORTED: each chunk of the row is stored sorted by source (tile_sort.h) and
-// pos[e] is the edge's offset within its chunk; values go back to edge order
-// in LDS, and g.value sees the edge's own (unsorted) index.
-template <int IPT, int kQ = 8, bool SORTED = false, class G>
-__device__ float long_row_sum(const int32_t* __restrict__ col, const uint16_t* __restrict__ pos,
-                              int e_max, int e0, int ne, const G& g, float s0,
-                              float* __restrict__ buf0, float* __restrict__ buf1,
-                              float* __restrict__ s_out) {
-  constexpr int C = kBlock * IPT;
-  const int tid = threadIdx.x;
-  const int nchunks = (ne + C - 1) / C;
-  auto slot = [&](int c, int j) {  // offset of lane tid, slot j in chunk c (clamped into the row)
-    return min(tid + j * kBlock, min(C, ne - c * C) - 1);
-  };
-  int idx[IPT], p[IPT], ps[IPT];
-  float v[IPT];
-  auto load_idx = [&](int c) {
-#pragma unroll
-    for (int j = 0; j < IPT; j++) idx[j] = col[min(e0 + c * C + slot(c, j), e_max)];
-    if constexpr (SORTED) {
-#pragma unroll
-      for (int j = 0; j < IPT; j++) p[j] = (int)pos[min(e0 + c * C + slot(c, j), e_max)];
-    }
-  };
-  auto gather = [&](int c) {
-#pragma unroll
-    for (int j = 0; j < IPT; j++) {
-      const int o = SORTED ? p[j] : slot(c, j);
-      v[j] = g.value(e0 + c * C + o, idx[j]);
-      ps[j] = o;
-    }
-  };
-  auto store = [&](int c, float* buf) {
-    const int nc = min(C, ne - c * C);
-#pragma unroll
-    for (int j = 0; j < IPT; j++)
-      if (tid + j * kBlock < nc) buf[ps[j]] = v[j];
-  };
-  // The chain's wavefront issues first on its SIMD: the row tiles that share
-  // the CU would otherwise take most of its issue slots and stretch every
-  // dependent add.
-  const bool chain_wave = tid < kWave;
-  if (chain_wave) __builtin_amdgcn_s_setprio(3);
-  float s = s0;
-  if (nchunks > 0) {
-    load_idx(0);
-    gather(0);
-    if (nchunks > 1) load_idx(1);
-    store(0, buf0);
-    __syncthreads();
-  }
-  for (int c = 0; c < nchunks; c++) {
-    float* cur = (c & 1) ? buf1 : buf0;
-    float* nxt = (c & 1) ? buf0 : buf1;
-    const bool more = c + 1 < nchunks;
-    if (more) gather(c + 1);               // indices arrived during chunk c-1's adds
-    if (c + 2 < nchunks) load_idx(c + 2);  // in flight during this chunk's adds
-    if (tid == 0) s = lane_seq_sum<kQ>(s, cur, min(C, ne - c * C));
-    if (more) store(c + 1, nxt);
-    __syncthreads();
-  }
-  if (chain_wave) __builtin_amdgcn_s_setprio(0);
-  if (tid == 0) *s_out = s;
-  __syncthreads();
-  return *s_out;
-}
-
 
 // s0 + the gathered values of edges [e0, e0+ne) in edge order, rounded as the
 // sequential FP32 loop, with the workgroup split by role: lane 0 of wave 0

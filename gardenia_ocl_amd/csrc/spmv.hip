@@ -6,7 +6,7 @@
 // y[row] in CSR order, which is exactly the reference's arithmetic; so are
 // the longer rows of a tile (the owner thread's chain just runs longer).
 // Rows longer than a tile get a workgroup each that feeds one lane's
-// sequential chain in CSR order (long_row_sum), so every row is the
+// sequential chain in CSR order (hub_chain_sum), so every row is the
 // reference's sum bit for bit.
 
 #include <hip/hip_runtime.h>
@@ -43,7 +43,7 @@ struct SpEpilogue {
 };
 
 // Tile mode, one launch: workgroups [0, n_long) each add one row longer than
-// a tile (2048 nonzeros) in CSR order on one lane (long_row_sum: products of
+// a tile (2048 nonzeros) in CSR order on one lane (hub_chain_sum: products of
 // both signs), the rest one LDS row tile each (every row summed by its owner
 // thread in CSR order).  Every row is the reference's sum bit for bit.  LONG =
 // false (no long rows) leaves the long-row code, and its registers, out.
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kBlock) void spmv_pull_kernel(
     const int32_t* __restrict__ Ap, const int32_t* __restrict__ Aj, int e_max,
     const float* __restrict__ Ax, const float* __restrict__ x, float* __restrict__ y) {
   // a row tile's LDS, or a long row's two chunk buffers + the broadcast sum
-  static_assert(sizeof(RowTileLds<kSpIPT>) >= (2 * kSpTileNnz + 1) * sizeof(float), "LDS union");
+  static_assert(sizeof(RowTileLds<kSpIPT>) >= (2 * kHubChunk + 1) * sizeof(float), "LDS union");
   __shared__ __align__(16) unsigned char smem[sizeof(RowTileLds<kSpIPT>)];
   RowTileLds<kSpIPT>& lds = *reinterpret_cast<RowTileLds<kSpIPT>*>(smem);
   const SpGather g{x, Ax};
@@ -61,8 +61,8 @@ __global__ __launch_bounds__(kBlock) void spmv_pull_kernel(
   if (LONG && b < n_long) {
     float* const buf = reinterpret_cast<float*>(smem);
     const Tile t = long_tiles[b];
-    const float s = long_row_sum<kSpIPT>(Aj, nullptr, e_max, t.z, t.w - t.z, g, y[t.x], buf,
-                                         buf + kSpTileNnz, buf + 2 * kSpTileNnz);
+    const float s = hub_chain_sum<8, false>(Aj, nullptr, e_max, t.z, t.w - t.z, g, y[t.x], buf,
+                                            buf + kHubChunk, buf + 2 * kHubChunk);
     if (threadIdx.x == 0) y[t.x] = s;
   } else {
     const Tile t = tiles[b - n_long];

@@ -88,17 +88,16 @@ struct SymgsPolicy {
   }
 };
 
-constexpr int kSgChunk = kBlock * kSgIPT;  // entries of a long row per LDS chunk
 
 // One color in one launch: workgroups [0, n_hub_tiles) add one long row each
-// (long_row_sum), the rest one LDS row tile each (owner thread per row, in
+// (hub_chain_sum), the rest one LDS row tile each (owner thread per row, in
 // order).  Long rows first: they take longest.  LONG = false (colors without
 // long rows) leaves the long-row code, and its registers, out of the kernel.
 template <bool LONG>
 __global__ __launch_bounds__(kBlock) void symgs_color_kernel(const SymgsPolicy::Args a,
                                                              const IterArgs it) {
   // a row tile's LDS, or a long row's two chunk buffers + the broadcast sum
-  static_assert(sizeof(RowTileLds<kSgIPT>) >= (2 * kSgChunk + 1) * sizeof(float), "LDS union");
+  static_assert(sizeof(RowTileLds<kSgIPT>) >= (2 * kHubChunk + 1) * sizeof(float), "LDS union");
   __shared__ __align__(16) unsigned char smem[sizeof(RowTileLds<kSgIPT>)];
   RowTileLds<kSgIPT>& lds = *reinterpret_cast<RowTileLds<kSgIPT>*>(smem);
   float* const buf = reinterpret_cast<float*>(smem);
@@ -108,8 +107,8 @@ __global__ __launch_bounds__(kBlock) void symgs_color_kernel(const SymgsPolicy::
   if (LONG && b < it.n_hub_tiles) {
     const Tile t = it.hub_tiles[b];
     const auto pre = e.pre(t.x);
-    const float rsum = long_row_sum<kSgIPT>(it.col, nullptr, it.e_max, t.z, t.w - t.z, g, 0.0f, buf,
-                                            buf + kSgChunk, buf + 2 * kSgChunk);
+    const float rsum = hub_chain_sum<8, false>(it.col, nullptr, it.e_max, t.z, t.w - t.z, g, 0.0f,
+                                               buf, buf + kHubChunk, buf + 2 * kHubChunk);
     if (threadIdx.x == 0) (void)e.finish(t.x, rsum, pre);
   } else {
     const Tile t = it.tiles[b - it.n_hub_tiles];

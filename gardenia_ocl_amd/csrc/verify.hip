@@ -8,7 +8,7 @@
 // order.  Here the out-CSR is transposed with a stable sort (each
 // destination's sources in exactly that order) and pulled by the iteration
 // engine with every row of more than 64 sources summed in order with
-// sequential FP32 rounding by one lane of its own workgroup (long_row_sum),
+// sequential FP32 rounding by one lane of its own workgroup (hub_chain_sum),
 // shorter rows by their owner thread in order: every per-vertex term is the reference's bit for bit; only
 // the FP64 sum of the terms runs in a different (fixed) order.
 
