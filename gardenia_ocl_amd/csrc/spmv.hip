@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void spmv_pull_kernel(
   if (LONG && b < n_long) {
     float* const buf = reinterpret_cast<float*>(smem);
     const Tile t = long_tiles[b];
-    const float s = hub_chain_sum<8, false>(Aj, nullptr, e_max, t.z, t.w - t.z, g, y[t.x], buf,
+    const float s = hub_chain_sum<16, false>(Aj, nullptr, e_max, t.z, t.w - t.z, g, y[t.x], buf,
                                             buf + kHubChunk, buf + 2 * kHubChunk);
     if (threadIdx.x == 0) y[t.x] = s;
   } else {

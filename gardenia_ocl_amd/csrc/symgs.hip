@@ -107,7 +107,8 @@ __global__ __launch_bounds__(kBlock) void symgs_color_kernel(const SymgsPolicy::
   if (LONG && b < it.n_hub_tiles) {
     const Tile t = it.hub_tiles[b];
     const auto pre = e.pre(t.x);
-    const float rsum = hub_chain_sum<8, false>(it.col, nullptr, it.e_max, t.z, t.w - t.z, g, 0.0f,
+    // 16 float4 LDS reads ahead: the chains bound these sweeps (8 ahead: 9% slower)
+    const float rsum = hub_chain_sum<16, false>(it.col, nullptr, it.e_max, t.z, t.w - t.z, g, 0.0f,
                                                buf, buf + kHubChunk, buf + 2 * kHubChunk);
     if (threadIdx.x == 0) (void)e.finish(t.x, rsum, pre);
   } else {
