@@ -34,6 +34,10 @@ struct IterEngine {
   DeviceBuffer dummy_col, col_sorted, pos;
   Schedule sched;
   int n_tiles = 0, n_hubs = 0, n_hub_tiles = 0, fin_blocks = 1;
+  // Hub chains read 16 float4 ahead (176 VGPRs) when the longest chained hub
+  // row, not the row tiles, bounds the pull; else 8 ahead (132 VGPRs), which
+  // lets a hub workgroup share a CU with four row-tile waves per SIMD.
+  bool deep_hub_chain = false;
   DeviceBuffer tiles, hub_tiles, hub_row, hub_slot, hub_nseg, hub_part, err_part, fin_part, state,
       errors;
   int errors_cap = 0;
@@ -123,6 +127,15 @@ struct IterEngine {
     sched.tiles = row_tiles;
     n_tiles = (int)row_tiles.size();
     n_hub_tiles = (int)hubs.size();
+    {
+      // Estimates measured on the MI355X: ~3.5 ns per chained term, row tiles
+      // at ~5 ps per edge (~200 G gathered edges/s).
+      int64_t chain = 0, tile_edges = 0;
+      for (const Tile& t : hubs)
+        if (t.w - t.z <= kScanRow) chain = std::max<int64_t>(chain, t.w - t.z);
+      for (const Tile& t : row_tiles) tile_edges += t.w - t.z;
+      deep_hub_chain = chain * 3500 > tile_edges * 5;
+    }
     if (n_hub_tiles) {
       rc = hub_tiles.alloc(sizeof(Tile) * n_hub_tiles);
       if (rc) return rc;
@@ -200,9 +213,14 @@ struct IterEngine {
       ts = side;
     }
     if (n_hub_tiles) {
-      if constexpr (P::kExactHubs)
-        hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED>), dim3(n_hub_tiles), dim3(kBlock), 0,
-                           stream, a, it);
+      if constexpr (P::kExactHubs) {
+        if (deep_hub_chain)
+          hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED, 16>), dim3(n_hub_tiles),
+                             dim3(kBlock), 0, stream, a, it);
+        else
+          hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED, 8>), dim3(n_hub_tiles),
+                             dim3(kBlock), 0, stream, a, it);
+      }
     }
     if (n_tiles > 0) {
       if (persistent)

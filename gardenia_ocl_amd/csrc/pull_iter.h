@@ -162,9 +162,8 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
 // are kept near a row tile's so that a hub workgroup fits on a CU beside four
 // of them.  With sorted tiles the chained rows are stored source-sorted per
 // kHubChunk edges (fewer L2 requests per gather), scanned rows in edge order.
-constexpr int kHubQ = 8;  // float4 LDS reads per step of a hub chain
-
-template <class P, int IPT, int NT, bool SORTED>
+// KQ: float4 LDS reads ahead in a hub chain (IterEngine::deep_hub_chain).
+template <class P, int IPT, int NT, bool SORTED, int KQ>
 __global__ __launch_bounds__(kBlock) void exact_hub_kernel(const typename P::Args a,
                                                            const IterArgs it) {
   if (it.st->done) return;
@@ -184,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void exact_hub_kernel(const typename P::Arg
     sum = exact_seq_sum<IPT, (NT > 0), false>(it.col, nullptr, it.e_max, t.z, ne, g, e.init(pre),
                                               buf, xs);
   else
-    sum = hub_chain_sum<kHubQ, SORTED>(it.col, it.pos, it.e_max, t.z, ne, g, e.init(pre), buf,
+    sum = hub_chain_sum<KQ, SORTED>(it.col, it.pos, it.e_max, t.z, ne, g, e.init(pre), buf,
                                        buf + kHubChunk, buf + 2 * kHubChunk);
   double err = 0.0;
   if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
