@@ -214,12 +214,13 @@ def _star(n_leaves, hub=0):
 
 
 @pytest.mark.parametrize("n_leaves", [1, 63, 64, 65, 2047, 2048, 2049, 4096, 4097, 100_000,
-                                      140_000])
+                                      131_072, 131_073, 140_000])
 def test_pr_star_hub_boundaries(device, n_leaves):
     """Hub of n in-edges, all addends equal: the reference's sequential FP32 sum
     drifts (its L1 change plateaus at ~3e-6 for n=65, ~7e-5 for n=2048 and the
     hub score is 3.5e-5 off the FP64 fixed point).  Hubs of <= 64 in-edges or
-    longer than a tile are summed in the reference's order: bit-identical
+    longer than a tile are summed in the reference's order (one lane's chain
+    up to 2^17 = 131,072 in-edges, the binade scan beyond): bit-identical
     results.  In
     between the tree sum is used, and the bar is the FP64 solution: the GPU
     must be at least as close to it as the reference."""
