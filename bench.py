@@ -87,6 +87,19 @@ def cpu_baseline_extras(spmv_log2, seed):
     ms = (time.perf_counter() - t0) * 1e3
     out["symgs"] = {"value": round(ms, 1), "unit": "ms per symmetric sweep", "cores": 1,
                     "kind": "port", "sample": "stencil27 128^3, 8 colors, one sweep"}
+    del c, colors, idx, co, b
+    # the small-graph workload: RMAT-16 to L1 tol 1e-4 (the OpenMP pull loop)
+    o16, i16 = graph.rmat(16, 16, seed=seed)
+    best, iters = float("inf"), 0
+    for _ in range(10):
+        t0 = time.perf_counter()
+        _, iters, _ = oracle.pr(i16.m, i16.row_offsets, i16.column_indices, o16.degree,
+                                tol=1e-4, max_iter=100)
+        best = min(best, time.perf_counter() - t0)
+    out["pr_rmat16"] = {"value": round(best * 1e3, 3), "unit": "ms per solve",
+                        "us_per_iteration": round(best * 1e6 / max(1, iters), 2),
+                        "iterations": iters, "cores": oracle.num_threads(), "kind": "port",
+                        "sample": "RMAT-16 pull PageRank to L1 tol 1e-4, best of 10 solves"}
     return out
 
 
