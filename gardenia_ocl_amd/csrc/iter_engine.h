@@ -34,6 +34,7 @@ struct IterEngine {
   DeviceBuffer dummy_col, col_sorted, pos;
   Schedule sched;
   int n_tiles = 0, n_hubs = 0, n_hub_tiles = 0, fin_blocks = 1;
+  bool fused = false;  // the pull launches finish each iteration (no finalize launch)
   // Hub chains read 16 float4 ahead (176 VGPRs) when the longest chained hub
   // row, not the row tiles, bounds the pull; else 8 ahead (132 VGPRs), which
   // lets a hub workgroup share a CU with four row-tile waves per SIMD.
@@ -182,13 +183,19 @@ struct IterEngine {
     }
     fin_blocks = std::max(
         1, std::min(kFinBlocks, std::max(pull_grid + n_hub_tiles, n_hubs) / kBlock));
+    // small iterations end in their last pull workgroup (fused_finish): no hub
+    // segments to close, one finalize workgroup's worth of partials
+    const int parts = (n_tiles > 0 ? pull_grid : 0) + n_hub_tiles;
+    fused = !persistent && (exact_len > 0 || n_hubs == 0) && parts > 0 && parts < 2 * kBlock;
     return GD_OK;
   }
 
   IterArgs iter_args() {
+    const int parts = (n_tiles > 0 ? pull_grid : 0) + n_hub_tiles;
     return IterArgs{tiles.as<Tile>(), n_tiles, hub_tiles.as<Tile>(), n_hub_tiles,
                     n_tiles > 0 ? pull_grid : 0, off, col, e_max, pos.as<uint16_t>(),
-                    hub_part.as<float>(), err_part.as<double>(), state.as<IterState>()};
+                    hub_part.as<float>(), err_part.as<double>(), state.as<IterState>(),
+                    fused ? parts : 0, errors.as<double>()};
   }
 
   // exact hubs finish their rows in the pull launch: no hub epilogues here
@@ -205,6 +212,12 @@ struct IterEngine {
     // (longest row first) are placed before the row tiles fill the CUs; the
     // row tiles fork onto the side stream (after the same prior work) and
     // join back.
+    if (fused) {  // small iteration: one launch (hubs first, then row tiles)
+      hipLaunchKernelGGL((pull_small_kernel<P, IPT, NT, SORTED>),
+                         dim3(n_hub_tiles + (n_tiles > 0 ? pull_grid : 0)), dim3(kBlock), 0,
+                         stream, a, it);
+      return;
+    }
     const bool fork = n_hub_tiles > 0 && n_tiles > 0;
     hipStream_t ts = stream;
     if (fork) {
@@ -302,7 +315,9 @@ struct IterEngine {
     stream = cap_stream;  // launch_pull forks from / joins to `stream`
     for (int i = 0; i < n; i++) {
       launch_pull<P>(a, it);
-      hipLaunchKernelGGL(finalize_kernel<P>, dim3(fin_blocks), dim3(kBlock), 0, stream, a, it, f);
+      if (!fused)
+        hipLaunchKernelGGL(finalize_kernel<P>, dim3(fin_blocks), dim3(kBlock), 0, stream, a, it,
+                           f);
     }
     stream = user;
     hipError_t e = hipStreamEndCapture(cap_stream, &g.graph);
@@ -357,8 +372,9 @@ struct IterEngine {
           if (profile) GD_HIP_TRY(hipEventRecord(ev[2 * k], stream));
           launch_pull<P>(a, it);
           if (profile) GD_HIP_TRY(hipEventRecord(ev[2 * k + 1], stream));
-          hipLaunchKernelGGL(finalize_kernel<P>, dim3(fin_blocks), dim3(kBlock), 0, stream, a,
-                             it, f);
+          if (!fused)
+            hipLaunchKernelGGL(finalize_kernel<P>, dim3(fin_blocks), dim3(kBlock), 0, stream, a,
+                               it, f);
         }
       }
       GD_HIP_TRY(hipGetLastError());

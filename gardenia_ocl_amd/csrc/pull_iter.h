@@ -52,6 +52,10 @@ struct IterArgs {
   float* __restrict__ hub_part;
   double* __restrict__ err_part;
   IterState* st;
+  // > 0: small iteration finished by its last pull workgroup (fused_finish)
+  // over this many L1 partials, instead of a finalize launch
+  int fuse_parts;
+  double* __restrict__ errors;  // per-iteration L1 (fused_finish)
 };
 
 struct FinArgs {
@@ -72,16 +76,51 @@ static __global__ void iter_state_init_kernel(IterState* st, int max_iter, doubl
   st->tol = tol;
 }
 
+// Finish of a small iteration (it.fuse_parts > 0: fewer than 2*kBlock L1
+// partials, no hub segments to close): every workgroup of the iteration's
+// pull launches (row tiles and exact hubs) takes a ticket after writing its
+// partial, and the last to arrive adds the partials -- in the grouping
+// finalize_kernel uses with one workgroup, so the L1 values are the same bits
+// -- and advances the iteration state.  Saves the finalize launch and its
+// dependency, which dominate the iteration time of small graphs.
+__device__ __forceinline__ void fused_finish(const IterArgs& it, double* s_err, int* s_last) {
+  IterState* st = it.st;
+  if (threadIdx.x == 0) {
+    __threadfence();  // release this workgroup's err_part before taking a ticket
+    const unsigned arrived = atomicAdd(&st->ticket, 1u);
+    *s_last = (arrived == (unsigned)it.fuse_parts - 1);
+  }
+  __syncthreads();
+  if (!*s_last) return;
+  __threadfence();  // acquire: every partial of the iteration is visible
+  const int n = it.fuse_parts;
+  double err = 0.0;
+  for (int i = threadIdx.x; i < n; i += 4 * kBlock) {
+    const double a0 = it.err_part[i];
+    const double a1 = (i + kBlock < n) ? it.err_part[i + kBlock] : 0.0;
+    const double a2 = (i + 2 * kBlock < n) ? it.err_part[i + 2 * kBlock] : 0.0;
+    const double a3 = (i + 3 * kBlock < n) ? it.err_part[i + 3 * kBlock] : 0.0;
+    err += ((a0 + a1) + a2) + a3;
+  }
+  const double total = block_sum<kWaves>(err, s_err);
+  if (threadIdx.x == 0) {
+    const int i = st->iter;
+    it.errors[i] = total;
+    st->iter = i + 1;
+    st->ticket = 0;
+    if (total < st->tol || i + 1 >= st->max_iter) st->done = 1;
+  }
+}
+
 // One workgroup per tile.  SORTED: each tile's edges are stored in source
 // order with their tile offsets (tile_sort.h); sums are still in CSR order.
+// tile_body: row tile b of the iteration (the caller checked st->done).
 template <class P, int IPT, int NT, bool SORTED>
-__global__ __launch_bounds__(kBlock) void pull_tile_kernel(const typename P::Args a,
-                                                           const IterArgs it) {
-  if (it.st->done) return;
+__device__ __forceinline__ void tile_body(const typename P::Args& a, const IterArgs& it, int b) {
   const int odd = it.st->iter & 1;
   __shared__ RowTileLds<IPT> lds;
   __shared__ double s_err[kWaves];
-  const Tile t = it.tiles[blockIdx.x];
+  const Tile t = it.tiles[b];
   const auto g = P::template gather<NT>(a, odd);
   double err = 0.0;
   if (t.y >= 0) {
@@ -93,7 +132,18 @@ __global__ __launch_bounds__(kBlock) void pull_tile_kernel(const typename P::Arg
     if (threadIdx.x == 0) it.hub_part[-t.y - 1] = p;
   }
   const double tot = block_sum<kWaves>(err, s_err);
-  if (threadIdx.x == 0) it.err_part[blockIdx.x] = tot;
+  if (threadIdx.x == 0) it.err_part[b] = tot;
+  if (it.fuse_parts > 0) {
+    __shared__ int s_last;
+    fused_finish(it, s_err, &s_last);
+  }
+}
+
+template <class P, int IPT, int NT, bool SORTED>
+__global__ __launch_bounds__(kBlock) void pull_tile_kernel(const typename P::Args a,
+                                                           const IterArgs it) {
+  if (it.st->done) return;
+  tile_body<P, IPT, NT, SORTED>(a, it, blockIdx.x);
 }
 
 // Persistent, software-pipelined form: workgroup b walks tiles b, b+G, ...;
@@ -163,17 +213,16 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
 // of them.  With sorted tiles the chained rows are stored source-sorted per
 // kHubChunk edges (fewer L2 requests per gather), scanned rows in edge order.
 // KQ: float4 LDS reads ahead in a hub chain (IterEngine::deep_hub_chain).
+// hub_body: exact hub h of the iteration (the caller checked st->done).
 template <class P, int IPT, int NT, bool SORTED, int KQ>
-__global__ __launch_bounds__(kBlock) void exact_hub_kernel(const typename P::Args a,
-                                                           const IterArgs it) {
-  if (it.st->done) return;
+__device__ __forceinline__ void hub_body(const typename P::Args& a, const IterArgs& it, int h) {
   const int odd = it.st->iter & 1;
   constexpr int kBuf = (2 * kHubChunk > kBlock * IPT + kBlock ? 2 * kHubChunk
                                                               : kBlock * IPT + kBlock) + 4;
   __shared__ __align__(16) float buf[kBuf];
   __shared__ double s_err[kWaves];
   __shared__ ExactLds xs;
-  const Tile t = it.hub_tiles[blockIdx.x];
+  const Tile t = it.hub_tiles[h];
   const auto g = P::template gather<NT>(a, odd);
   const auto e = P::template epilogue<NT>(a, odd);
   const auto pre = e.pre(t.x);
@@ -184,11 +233,40 @@ __global__ __launch_bounds__(kBlock) void exact_hub_kernel(const typename P::Arg
                                               buf, xs);
   else
     sum = hub_chain_sum<KQ, SORTED>(it.col, it.pos, it.e_max, t.z, ne, g, e.init(pre), buf,
-                                       buf + kHubChunk, buf + 2 * kHubChunk);
+                                    buf + kHubChunk, buf + 2 * kHubChunk);
   double err = 0.0;
   if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
   const double tot = block_sum<kWaves>(err, s_err);
-  if (threadIdx.x == 0) it.err_part[it.hub_part_base + blockIdx.x] = tot;
+  if (threadIdx.x == 0) it.err_part[it.hub_part_base + h] = tot;
+  if (it.fuse_parts > 0) {
+    __shared__ int s_last;
+    fused_finish(it, s_err, &s_last);
+  }
+}
+
+template <class P, int IPT, int NT, bool SORTED, int KQ>
+__global__ __launch_bounds__(kBlock) void exact_hub_kernel(const typename P::Args a,
+                                                           const IterArgs it) {
+  if (it.st->done) return;
+  hub_body<P, IPT, NT, SORTED, KQ>(a, it, blockIdx.x);
+}
+
+// Small iterations (it.fuse_parts > 0) in one launch: workgroups [0,
+// n_hub_tiles) take the exact hubs (longest first), the rest the row tiles,
+// and the last workgroup to finish closes the iteration (fused_finish) --
+// one kernel per iteration, no fork, join or finalize.  Registers are the hub
+// body's; with under 2*kBlock workgroups occupancy does not bound them.
+template <class P, int IPT, int NT, bool SORTED>
+__global__ __launch_bounds__(kBlock) void pull_small_kernel(const typename P::Args a,
+                                                            const IterArgs it) {
+  if (it.st->done) return;
+  if constexpr (P::kExactHubs) {
+    if ((int)blockIdx.x < it.n_hub_tiles) {
+      hub_body<P, IPT, NT, SORTED, 16>(a, it, blockIdx.x);
+      return;
+    }
+  }
+  tile_body<P, IPT, NT, SORTED>(a, it, blockIdx.x - it.n_hub_tiles);
 }
 
 // Hub-row epilogues (segment partials summed in segment order) and the L1
