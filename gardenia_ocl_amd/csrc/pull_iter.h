@@ -1,11 +1,14 @@
 // pull_iter.h -- device side of the fixed-point pull iteration, generic over a
 // policy (plain / personalised PageRank, weighted PageRank, its adjoint).
 //
-// One iteration = one pull launch over the tile schedule (gathers, per-row
-// epilogue, per-workgroup L1 partial) + one finalize launch (hub rows from
-// their segment partials, deterministic L1 reduction, convergence flag).  The
-// iteration counter and the done flag live on the device (IterState), so every
-// iteration's launches are identical; launches after `done` return at once.
+// One iteration = the pull launches over the tile schedule (row tiles, and
+// the exact hub rows beside them: gathers, per-row epilogue, per-workgroup L1
+// partial) + one finalize launch (hub rows from their segment partials,
+// deterministic L1 reduction, convergence flag); small iterations are one
+// launch whose last workgroup does the finalize's work (pull_small_kernel,
+// fused_finish).  The iteration counter and the done flag live on the device
+// (IterState), so every iteration's launches are identical; launches after
+// `done` return at once.
 //
 // A policy P provides
 //   struct P::Args                      kernel arguments (pointers, scalars)
