@@ -623,7 +623,7 @@ __device__ __forceinline__ float exact_seq_sum(const int32_t* __restrict__ col, 
 // Long rows summed in edge order by one lane.
 //
 // The reference adds a row's terms one after the other in FP32
-// (src/pr/omp_base.cc:34-37, src/spmv/omp_base.cc:21-24); for a row longer
+// (src/pr/omp_base.cc:34-37, src/spmv/omp_base.cc:21-26); for a row longer
 // than a tile the fastest way to the same bits is that dependent chain
 // itself, on one lane, with the rest of the workgroup keeping it fed
 // (hub_chain_sum).  Measured on the MI355X the chain costs 2.8-3.4 ns per

@@ -1,6 +1,6 @@
 // symgs.hip -- multicolor symmetric Gauss-Seidel sweep (the reference's
 // SymGSSolver, src/symgs/symgs.h:28; algorithm src/symgs/omp_base.cc:8-39 and
-// src/symgs/verifier.cc:48-76), bit-exact with the sequential sweep.
+// src/symgs/verifier.cc:50-77), bit-exact with the sequential sweep.
 //
 // Semantics.  For each color c = 0..C-1, then C-1..0: every row r listed in
 // indices[color_offsets[c] .. color_offsets[c+1]) gets
@@ -44,7 +44,7 @@ constexpr int kSgIPT = 8;  // edges per thread per row tile (tile = 2048 edges)
 struct SymgsGather {
   const float* __restrict__ x;
   const float* __restrict__ val;
-  // x[j] * Ax[jj] (src/symgs/verifier.cc:59): rounded here, added in order later
+  // x[j] * Ax[jj] (src/symgs/verifier.cc:61): rounded here, added in order later
   __device__ __forceinline__ float value(int e, int u) const { return x[u] * val[e]; }
 };
 
@@ -63,7 +63,7 @@ struct SymgsEpilogue {
     return Pre{diag[v], b[r], r};
   }
   __device__ __forceinline__ float init(const Pre&) const { return 0.0f; }
-  // if (diag != 0) x[inew] = (b[inew] - rsum) / diag   (verifier.cc:62)
+  // if (diag != 0) x[inew] = (b[inew] - rsum) / diag   (verifier.cc:63)
   __device__ __forceinline__ double finish(int /*v*/, float rsum, const Pre& p) const {
     if (p.d != 0.0f) x[p.r] = (p.bv - rsum) / p.d;
     return 0.0;

@@ -104,7 +104,7 @@ def PRVerifier(m: int, in_row_offsets, in_column_indices, out_row_offsets, out_c
                degree, scores_to_test, target_error=EPSILON, *, damping=K_DAMP):
     """PRVerifier(m, in_off, in_col, out_off, out_col, degree, scores, target_error)
     (src/pr/pr.h:24) with the residual computed on the GPU.  Prints
-    "Correct" or "Total Error: %f" as the reference does (verifier.cc:66-67)
+    "Correct" or "Total Error: %f" as the reference does (verifier.cc:67-68)
     and returns (correct, residual).  The reference's discarded serial
     re-run of the solver (verifier.cc:11-41) is not repeated; the in-CSR
     arguments are accepted for the signature and unused."""

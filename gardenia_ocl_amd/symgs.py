@@ -5,7 +5,7 @@ over the HIP library.
 arrays and x updated in place by one symmetric sweep (colors forward, then
 backward; src/symgs/omp_base.cc:27-42).  ``SymGSPlan`` is the device-resident
 form over torch tensors, for repeated sweeps.  Results are bit-exact with the
-reference's sequential sweep (src/symgs/verifier.cc:48-76) for a proper
+reference's sequential sweep (src/symgs/verifier.cc:50-77) for a proper
 coloring; ``graph.color_first_fit`` + ``graph.color_order`` give one.
 """
 from __future__ import annotations

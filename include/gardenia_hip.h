@@ -356,7 +356,7 @@ int gd_spmv_bwd_plan_destroy(gd_spmv_bwd_plan *plan);
  *   x[r] = (b[r] - sum_{j != r} x[j] * A[r][j]) / A[r][r]   unless A[r][r] == 0,
  * the sum in CSR order with sequential FP32 rounding and A[r][r] the row's last
  * diagonal entry: bit-exact with the reference's sequential sweep
- * (src/symgs/verifier.cc:48-76).  The coloring must be proper (no off-diagonal
+ * (src/symgs/verifier.cc:50-77).  The coloring must be proper (no off-diagonal
  * entry joins two rows of one color) and list each row at most once; the plan
  * checks both.  Device arrays: d_Ap[m+1], d_Aj/d_Ax[nnz], d_indices (the
  * entries color_offsets[0] .. color_offsets[n_colors]-1 are read); host

@@ -138,7 +138,7 @@ def ppr(m, in_off, in_col, degree, seed, *, damping=0.85, tol=1e-3, max_iter=30)
 
 
 def pr_residual(m, out_off, out_col, degree, scores, damping=0.85) -> float:
-    """PRVerifier's push residual (src/pr/verifier.cc:164-180)."""
+    """PRVerifier's push residual (src/pr/verifier.cc:50-68)."""
     out_off, out_col, degree = _i32(out_off), _i32(out_col), _i32(degree)
     scores = np.ascontiguousarray(scores, np.float32)
     return lib().orc_pr_residual(m, _p(out_off), _p(out_col), _p(degree), _p(scores), damping)
@@ -171,7 +171,7 @@ def spmv(m, Ap, Aj, Ax, x, y):
 
 
 def symgs(Ap, Aj, indices, Ax, x, b, color_offsets, sweeps=1):
-    """src/symgs/verifier.cc:48-76 (gs_serial over the colors forward, then
+    """src/symgs/verifier.cc:50-77 (gs_serial over the colors forward, then
     backward); returns a new x."""
     Ap, Aj, indices, co = _i32(Ap), _i32(Aj), _i32(indices), _i32(color_offsets)
     Ax, b = np.ascontiguousarray(Ax, np.float32), np.ascontiguousarray(b, np.float32)

@@ -247,11 +247,11 @@ int orc_ppr(int m, const int* row_offsets, const int* column_indices, const int*
   return iter;
 }
 
-// src/pr/verifier.cc:164-180: one push iteration over the OUT-CSR from the
+// src/pr/verifier.cc:50-68: one push iteration over the OUT-CSR from the
 // tested scores; returns the L1 residual  sum |base + d*in_sum - test|.
 double orc_pr_residual(int m, const int* out_row_offsets, const int* out_column_indices,
                        const int* degree, const float* scores_to_test, float damp) {
-  const float base_score = (1.0f - damp) / m;  // verifier.cc:126
+  const float base_score = (1.0f - damp) / m;  // verifier.cc:12
   std::vector<float> sums(m, 0.0f);
   for (int s = 0; s < m; s++) {
     float oc = scores_to_test[s] / degree[s];
@@ -359,7 +359,7 @@ float orc_max_rel_err(const float* A, const float* B, long long N) {
 
 // ---------------------------------------------------------------- SymGS
 
-// src/symgs/verifier.cc:48-64 (gs_serial): rows indices[row_start..row_stop)
+// src/symgs/verifier.cc:50-65 (gs_serial): rows indices[row_start..row_stop)
 // in order; each row's off-diagonal products summed in CSR order (product
 // rounded before the add), the diagonal the row's LAST entry with column ==
 // row; x[row] = (b[row] - rsum) / diag unless diag == 0.
@@ -379,7 +379,7 @@ static void gs_serial(const int* Ap, const int* Aj, const int* indices, const fl
   }
 }
 
-// src/symgs/verifier.cc:70-76 (SymGSVerifier's sweep) = src/symgs/omp_base.cc:
+// src/symgs/verifier.cc:74-77 (SymGSVerifier's sweep) = src/symgs/omp_base.cc:
 // 35-38: colors 0..C-1 forward, then C-1..0 backward; `sweeps` repetitions.
 void orc_symgs(const int* Ap, const int* Aj, const int* indices, const float* Ax, float* x,
                const float* b, const int* color_offsets, int n_colors, int sweeps) {

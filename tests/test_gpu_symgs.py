@@ -1,6 +1,6 @@
 """SymGS on the GPU (gd_symgs_*) against the reference's own SymGSSolver
 (golden vectors, tests/golden/ref_symgs.npz) and the oracle's sequential sweep
-(src/symgs/verifier.cc:48-76): bit-exact, including rows of more than 64 and
+(src/symgs/verifier.cc:50-77): bit-exact, including rows of more than 64 and
 more than 2048 off-diagonal entries (the exact-order workgroup scan), several
 sweeps, empty colors, partial colorings, and the coloring checks."""
 import json

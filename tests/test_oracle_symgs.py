@@ -1,4 +1,4 @@
-"""The SymGS oracle (oracle.symgs, a restatement of src/symgs/verifier.cc:48-76)
+"""The SymGS oracle (oracle.symgs, a restatement of src/symgs/verifier.cc:50-77)
 pinned against the reference's own SymGSSolver compiled from source
 (tests/golden/ref_symgs.npz), a hand-computed sweep, and convergence of
 repeated sweeps to the dense solve of an SPD system."""
