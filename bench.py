@@ -56,10 +56,12 @@ def cpu_baseline(in_csr, out_deg, iters_per_pr, target_s):
     }
 
 
-def cpu_baseline_extras(spmv_log2, seed):
+def cpu_baseline_extras(spmv_log2, seed, verify=None):
     """The oracle on the secondary workloads: SpMV y += A x on the ER matrix
-    (OpenMP, src/spmv/omp_base.cc restated) and one SymGS sweep on the 128^3
-    stencil (sequential, src/symgs/verifier.cc restated)."""
+    (OpenMP, src/spmv/omp_base.cc restated), one SymGS sweep on the 128^3
+    stencil (sequential, src/symgs/verifier.cc restated), the RMAT-16 solve,
+    and (verify = (out-CSR, scores)) the PRVerifier residual (sequential,
+    src/pr/verifier.cc:50-68 restated)."""
     import numpy as np
 
     import oracle
@@ -100,6 +102,13 @@ def cpu_baseline_extras(spmv_log2, seed):
                         "us_per_iteration": round(best * 1e6 / max(1, iters), 2),
                         "iterations": iters, "cores": oracle.num_threads(), "kind": "port",
                         "sample": "RMAT-16 pull PageRank to L1 tol 1e-4, best of 10 solves"}
+    if verify is not None:
+        oc, sc = verify
+        t0 = time.perf_counter()
+        oracle.pr_residual(oc.m, oc.row_offsets, oc.column_indices, oc.degree, sc)
+        out["verify"] = {"value": round((time.perf_counter() - t0) * 1e3, 1), "unit": "ms",
+                         "cores": 1, "kind": "port",
+                         "sample": "PRVerifier push residual over the RMAT out-CSR, one call"}
     return out
 
 
@@ -196,6 +205,38 @@ def bench_tol_run(plan, off, col, deg, tol, max_iter):
             "final_l1_change": st["final_error"], "near_tie": bool(st["near_tie"]),
             "solve_ms": round(st["solve_ms"], 3), "l1_vs_fp64": l1,
             "max_rel_vs_fp64": rel, "fp64_iterations": it64}
+
+
+def bench_verify(scale, seed, scores_h):
+    """Device PageRank verifier (gd_pr_verify) on the RMAT out-CSR: the
+    reference PRVerifier's push residual (src/pr/verifier.cc:50-68) of the
+    headline's final scores.  One call transposes the out-CSR, builds the
+    schedule and pulls once; best of 3 calls after a warm-up.  Returns the
+    record and the host out-CSR (for the CPU baseline)."""
+    import numpy as np
+    import torch
+
+    from gardenia_ocl_amd import graph, verify_residual
+
+    dev = torch.device("cuda:0")
+    out_csr, _ = graph.rmat(scale, 16, seed=seed, want_in=False)
+    off = torch.from_numpy(out_csr.row_offsets).to(dev)
+    col = torch.from_numpy(out_csr.column_indices).to(dev)
+    deg = torch.from_numpy(out_csr.degree).to(dev)
+    s = torch.from_numpy(np.ascontiguousarray(scores_h, np.float32)).to(dev)
+    verify_residual(off, col, deg, s)
+    torch.cuda.synchronize()
+    best, res = float("inf"), 0.0
+    for _ in range(3):
+        t0 = time.perf_counter()
+        res = verify_residual(off, col, deg, s)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    rec = {"workload": f"RMAT-{scale} out-CSR ({out_csr.nnz} edges), residual of the "
+                       "20-iteration scores", "ms": round(best * 1e3, 3), "residual": res,
+           "verdict": "Correct" if res < 1e-3 else f"Total Error: {res:f}"}
+    del off, col, deg, s
+    return rec, out_csr
 
 
 def bench_small(scale, tol, max_iter, seed, reps=7):
@@ -608,6 +649,7 @@ def main():
     dt = time.perf_counter() - t0
     log(f"timed {args.steps} steps in {dt:.3f} s")
 
+    scores_h = scores.cpu().numpy()  # the 20-iteration scores (verifier input)
     gteps = E * args.iters * args.steps / dt / 1e9
     avg_pull_ms = pull_ms / max(1, pull_n)
     bytes_iter = 4 * E + 24 * V + 4
@@ -626,6 +668,9 @@ def main():
         log(f"run to tol {args.tol_run} ...")
         extras["tol_run"] = bench_tol_run(plan, off, col, deg, args.tol_run, 200)
         del plan, off, col, deg, scores
+        torch.cuda.empty_cache()
+        log("device verifier ...")
+        extras["verify"], verify_csr = bench_verify(args.scale, args.seed, scores_h)
         torch.cuda.empty_cache()
         log("small graph ...")
         extras["small"] = bench_small(16, 1e-4, 100, args.seed)
@@ -653,7 +698,8 @@ def main():
         cpu = cpu_baseline(in_csr, out_deg, args.iters, args.cpu_seconds)
         if not args.no_extras:
             del in_csr
-            cpu["secondary"] = cpu_baseline_extras(args.spmv_log2, args.seed)
+            cpu["secondary"] = cpu_baseline_extras(args.spmv_log2, args.seed,
+                                                   verify=(verify_csr, scores_h))
     rec = {
         "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),

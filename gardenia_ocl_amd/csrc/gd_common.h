@@ -71,6 +71,11 @@ int build_schedule(int m, int64_t nnz, const int32_t* row_offsets, int tile_nnz,
 // GD_OK, GD_ERR_INVALID or GD_ERR_DEVICE.
 int check_columns(const int32_t* d_col, int64_t nnz, int32_t ncols, hipStream_t stream);
 
+// Checks on the device that d_off[0..m] is a CSR prefix sum of nnz edges
+// (off[0] == 0, off[m] == nnz, non-decreasing), with build_schedule's error
+// texts; returns GD_OK, GD_ERR_INVALID or GD_ERR_DEVICE.
+int check_offsets(const int32_t* d_off, int m, int64_t nnz, hipStream_t stream);
+
 struct DeviceBuffer {
   void* ptr = nullptr;
   size_t bytes = 0;

@@ -146,6 +146,39 @@ int check_columns(const int32_t* d_col, int64_t nnz, int32_t ncols, hipStream_t 
   return GD_OK;
 }
 
+// flag[0]: off[0] != 0 or off[m] != nnz; flag[1]: the first row r with
+// off[r+1] < off[r] (INT_MAX if none).
+__global__ void check_offsets_kernel(const int32_t* __restrict__ off, int m, int64_t nnz,
+                                     int* flag) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < m; r += gridDim.x * blockDim.x)
+    if (off[r + 1] < off[r]) atomicMin(&flag[1], r);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (off[0] != 0 || (int64_t)off[m] != nnz)) flag[0] = 1;
+}
+
+int check_offsets(const int32_t* d_off, int m, int64_t nnz, hipStream_t stream) {
+  DeviceBuffer flag;
+  int rc = flag.alloc(2 * sizeof(int));
+  if (rc) return rc;
+  const int init[2] = {0, 0x7fffffff};
+  GD_HIP_TRY(hipMemcpyAsync(flag.ptr, init, sizeof(init), hipMemcpyHostToDevice, stream));
+  const int blocks = std::max(1, std::min((m + 255) / 256, 4096));
+  hipLaunchKernelGGL(check_offsets_kernel, dim3(blocks), dim3(256), 0, stream, d_off, m, nnz,
+                     flag.as<int>());
+  GD_HIP_TRY(hipGetLastError());
+  int h[2] = {0, 0};
+  GD_HIP_TRY(hipMemcpyAsync(h, flag.ptr, sizeof(h), hipMemcpyDeviceToHost, stream));
+  GD_HIP_TRY(hipStreamSynchronize(stream));
+  if (h[0]) {
+    set_error("row_offsets[0] must be 0 and row_offsets[m] must equal nnz");
+    return GD_ERR_INVALID;
+  }
+  if (h[1] != 0x7fffffff) {
+    set_error("row_offsets must be non-decreasing (row " + std::to_string(h[1]) + ")");
+    return GD_ERR_INVALID;
+  }
+  return GD_OK;
+}
+
 }  // namespace gd
 
 extern "C" const char* gd_last_error(void) { return gd::g_last_error.c_str(); }

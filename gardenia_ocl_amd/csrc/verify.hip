@@ -95,16 +95,8 @@ extern "C" int gd_pr_verify(int m, int64_t nnz, const int32_t* d_out_row_offsets
   }
   hipStream_t s = (hipStream_t)stream;
   int rc = check_columns(d_out_column_indices, nnz, m, s);
+  if (!rc) rc = check_offsets(d_out_row_offsets, m, nnz, s);  // before the transpose reads them
   if (rc) return rc;
-  {
-    std::vector<int32_t> h_off(m + 1);
-    GD_HIP_TRY(hipMemcpyAsync(h_off.data(), d_out_row_offsets, 4 * ((size_t)m + 1),
-                              hipMemcpyDeviceToHost, s));
-    GD_HIP_TRY(hipStreamSynchronize(s));
-    Schedule tmp;  // validates the offsets
-    rc = build_schedule(m, nnz, h_off.data(), kBlock * 16, kTileRows, &tmp);
-    if (rc) return rc;
-  }
   Transpose tr;  // destination-major, sources in increasing order
   rc = build_transpose(m, m, nnz, d_out_row_offsets, d_out_column_indices, s, &tr);
   if (rc) return rc;

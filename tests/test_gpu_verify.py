@@ -79,3 +79,31 @@ def test_verifier_rmat_after_gpu_solve(device):
     ref = oracle.pr_residual(out_csr.m, out_csr.row_offsets, out_csr.column_indices,
                              out_csr.degree, sh)
     assert abs(res - ref) <= 1e-12 * max(1.0, ref) and res < 1e-3
+
+
+@pytest.mark.parametrize("bad", ["decreasing", "last", "first", "column"])
+def test_verifier_rejects_invalid_csr(device, bad):
+    """Offsets and columns are checked on the device before the transpose
+    reads them: a bad CSR is an error with build_schedule's text, not a fault."""
+    from gardenia_ocl_amd import GardeniaError
+
+    off = np.array([0, 2, 3, 5, 6], np.int32)
+    col = np.array([1, 2, 0, 3, 1, 0], np.int32)
+    deg = np.diff(off).astype(np.int32)
+    if bad == "decreasing":
+        off[2] = 1  # 0 2 1 5 6
+        msg = "non-decreasing (row 1)"
+    elif bad == "last":
+        off[4] = 5
+        msg = "row_offsets[m] must equal nnz"
+    elif bad == "first":
+        off[0] = 1
+        msg = "row_offsets[0] must be 0"
+    else:
+        col[3] = 4
+        msg = "column index out of range"
+    s = np.full(4, 0.25, np.float32)
+    with pytest.raises(GardeniaError, match=msg.replace("[", r"\[").replace("]", r"\]")
+                       .replace("(", r"\(").replace(")", r"\)")):
+        verify_residual(_t(off, np.int32, device), _t(col, np.int32, device),
+                        _t(deg, np.int32, device), _t(s, np.float32, device))
