@@ -65,6 +65,7 @@ struct WprEpilogue {
 struct WprPolicy {
   // as plain PageRank: with unit weights the forward is its arithmetic exactly
   static constexpr bool kExactHubs = true;
+  static constexpr bool kSeqAll = false;
   struct Args {
     float* x;
     float* c0;
@@ -110,6 +111,7 @@ struct AdjEpilogue {
 
 struct AdjPolicy {
   static constexpr bool kExactHubs = false;
+  static constexpr bool kSeqAll = false;
   struct Args {
     float* lam;
     float* c0;

@@ -47,6 +47,7 @@ struct PrPolicy {
   // hub rows in the reference's order and rounding: its sequential FP32 sum of
   // a 10^5-edge row drifts by ~1e-4 relative, which a tree sum does not match
   static constexpr bool kExactHubs = true;
+  static constexpr bool kSeqAll = false;
   struct Args {
     float* scores;
     float* c0;

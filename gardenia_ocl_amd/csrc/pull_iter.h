@@ -16,6 +16,9 @@
 //                                       edge order with sequential FP32
 //                                       rounding (exact_hub_kernel), else hub
 //                                       segments tree-summed + finalize
+//   static constexpr bool kSeqAll       every row of a row tile summed by its
+//                                       owner thread in edge order (else rows
+//                                       of > 64 edges by a wavefront tree)
 //   template <int NT> static G gather(const Args&, int odd)
 //   template <int NT> static E epilogue(const Args&, int odd)
 // with G / E as described in pull_engine.h (value(e, u); Pre, pre, init,
@@ -128,8 +131,8 @@ __device__ __forceinline__ void tile_body(const typename P::Args& a, const IterA
   double err = 0.0;
   if (t.y >= 0) {
     const auto e = P::template epilogue<NT>(a, odd);
-    err = row_tile<IPT, (NT > 0), SORTED>(it.off, it.col, it.pos, it.e_max, t.x, t.y, t.z,
-                                          t.w - t.z, g, e, lds);
+    err = row_tile<IPT, (NT > 0), SORTED, decltype(g), decltype(e), P::kSeqAll>(
+        it.off, it.col, it.pos, it.e_max, t.x, t.y, t.z, t.w - t.z, g, e, lds);
   } else if constexpr (!P::kExactHubs) {
     const float p = segment_sum<IPT, (NT > 0), SORTED>(it.col, it.pos, t.z, t.w - t.z, g, lds.val);
     if (threadIdx.x == 0) it.hub_part[-t.y - 1] = p;
@@ -185,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
       if (cur.y >= 0) {
         stage_row_tile<IPT, SORTED>(cur, R, v, lds);
         __syncthreads();
-        err += reduce_row_tile<IPT>(cur.x, cur.y - cur.x, e, R.pre, lds);
+        err += reduce_row_tile<IPT, decltype(e), P::kSeqAll>(cur.x, cur.y - cur.x, e, R.pre, lds);
       } else if constexpr (!P::kExactHubs) {
         float sacc = 0.0f;
 #pragma unroll

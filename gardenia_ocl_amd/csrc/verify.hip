@@ -7,10 +7,11 @@
 // Each destination therefore accumulates its sources in increasing source
 // order.  Here the out-CSR is transposed with a stable sort (each
 // destination's sources in exactly that order) and pulled by the iteration
-// engine with every row of more than 64 sources summed in order with
-// sequential FP32 rounding by one lane of its own workgroup (hub_chain_sum),
-// shorter rows by their owner thread in order: every per-vertex term is the reference's bit for bit; only
-// the FP64 sum of the terms runs in a different (fixed) order.
+// engine with every row of more than 4096 sources summed in order with
+// sequential FP32 rounding by its own workgroup (exact_hub_kernel), every
+// other row by its owner thread in order (kSeqAll): every per-vertex term is
+// the reference's bit for bit; only the FP64 sum of the terms runs in a
+// different (fixed) order.
 
 #include <hip/hip_runtime.h>
 
@@ -50,6 +51,7 @@ struct VerifyEpilogue {
 
 struct VerifyPolicy {
   static constexpr bool kExactHubs = true;
+  static constexpr bool kSeqAll = true;  // every row in source order, bit for bit
   struct Args {
     const float* contrib;
     const float* scores;
@@ -110,7 +112,7 @@ extern "C" int gd_pr_verify(int m, int64_t nnz, const int32_t* d_out_row_offsets
   eng.ipt = 16;
   eng.nt = 0;
   eng.persistent = false;
-  eng.exact_len = kSeqRow;  // rows longer than 64: in-order exact sums, not tree sums
+  eng.exact_len = kBlock * 16;  // longer rows: in-order exact sums by their own workgroup
   rc = eng.build(m, nnz, tr.off.as<int32_t>(), tr.row.as<int32_t>(), s);
   if (!rc) rc = eng.setup_grid<VerifyPolicy>();
   if (rc) return rc;
