@@ -84,8 +84,10 @@ void gd_pr_default_plan_config(gd_pr_plan_config *cfg);
 
 typedef struct gd_pr_plan_info_t {
   int64_t n_tiles;        /* workgroups per iteration: row tiles + whole-row hub tiles */
-  int64_t n_hubs;         /* rows summed as hubs: longer than 2048 edges (whole-row
-                             tiles in the reference's summation order) */
+  int64_t n_hubs;         /* rows summed as hubs: longer than one tile, 256 *
+                             items_per_thread edges (whole-row tiles in the
+                             reference's summation order: one lane's chain up to
+                             2^17 edges, the workgroup's exact scan beyond) */
   int64_t n_hub_segments; /* tiles spent on hub rows */
   int32_t max_row_len;
   int32_t relabel;        /* 1 if the plan relabelled the vertices */
