@@ -32,6 +32,21 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """The host CPU model and the logical CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return f"{model}, {n} logical CPUs available to the process"
+
+
 def cpu_baseline(in_csr, out_deg, iters_per_pr, target_s):
     """Time the oracle's OpenMP pull loop (port of src/pr/omp_base.cc) for a
     bounded number of iterations; returns GTEPS and the sample description."""
@@ -53,6 +68,7 @@ def cpu_baseline(in_csr, out_deg, iters_per_pr, target_s):
         "value": round(gteps, 4), "unit": "GTEPS", "cores": oracle.num_threads(), "kind": "port",
         "sample": (f"oracle OpenMP pull loop (restates src/pr/omp_base.cc, g++ -O3 -fopenmp) on the "
                    f"same RMAT in-CSR, {n} iterations from 1/V, tol 0, {dt:.1f} s"),
+        "host": host_cpu(),
     }
 
 
