@@ -48,6 +48,49 @@ def host_cpu():
 
 
 def cpu_baseline(in_csr, out_deg, iters_per_pr, target_s):
+    """The CPU baseline on the same in-CSR.  Where oracle/_ref/libref.so was
+    built (the reference's own src/pr/omp_base.cc compiled from source), the
+    reference's PRSolver itself: solves from 1/V to its compile-time EPSILON
+    1e-3 (MAX_ITER 30), repeated for about target_s seconds, GTEPS from the
+    iterations and runtime it prints (its own Timer, include/timer.h), median
+    over the solves.  The oracle's OpenMP pull loop (the restatement) is timed
+    beside it for a bounded number of fixed iterations."""
+    port = cpu_baseline_port(in_csr, out_deg, iters_per_pr, target_s)
+    import oracle
+
+    if not oracle.ref_available():
+        return port
+    import statistics
+    import tempfile
+
+    import numpy as np
+
+    m, E = in_csr.m, in_csr.nnz
+    init = np.float32(1.0) / np.float32(m)
+    runs, total = [], 0.0
+    with tempfile.TemporaryDirectory() as tmp:
+        log_path = os.path.join(tmp, "pr.log")
+        while total < target_s and len(runs) < 10:
+            sc = np.full(m, init, np.float32)
+            t0 = time.perf_counter()
+            its, ms, _ = oracle.ref_pr_solve(m, E, in_csr.row_offsets, in_csr.column_indices,
+                                             out_deg, sc, log_path)
+            total += time.perf_counter() - t0
+            runs.append((its, ms))
+    its = runs[0][0]
+    gteps = statistics.median(E * i / (ms * 1e-3) / 1e9 for i, ms in runs)
+    return {
+        "value": round(gteps, 4), "unit": "GTEPS", "cores": oracle.num_threads(),
+        "kind": "reference",
+        "sample": (f"the reference's own OpenMP PRSolver (src/pr/omp_base.cc compiled from source, "
+                   f"g++ -O3 -fopenmp) on the same RMAT in-CSR from 1/V: {its} iterations to its "
+                   f"EPSILON 1e-3, runtime from its own Timer, median of {len(runs)} solves "
+                   f"({total:.1f} s)"),
+        "host": host_cpu(), "port": port,
+    }
+
+
+def cpu_baseline_port(in_csr, out_deg, iters_per_pr, target_s):
     """Time the oracle's OpenMP pull loop (port of src/pr/omp_base.cc) for a
     bounded number of iterations; returns GTEPS and the sample description."""
     import numpy as np
@@ -72,7 +115,38 @@ def cpu_baseline(in_csr, out_deg, iters_per_pr, target_s):
     }
 
 
-def cpu_baseline_extras(spmv_log2, seed, verify=None):
+def tol_parity(in_csr, out_deg, tol, max_iter, scores, errs, x64, rec):
+    """The oracle's run to the same tolerance on the same in-CSR, beside the
+    GPU's: iteration counts, the L1 values at the deciding iterations, the
+    largest per-iteration L1 difference, and whether the oracle's own L1 lies
+    within that difference of tol at any iteration up to its stop (only then
+    could two correct evaluations stop at different counts)."""
+    import numpy as np
+
+    import oracle
+
+    t0 = time.perf_counter()
+    ref, it, ref_errs = oracle.pr(in_csr.m, in_csr.row_offsets, in_csr.column_indices, out_deg,
+                                  tol=tol, max_iter=max_iter)
+    dt = time.perf_counter() - t0
+    g = rec["iterations"]
+    n = min(len(errs), len(ref_errs))
+    gap = float(np.max(np.abs(np.asarray(errs[:n]) - ref_errs[:n]))) if n else 0.0
+    tie = bool(np.any(np.abs(ref_errs[:it] - tol) <= gap))
+    rel = np.abs(scores.astype(np.float64) - ref) / np.maximum(np.abs(ref.astype(np.float64)),
+                                                               1e-300)
+    out = {"oracle_iterations": int(it), "iterations_match": g == it,
+           "l1_at_stop": {"gpu": float(errs[g - 1]), "oracle": float(ref_errs[it - 1])},
+           "l1_before_stop": {"gpu": float(errs[g - 2]) if g > 1 else None,
+                              "oracle": float(ref_errs[it - 2]) if it > 1 else None},
+           "max_l1_gap": gap, "oracle_l1_within_gap_of_tol": tie,
+           "max_rel_vs_oracle": float(rel.max()), "oracle_solve_s": round(dt, 3)}
+    if x64 is not None:
+        out["oracle_l1_vs_fp64"] = float(np.abs(ref.astype(np.float64) - x64).sum())
+    return out
+
+
+def cpu_baseline_extras(spmv_log2, seed, verify=None, checks=None, extras=None):
     """The oracle on the secondary workloads: SpMV y += A x on the ER matrix
     (OpenMP, src/spmv/omp_base.cc restated), one SymGS sweep on the 128^3
     stencil (sequential, src/symgs/verifier.cc restated), the RMAT-16 solve,
@@ -88,6 +162,9 @@ def cpu_baseline_extras(spmv_log2, seed, verify=None):
     x = graph.uniform_pm1(er.m, seed + 1)
     y = np.zeros(er.m, np.float32)
     oracle.spmv(er.m, er.row_offsets, er.column_indices, er.weight, x, y)
+    if checks and "spmv" in checks and extras and "spmv" in extras:
+        # the GPU's y = A x from y0 = 0 (both modes gave the same bits)
+        extras["spmv"]["bit_exact_vs_oracle"] = bool(np.array_equal(checks["spmv"], y))
     reps = 5
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -114,6 +191,15 @@ def cpu_baseline_extras(spmv_log2, seed, verify=None):
         _, iters, _ = oracle.pr(i16.m, i16.row_offsets, i16.column_indices, o16.degree,
                                 tol=1e-4, max_iter=100)
         best = min(best, time.perf_counter() - t0)
+    if checks and "small" in checks and extras and "small" in extras:
+        ref, _, ref_errs = oracle.pr(i16.m, i16.row_offsets, i16.column_indices, o16.degree,
+                                     tol=1e-4, max_iter=100)
+        gs, gerrs, gits = checks["small"]
+        n = min(len(gerrs), len(ref_errs))
+        extras["small"].update({
+            "oracle_iterations": iters, "iterations_match": gits == iters,
+            "max_l1_gap": float(np.max(np.abs(gerrs[:n] - ref_errs[:n]))) if n else 0.0,
+            "max_rel_vs_oracle": float(np.max(np.abs(gs - ref) / ref))})
     out["pr_rmat16"] = {"value": round(best * 1e3, 3), "unit": "ms per solve",
                         "us_per_iteration": round(best * 1e6 / max(1, iters), 2),
                         "iterations": iters, "cores": oracle.num_threads(), "kind": "port",
@@ -147,9 +233,17 @@ def bench_spmv(log2_rows, nnz_per_row, reps, seed):
     st = torch.cuda.current_stream()
     out = {"workload": f"ER 2^{log2_rows} x 2^{log2_rows}, {nnz_per_row} nnz/row, y += A x",
            "nnz": nnz, "reps": reps, "algorithmic_bytes": byts}
+    # y = A x in FP64 (torch CSR product) for the L1 figure
+    A64 = torch.sparse_csr_tensor(Ap, Aj, Ax.double(), (M, M))
+    y64 = (A64 @ x.double().unsqueeze(1)).squeeze(1)
+    del A64
+    firsts = {}
     for mode in ("auto", "tiles"):
         plan = SpmvPlan(Ap, Aj, Ax, mode=mode)
-        for _ in range(3):
+        y.zero_()
+        plan.run(x, y)  # y = A x from y0 = 0: checked against FP64 here, the oracle later
+        firsts[mode] = y.cpu().numpy()
+        for _ in range(2):
             plan.run(x, y)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -165,8 +259,12 @@ def bench_spmv(log2_rows, nnz_per_row, reps, seed):
         out[key + "gflops"] = round(2 * nnz / (ms * 1e-3) / 1e9, 2)
         out[key + "achieved_gbs"] = round(byts / (ms * 1e-3) / 1e9, 1)
         out[key + "frac_of_hbm_peak"] = round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        yf = torch.from_numpy(firsts[mode]).to(dev).double()
+        out[key + "l1_vs_fp64"] = float((yf - y64).abs().sum())
+        out[key + "max_abs_vs_fp64"] = float((yf - y64).abs().max())
         plan.close()
-    return out
+    out["both_modes_identical"] = bool((firsts["auto"] == firsts["tiles"]).all())
+    return out, firsts["auto"]
 
 
 def fp64_pagerank(off, col, outdeg, *, damping=0.85, seeds=None, tol=1e-13, max_iter=1000):
@@ -217,10 +315,11 @@ def bench_tol_run(plan, off, col, deg, tol, max_iter):
     x64, it64 = fp64_pagerank(off, col, deg, tol=1e-13)
     l1 = float((s.double() - x64).abs().sum())
     rel = float(((s.double() - x64).abs() / x64.clamp(min=1e-300)).max())
-    return {"tol": tol, "iterations": st["iterations"], "converged": bool(st["converged"]),
-            "final_l1_change": st["final_error"], "near_tie": bool(st["near_tie"]),
-            "solve_ms": round(st["solve_ms"], 3), "l1_vs_fp64": l1,
-            "max_rel_vs_fp64": rel, "fp64_iterations": it64}
+    rec = {"tol": tol, "iterations": st["iterations"], "converged": bool(st["converged"]),
+           "final_l1_change": st["final_error"], "near_tie": bool(st["near_tie"]),
+           "solve_ms": round(st["solve_ms"], 3), "l1_vs_fp64": l1,
+           "max_rel_vs_fp64": rel, "fp64_iterations": it64}
+    return rec, s.cpu().numpy(), errs, x64.cpu().numpy()
 
 
 def bench_verify(scale, seed, scores_h):
@@ -280,15 +379,19 @@ def bench_small(scale, tol, max_iter, seed, reps=7):
         s.fill_(init)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        st, _ = plan.run(s, tol=tol, max_iter=max_iter)
+        st, errs = plan.run(s, tol=tol, max_iter=max_iter)
         wall.append((time.perf_counter() - t0) * 1e3)
         ev.append(st["solve_ms"])
     its = st["iterations"]
     ms = float(np.median(ev))
-    return {"workload": f"RMAT-{scale} ({E} edges) pull PageRank to L1 tol {tol}, max {max_iter}",
-            "iterations": its, "solve_ms": round(ms, 4), "wall_ms": round(float(np.median(wall)), 4),
-            "us_per_iteration": round(ms / max(its, 1) * 1e3, 2),
-            "gteps": round(E * its / (ms * 1e-3) / 1e9, 2)}
+    x64, _ = fp64_pagerank(off, col, deg, tol=1e-13)
+    rec = {"workload": f"RMAT-{scale} ({E} edges) pull PageRank to L1 tol {tol}, max {max_iter}",
+           "iterations": its, "solve_ms": round(ms, 4), "wall_ms": round(float(np.median(wall)), 4),
+           "us_per_iteration": round(ms / max(its, 1) * 1e3, 2),
+           "gteps": round(E * its / (ms * 1e-3) / 1e9, 2),
+           "l1_vs_fp64": float((s.double() - x64).abs().sum())}
+    plan.close()
+    return rec, (s.cpu().numpy(), errs, its)
 
 
 def bench_ppr(scale, k, tol, seed, fp64_seeds):
@@ -339,10 +442,20 @@ def bench_ppr(scale, k, tol, seed, fp64_seeds):
         X64, _ = fp64_pagerank(off, col, deg, seeds=seeds[:fp64_seeds], tol=1e-13)
         l1_b = [float((X[:, j].double() - X64[:, j]).abs().sum()) for j in range(fp64_seeds)]
         l1_l = [float((looped[j].double() - X64[:, j]).abs().sum()) for j in range(fp64_seeds)]
+    # algorithmic bytes of the batched solve (every array once per iteration,
+    # the k-wide score / contrib rows of the running columns only):
+    # iterations * (4(V+1) + 4E + 4V) + 16 V * sum over iterations of the running columns
+    col_its = int(bp.column_iterations().sum())
+    E = ic.nnz
+    abytes = bst["iterations"] * (4 * (V + 1) + 4 * E + 4 * V) + 16 * V * col_its
+    achieved = abytes / t_batched / 1e9
     return {"workload": f"RMAT-{scale} ({ic.nnz} edges), {k} one-hot seeds, per-seed L1 tol {tol}",
+            "algorithmic_bytes": abytes, "achieved_gbs": round(achieved, 1),
+            "frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 4),
+            "gather_bytes": 4 * E * col_its,
             "batched_s": round(t_batched, 4), "looped_s": round(t_looped, 4),
             "speedup": round(t_looped / t_batched, 2), "batched_iterations": bst["iterations"],
-            "batched_column_iterations_total": int(bp.column_iterations().sum()),
+            "batched_column_iterations_total": col_its,
             "looped_iterations_total": its, "k_chunk": bp.info()["k_chunk"],
             "l1_vs_fp64_batched": l1_b, "l1_vs_fp64_looped": l1_l,
             "fp64_seeds_checked": fp64_seeds}
@@ -464,6 +577,12 @@ def bench_multigraph(n, lo, hi, tol, seed):
         out[f"{key}_solve_ms"] = round(ms, 3)
     out["iterations_max"] = int(its.max())
     out["iterations_total"] = int(its.sum())
+    l1 = []
+    for (g_off, g_col, g_deg), sc in zip(gs, scores):
+        x64, _ = fp64_pagerank(g_off, g_col, g_deg, tol=1e-13)
+        l1.append(float((sc.double() - x64).abs().sum()))
+    out["l1_vs_fp64_max"] = max(l1)
+    out["l1_vs_fp64_mean"] = sum(l1) / len(l1)
     plans = [PageRankPlan(*g) for g in gs]
     for p, s, v in zip(plans, scores, init):  # warm-up
         s.fill_(v)
@@ -605,10 +724,10 @@ def main():
         print(json.dumps({"symgs": bench_symgs(128, 20, args.seed)}), flush=True)
         return
     if args.only == "small":
-        print(json.dumps({"small": bench_small(16, 1e-4, 100, args.seed)}), flush=True)
+        print(json.dumps({"small": bench_small(16, 1e-4, 100, args.seed)[0]}), flush=True)
         return
     if args.only == "spmv":
-        print(json.dumps({"spmv": bench_spmv(args.spmv_log2, 32, 50, args.seed)}), flush=True)
+        print(json.dumps({"spmv": bench_spmv(args.spmv_log2, 32, 50, args.seed)[0]}), flush=True)
         return
     if args.only == "csr_build":
         t0 = time.perf_counter()
@@ -679,22 +798,37 @@ def main():
         "avg_launch_ms": round(avg_pull_ms, 5),
         "algorithmic_bytes_per_launch": bytes_iter,
     }
+    # the same steps in production mode: chunks replayed from captured hipGraphs,
+    # no per-launch events (the timed region above launches eagerly to time
+    # every pull launch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(False)
+    torch.cuda.synchronize()
+    dt_prod = time.perf_counter() - t0
+    production = {"ms_per_step": round(dt_prod / args.steps * 1e3, 4),
+                  "gteps": round(E * args.iters * args.steps / dt_prod / 1e9, 3),
+                  "mode": "hipGraph replay of the iteration chunks, no events"}
     extras = {}
+    checks = {}  # GPU results the CPU leg compares with the oracle
     if not args.no_extras:
         log(f"run to tol {args.tol_run} ...")
-        extras["tol_run"] = bench_tol_run(plan, off, col, deg, args.tol_run, 200)
+        extras["tol_run"], tol_scores, tol_errs, tol_x64 = bench_tol_run(plan, off, col, deg,
+                                                                         args.tol_run, 200)
+        checks["tol_run"] = (tol_scores, tol_errs, tol_x64)
         del plan, off, col, deg, scores
         torch.cuda.empty_cache()
         log("device verifier ...")
         extras["verify"], verify_csr = bench_verify(args.scale, args.seed, scores_h)
         torch.cuda.empty_cache()
         log("small graph ...")
-        extras["small"] = bench_small(16, 1e-4, 100, args.seed)
+        extras["small"], checks["small"] = bench_small(16, 1e-4, 100, args.seed)
         log("spmv ...")
-        extras["spmv"] = bench_spmv(args.spmv_log2, 32, 50, args.seed)
+        extras["spmv"], checks["spmv"] = bench_spmv(args.spmv_log2, 32, 50, args.seed)
         torch.cuda.empty_cache()
         log("batched ppr ...")
-        extras["ppr"] = bench_ppr(args.ppr_scale, args.ppr_k, args.ppr_tol, args.seed, 2)
+        extras["ppr"] = bench_ppr(args.ppr_scale, args.ppr_k, args.ppr_tol, args.seed, 8)
         torch.cuda.empty_cache()
         log("device CSR build ...")
         extras["csr_build"] = bench_csr_build(args.scale, args.seed, host_build_s, E)
@@ -713,9 +847,13 @@ def main():
         log("cpu baseline ...")
         cpu = cpu_baseline(in_csr, out_deg, args.iters, args.cpu_seconds)
         if not args.no_extras:
+            log("oracle comparison of the tol run ...")
+            extras["tol_run"].update(tol_parity(in_csr, out_deg, args.tol_run, 200,
+                                                *checks["tol_run"], extras["tol_run"]))
             del in_csr
             cpu["secondary"] = cpu_baseline_extras(args.spmv_log2, args.seed,
-                                                   verify=(verify_csr, scores_h))
+                                                   verify=(verify_csr, scores_h), checks=checks,
+                                                   extras=extras)
     rec = {
         "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
@@ -729,7 +867,7 @@ def main():
                    "nontemporal": info["nontemporal"], "persistent": bool(info["persistent"]),
                    "pull_grid": info["pull_grid"], "sort_tiles": bool(info["sort_tiles"]),
                    "plan_ms": round(info["plan_ms"], 1)},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "production": production,
     }
     if "ppr" in extras:
         rec["ppr_speedup_vs_looped"] = extras["ppr"]["speedup"]

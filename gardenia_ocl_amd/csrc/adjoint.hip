@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "gardenia_hip.h"
@@ -260,17 +261,15 @@ int gd_wpr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* 
     set_error("no HIP device visible");
     return GD_ERR_NO_DEVICE;
   }
-  gd_wpr_plan* p = new gd_wpr_plan();
+  std::unique_ptr<gd_wpr_plan> owner(new gd_wpr_plan());
+  gd_wpr_plan* p = owner.get();
   p->m = m;
   p->nnz = nnz;
   p->stream = (hipStream_t)stream;
   p->off = d_off;
   p->col = d_col;
   p->a = d_weight;
-  auto fail = [&](int rc) {
-    delete p;
-    return rc;
-  };
+  auto fail = [](int rc) { return rc; };  // owner frees the plan
   int rc = check_columns(d_col, nnz, m, p->stream);
   if (rc) return fail(rc);
   {
@@ -304,7 +303,7 @@ int gd_wpr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* 
     rc = gd_spmv_plan_create(m, m, nnz, p->tr.off.as<int32_t>(), p->tr.row.as<int32_t>(),
                              p->at.as<float>(), stream, &p->qplan);
   if (rc) return fail(rc);
-  *out = p;
+  *out = owner.release();
   return GD_OK;
 }
 
@@ -326,9 +325,9 @@ int gd_wpr_forward(gd_wpr_plan* p, float* d_x, const gd_pr_options* opts, gd_pr_
   GD_CHECK_ARG(o.max_iter >= 0, "max_iter must be >= 0");
   gd_pr_stats s;
   memset(&s, 0, sizeof(s));
-  hipEvent_t t0, t1;
-  GD_HIP_TRY(hipEventCreate(&t0));
-  GD_HIP_TRY(hipEventCreate(&t1));
+  EventPair evp;
+  GD_HIP_TRY(evp.create());
+  hipEvent_t t0 = evp.a, t1 = evp.b;
   GD_HIP_TRY(hipEventRecord(t0, p->stream));
   int rc = wpr_refresh_weights(p);  // weights may have changed since the last call
   if (rc) return rc;
@@ -342,8 +341,6 @@ int gd_wpr_forward(gd_wpr_plan* p, float* d_x, const gd_pr_options* opts, gd_pr_
   GD_HIP_TRY(hipEventRecord(t1, p->stream));
   GD_HIP_TRY(hipEventSynchronize(t1));
   GD_HIP_TRY(hipEventElapsedTime(&s.solve_ms, t0, t1));
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
   if (stats) *stats = s;
   return GD_OK;
 }
@@ -361,9 +358,9 @@ int gd_wpr_backward(gd_wpr_plan* p, const float* d_x, const float* d_w, float* d
   gd_pr_stats s;
   memset(&s, 0, sizeof(s));
   const int m = p->m;
-  hipEvent_t t0, t1;
-  GD_HIP_TRY(hipEventCreate(&t0));
-  GD_HIP_TRY(hipEventCreate(&t1));
+  EventPair evp;
+  GD_HIP_TRY(evp.create());
+  hipEvent_t t0 = evp.a, t1 = evp.b;
   GD_HIP_TRY(hipEventRecord(t0, p->stream));
   int rc = wpr_refresh_weights(p);
   if (rc) return rc;
@@ -396,8 +393,6 @@ int gd_wpr_backward(gd_wpr_plan* p, const float* d_x, const float* d_w, float* d
   GD_HIP_TRY(hipEventRecord(t1, p->stream));
   GD_HIP_TRY(hipEventSynchronize(t1));
   GD_HIP_TRY(hipEventElapsedTime(&s.solve_ms, t0, t1));
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
   if (stats) *stats = s;
   return GD_OK;
 }
@@ -432,17 +427,15 @@ int gd_spmv_bwd_plan_create(int rows, int cols, int64_t nnz, const int32_t* d_Ap
     set_error("no HIP device visible");
     return GD_ERR_NO_DEVICE;
   }
-  gd_spmv_bwd_plan* p = new gd_spmv_bwd_plan();
+  std::unique_ptr<gd_spmv_bwd_plan> owner(new gd_spmv_bwd_plan());
+  gd_spmv_bwd_plan* p = owner.get();
   p->rows = rows;
   p->cols = cols;
   p->nnz = nnz;
   p->stream = (hipStream_t)stream;
   p->Ap = d_Ap;
   p->Aj = d_Aj;
-  auto fail = [&](int rc) {
-    delete p;
-    return rc;
-  };
+  auto fail = [](int rc) { return rc; };  // owner frees the plan
   int rc = check_columns(d_Aj, nnz, cols, p->stream);
   if (rc) return fail(rc);
   if (rows > 0) {
@@ -464,7 +457,7 @@ int gd_spmv_bwd_plan_create(int rows, int cols, int64_t nnz, const int32_t* d_Ap
     rc = gd_spmv_plan_create(cols, rows, nnz, p->tr.off.as<int32_t>(), p->tr.row.as<int32_t>(),
                              p->at.as<float>(), stream, &p->tplan);
   if (rc) return fail(rc);
-  *out = p;
+  *out = owner.release();
   return GD_OK;
 }
 

@@ -76,6 +76,23 @@ int check_columns(const int32_t* d_col, int64_t nnz, int32_t ncols, hipStream_t 
 // texts; returns GD_OK, GD_ERR_INVALID or GD_ERR_DEVICE.
 int check_offsets(const int32_t* d_off, int m, int64_t nnz, hipStream_t stream);
 
+// Two timing events, destroyed on every exit path of the function that holds
+// them (the GD_HIP_TRY early returns included).
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  EventPair() = default;
+  EventPair(const EventPair&) = delete;
+  EventPair& operator=(const EventPair&) = delete;
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  hipError_t create() {
+    hipError_t e = hipEventCreate(&a);
+    return e != hipSuccess ? e : hipEventCreate(&b);
+  }
+};
+
 struct DeviceBuffer {
   void* ptr = nullptr;
   size_t bytes = 0;

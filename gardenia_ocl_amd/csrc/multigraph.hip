@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "gardenia_hip.h"
@@ -204,11 +205,9 @@ int gd_multi_pr_plan_create(int n_graphs, const int32_t* m, const int64_t* nnz,
     set_error("no HIP device visible");
     return GD_ERR_NO_DEVICE;
   }
-  gd_multi_pr_plan* p = new gd_multi_pr_plan();
-  auto fail = [&](int rc) {
-    delete p;
-    return rc;
-  };
+  std::unique_ptr<gd_multi_pr_plan> owner(new gd_multi_pr_plan());
+  gd_multi_pr_plan* p = owner.get();
+  auto fail = [](int rc) { return rc; };  // owner frees the plan
   p->n_graphs = n_graphs;
   p->total_m = (int)tm;
   p->total_nnz = te;
@@ -295,7 +294,7 @@ int gd_multi_pr_plan_create(int n_graphs, const int32_t* m, const int64_t* nnz,
   GD_HIP_TRY(up(p->hub_nseg, hub_nseg.data(), 4 * (size_t)nh));
   GD_HIP_TRY(up(p->d_vbase, p->vbase.data(), 4 * ((size_t)n_graphs + 1)));
   GD_HIP_TRY(hipStreamSynchronize(p->stream));
-  *out = p;
+  *out = owner.release();
   return GD_OK;
 }
 
@@ -342,9 +341,9 @@ int gd_multi_pr_run(gd_multi_pr_plan* p, float* const* d_scores, const gd_pr_opt
                  p->err_part.as<double>(), p->gs.as<GraphState>(), p->hub_row.as<int32_t>(),
                  p->hub_slot.as<int32_t>(), p->hub_nseg.as<int32_t>(), p->errors.as<double>(),
                  std::max(1, o.max_iter), o.tol, o.damping};
-  hipEvent_t t0, t1;
-  GD_HIP_TRY(hipEventCreate(&t0));
-  GD_HIP_TRY(hipEventCreate(&t1));
+  EventPair evp;
+  GD_HIP_TRY(evp.create());
+  hipEvent_t t0 = evp.a, t1 = evp.b;
   GD_HIP_TRY(hipEventRecord(t0, p->stream));
   int maxm = 0;
   for (int g = 0; g < G; g++) maxm = std::max(maxm, p->m[g]);
@@ -399,8 +398,6 @@ int gd_multi_pr_run(gd_multi_pr_plan* p, float* const* d_scores, const gd_pr_opt
   GD_HIP_TRY(hipEventSynchronize(t1));
   float ms = 0.0f;
   GD_HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
   if (solve_ms) *solve_ms = ms;
   std::vector<double> errs((size_t)G * std::max(1, o.max_iter));
   GD_HIP_TRY(hipMemcpy(errs.data(), p->errors.ptr, 8 * errs.size(), hipMemcpyDeviceToHost));

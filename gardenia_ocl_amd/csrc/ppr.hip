@@ -713,9 +713,9 @@ int gd_ppr_plan_run(gd_ppr_plan* p, const int32_t* d_seeds, float* d_X,
   const float damp = o.damping, teleport = 1.0f - damp;
   const int chunk = o.chunk > 0 ? o.chunk : (o.tol > 0 ? 4 : std::max(1, o.max_iter));
   const int k = p->k;
-  hipEvent_t t0, t1;
-  GD_HIP_TRY(hipEventCreate(&t0));
-  GD_HIP_TRY(hipEventCreate(&t1));
+  EventPair evp;
+  GD_HIP_TRY(evp.create());
+  hipEvent_t t0 = evp.a, t1 = evp.b;
   PprState* st = p->state.as<PprState>();
   float* X = p->relabel ? p->Xint.as<float>() : d_X;
   const int64_t n = (int64_t)m * k;
@@ -789,8 +789,6 @@ int gd_ppr_plan_run(gd_ppr_plan* p, const int32_t* d_seeds, float* d_X,
   GD_HIP_TRY(hipEventRecord(t1, p->stream));
   GD_HIP_TRY(hipEventSynchronize(t1));
   GD_HIP_TRY(hipEventElapsedTime(&s.solve_ms, t0, t1));
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
   s.iterations = h.iter;
   std::vector<int32_t> its(k);
   std::vector<double> ce(k);

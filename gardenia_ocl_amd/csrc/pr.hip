@@ -223,11 +223,13 @@ int gd_pr_plan_run(gd_pr_plan* p, float* d_scores, const gd_pr_options* opts, gd
   memset(&s, 0, sizeof(s));
   const int m = p->m;
   if (m == 0) {
-    // Empty graph: the reference's loop body does nothing, error 0 < tol.
+    // Empty graph: every iteration of the reference's loop has error 0, so it
+    // stops after the first when 0 < tol and otherwise runs max_iter of them.
     if (o.max_iter > 0) {
-      s.iterations = 1;
       s.converged = 0.0 < o.tol;
-      if (errors) errors[0] = 0.0;
+      s.iterations = s.converged ? 1 : o.max_iter;
+      if (errors)
+        for (int i = 0; i < s.iterations; i++) errors[i] = 0.0;
     }
     if (stats) *stats = s;
     return GD_OK;
@@ -240,9 +242,9 @@ int gd_pr_plan_run(gd_pr_plan* p, float* d_scores, const gd_pr_options* opts, gd
                               hipMemcpyDeviceToHost, p->stream));
     GD_HIP_TRY(hipStreamSynchronize(p->stream));
   }
-  hipEvent_t t0, t1;
-  GD_HIP_TRY(hipEventCreate(&t0));
-  GD_HIP_TRY(hipEventCreate(&t1));
+  EventPair evp;
+  GD_HIP_TRY(evp.create());
+  hipEvent_t t0 = evp.a, t1 = evp.b;
   float* work = p->relabel ? p->s_int.as<float>() : d_scores;
   GD_HIP_TRY(hipEventRecord(t0, p->stream));
   if (p->relabel) {
@@ -264,8 +266,6 @@ int gd_pr_plan_run(gd_pr_plan* p, float* d_scores, const gd_pr_options* opts, gd
   GD_HIP_TRY(hipEventRecord(t1, p->stream));
   GD_HIP_TRY(hipEventSynchronize(t1));
   GD_HIP_TRY(hipEventElapsedTime(&s.solve_ms, t0, t1));
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
   if (stats) *stats = s;
   return GD_OK;
 }

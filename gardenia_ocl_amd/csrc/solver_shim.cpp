@@ -36,8 +36,9 @@ void PRSolver(int m, int nnz, int* in_row_offsets, int* in_column_indices, int* 
                        out_column_indices, degree, scores, &o, &st, errors.data());
   if (rc != GD_OK) fail("PRSolver", rc);
   for (int i = 0; i < st.iterations; i++) printf(" %2d    %lf\n", i + 1, errors[i]);
-  // The reference prints iter+1 after its for-loop: MAX_ITER+1 when it never converged.
-  printf("\titerations = %d.\n", st.iterations + ((st.converged || o.max_iter == 0) ? 0 : 1));
+  // The reference prints iter+1 after its for-loop (src/pr/omp_base.cc:23,46):
+  // the iterations run when one converged, MAX_ITER+1 otherwise (1 for MAX_ITER 0).
+  printf("\titerations = %d.\n", st.iterations + (st.converged ? 0 : 1));
   printf("\truntime [%s] = %f ms.\n", "hip_pull", st.solve_ms);
 }
 

@@ -474,9 +474,9 @@ int gd_symgs_solve(int m, int nnz, const int32_t* Ap, const int32_t* Aj, const i
   rc = gd_symgs_plan_create(m, nnz, dAp.as<int32_t>(), dAj.as<int32_t>(), dAx.as<float>(),
                             didx.as<int32_t>(), color_offsets, n_colors, nullptr, &plan);
   if (rc) return rc;
-  hipEvent_t t0, t1;
-  GD_HIP_TRY(hipEventCreate(&t0));
-  GD_HIP_TRY(hipEventCreate(&t1));
+  EventPair evp;
+  GD_HIP_TRY(evp.create());
+  hipEvent_t t0 = evp.a, t1 = evp.b;
   GD_HIP_TRY(hipEventRecord(t0, nullptr));
   rc = gd_symgs_plan_run(plan, dx.as<float>(), db.as<float>(), 1, nullptr);
   GD_HIP_TRY(hipEventRecord(t1, nullptr));
@@ -484,8 +484,6 @@ int gd_symgs_solve(int m, int nnz, const int32_t* Ap, const int32_t* Aj, const i
   float ms = 0.0f;
   GD_HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
   if (kernel_ms) *kernel_ms = ms;
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
   gd_symgs_plan_destroy(plan);
   if (rc) return rc;
   if (m > 0) GD_HIP_TRY(hipMemcpy(x, dx.ptr, 4 * (size_t)m, hipMemcpyDeviceToHost));

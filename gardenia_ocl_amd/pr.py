@@ -71,7 +71,9 @@ def PRSolver(m: int, nnz: int, in_row_offsets, in_column_indices, out_row_offset
     if verbose:
         for i, e in enumerate(errs):
             print(f" {i + 1:2d}    {e:f}")
-        printed = stats.iterations + (0 if stats.converged or max_iter == 0 else 1)
+        # iter+1 after the reference's loop (src/pr/omp_base.cc:23,46): MAX_ITER+1
+        # when no iteration converged, 1 for max_iter 0
+        printed = stats.iterations + (0 if stats.converged else 1)
         print(f"\titerations = {printed}.")
         print(f"\truntime [{variant}] = {stats.solve_ms:f} ms.")
     return stats.as_dict(), errs

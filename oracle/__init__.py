@@ -219,3 +219,33 @@ def wpr_grad_fp64(m, in_off, in_col, a, x, w, *, damping=0.85, tol=1e-14, max_it
 
 def num_threads() -> int:
     return lib().orc_num_threads()
+
+
+def ref_available() -> bool:
+    """True where oracle/_ref/libref.so (the reference's own OpenMP sources,
+    compiled here by ``make -C oracle ref``) has been built."""
+    return os.path.exists(REF_LIB_PATH)
+
+
+def ref_pr_solve(m, nnz, in_off, in_col, degree, scores, log_path):
+    """The reference's own PRSolver (src/pr/omp_base.cc:9-49, built from
+    source with its compile-time EPSILON 1e-3 and MAX_ITER 30) on host arrays;
+    ``scores`` (float32) is updated in place and the reference's stdout goes to
+    ``log_path``.  Returns (printed iteration count, printed runtime in ms from
+    the reference's own Timer, printed per-iteration L1 values).  The out-CSR
+    arguments are unused by that solver and passed as NULL."""
+    import re
+
+    h = ctypes.CDLL(REF_LIB_PATH)
+    fn = h.ref_pr_solve
+    fn.restype = None
+    fn.argtypes = [_i, _i, _P, _P, _P, _P, _P, _P, ctypes.c_char_p]
+    in_off, in_col, degree = _i32(in_off), _i32(in_col), _i32(degree)
+    assert scores.dtype == np.float32 and scores.flags.c_contiguous
+    fn(m, nnz, _p(in_off), _p(in_col), None, None, _p(degree), _p(scores), os.fsencode(log_path))
+    with open(log_path) as f:
+        text = f.read()
+    its = int(re.findall(r"iterations = (\d+)\.", text)[-1])
+    ms = float(re.findall(r"runtime \[[^\]]*\] = ([0-9.]+) ms\.", text)[-1])
+    errs = [float(e) for e in re.findall(r"^ *\d+ +([0-9.]+)$", text, flags=re.M)]
+    return its, ms, errs

@@ -110,3 +110,4 @@ def test_oracle_is_not_imported_by_the_product():
             if f.endswith((".py", ".cpp", ".hip", ".h")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in text and "liboracle" not in text, f
+

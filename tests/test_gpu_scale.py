@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
+from conftest import L1_GAP, assert_same_iterations
 from gardenia_ocl_amd import PageRankPlan, graph
 
 pytestmark = pytest.mark.gpu
@@ -50,7 +51,7 @@ def test_rmat24_first_iteration(rmat24, device):
     exact = (deg <= 64) | (deg > 256 * plan.info()["items_per_thread"])
     np.testing.assert_array_equal(s[exact], ref[exact])
     assert (np.abs(s - ref) / ref).max() <= 1e-5
-    assert abs(errs[0] - ref_errs[0]) <= 2.4e-7 + 1e-4 * ref_errs[0]
+    assert abs(errs[0] - ref_errs[0]) <= L1_GAP
 
 
 def test_rmat24_twenty_fixed_iterations(rmat24, device):
@@ -61,7 +62,7 @@ def test_rmat24_twenty_fixed_iterations(rmat24, device):
                                   max_iter=20)
     rel = np.abs(s - ref) / ref
     assert rel.max() <= 1e-5, rel.max()
-    assert np.all(np.abs(errs[:20] - ref_errs[:20]) <= 2.4e-7 + 1e-4 * ref_errs[:20])
+    assert_same_iterations(st["iterations"], errs, it, ref_errs, 0.0)
 
 
 def test_rmat24_tol_1e6(rmat24, device):
@@ -70,13 +71,17 @@ def test_rmat24_tol_1e6(rmat24, device):
     ref, it, ref_errs = oracle.pr(ic.m, ic.row_offsets, ic.column_indices, od, tol=1e-6,
                                   max_iter=200)
     assert st["converged"]
-    if not st["near_tie"]:
-        assert st["iterations"] == it
+    tie, gap = assert_same_iterations(st["iterations"], errs, it, ref_errs, 1e-6)
+    # the oracle's last two L1 values are far from tol compared with the gap:
+    # the count is asserted, not excused
+    assert not tie and st["iterations"] == it
     rel = np.abs(s - ref) / ref
     assert rel.max() <= 1e-5, rel.max()
     x64, _ = oracle.pr_fp64(ic.m, ic.row_offsets, ic.column_indices, od, tol=1e-12)
     l1_gpu = np.abs(s.astype(np.float64) - x64).sum()
     l1_ref = np.abs(ref.astype(np.float64) - x64).sum()
-    print(f"RMAT-24 tol 1e-6: {st['iterations']} iterations (oracle {it}), max rel "
-          f"{rel.max():.3e}, L1 vs FP64 gpu {l1_gpu:.3e} reference {l1_ref:.3e}")
+    print(f"RMAT-24 tol 1e-6: {st['iterations']} iterations (oracle {it}), L1 at the stop "
+          f"{errs[it - 1]:.6e} (oracle {ref_errs[it - 1]:.6e}), before it {errs[it - 2]:.6e} "
+          f"(oracle {ref_errs[it - 2]:.6e}), max L1 gap {gap:.2e}, max rel {rel.max():.3e}, "
+          f"L1 vs FP64 gpu {l1_gpu:.3e} reference {l1_ref:.3e}")
     assert l1_gpu <= 1.05 * l1_ref + 1e-7
