@@ -66,7 +66,7 @@ struct WprEpilogue {
 struct WprPolicy {
   // as plain PageRank: with unit weights the forward is its arithmetic exactly
   static constexpr bool kExactHubs = true;
-  static constexpr bool kSeqAll = false;
+  static constexpr bool kSeqAll = true;
   struct Args {
     float* x;
     float* c0;

@@ -76,7 +76,8 @@ __global__ __launch_bounds__(kBlock) void mg_pull_kernel(const MgArgs a) {
   double err = 0.0;
   const PrEpilogue<0> e{a.scores, odd ? a.c0 : a.c1, a.outdeg, a.gs[g].base, a.damp, -1};
   if (t.y >= 0) {
-    err = row_tile<kMgIPT, false, false>(a.off, a.col, nullptr, a.e_max, t.x, t.y, t.z, t.w - t.z, gat, e, lds);
+    err = row_tile<kMgIPT, false, false, decltype(gat), decltype(e), true>(
+        a.off, a.col, nullptr, a.e_max, t.x, t.y, t.z, t.w - t.z, gat, e, lds);
   } else {  // whole hub row in the reference's order, as the single-graph plan
     const auto pre = e.pre(t.x);
     float* const buf = reinterpret_cast<float*>(smem);

@@ -7,10 +7,13 @@
 // chunk of kc = 64*VEC columns, lane l holds columns l*VEC..l*VEC+VEC-1, and
 // the gathered row segment C[u][chunk] is one coalesced 256*VEC-byte load.
 //
-// Each column is summed sequentially in CSR order (the scalar personalised
-// loop's order) except rows longer than kSegEdges, which are split into
-// segments whose partials are added in segment order.  The per-iteration
-// update is base + d*sum with two roundings (built with -ffp-contract=off).
+// Each column of every row is summed sequentially in CSR order, the scalar
+// personalised loop's order and FP32 rounding, so every column is the
+// single-seed run's bit for bit.  Rows longer than kSegEdges (hubs) run
+// beside the tiles: one workgroup per (hub row, 64 columns), three waves
+// gathering chunks of the row into LDS while the fourth adds them in edge
+// order, one column per lane.  The per-iteration update is base + d*sum with
+// two roundings (built with -ffp-contract=off).
 //
 // A persistent grid of workgroups walks the tile list; each workgroup keeps
 // per-column L1 partials for the whole sweep, and two small kernels reduce
@@ -42,7 +45,9 @@ namespace {
 
 constexpr int kPBlock = 256;
 constexpr int kPWaves = kPBlock / 64;
-constexpr int kSegEdges = 4096;   // rows longer than this are split into segments
+constexpr int kSegEdges = 4096;   // rows longer than this are hubs (ppr_hub_chain_kernel)
+constexpr int kHubEdgesW = 32;    // edges a gathering wave loads per hub chunk
+constexpr int kHubChunkE = 3 * kHubEdgesW;  // edges per hub chunk (three gathering waves)
 constexpr int kTileEdges = 1024;  // edges per wavefront tile
 constexpr int kTileRowsP = 256;   // rows per wavefront tile
 constexpr int kUnroll = 8;        // gathers in flight per lane
@@ -144,8 +149,7 @@ __global__ __launch_bounds__(kPBlock) void ppr_pull_kernel(
     const int32_t* __restrict__ col, const int32_t* __restrict__ outdeg,
     const int32_t* __restrict__ seeds, const int32_t* __restrict__ frozen, int ka, int64_t k,
     float* __restrict__ X, float* __restrict__ C0, float* __restrict__ C1,
-    float* __restrict__ seg_part, double* __restrict__ err_part,
-    const PprState* __restrict__ st, float teleport, float damp) {
+    double* __restrict__ err_part, const PprState* __restrict__ st, float teleport, float damp) {
   if (st->done) return;
   const int odd = st->iter & 1;
   const float* __restrict__ C = odd ? C1 : C0;
@@ -166,20 +170,12 @@ __global__ __launch_bounds__(kPBlock) void ppr_pull_kernel(
   if (col0 < ka) {  // lanes past the running columns gather nothing
     for (int t = blockIdx.x * kPWaves + wid; t < n_tiles; t += nw) {
       const Tile tl = tiles[t];
-      if (tl.y >= 0) {
-        for (int r = tl.x; r < tl.y; r++) {
-          float acc[VEC];
-#pragma unroll
-          for (int j = 0; j < VEC; j++) acc[j] = 0.0f;
-          sum_edges<VEC>(col, C, k, col0, off[r], off[r + 1], acc);
-          row_update<VEC>(r, acc, seed, fz, X, Cn, outdeg, k, col0, teleport, damp, err);
-        }
-      } else {
+      for (int r = tl.x; r < tl.y; r++) {
         float acc[VEC];
 #pragma unroll
         for (int j = 0; j < VEC; j++) acc[j] = 0.0f;
-        sum_edges<VEC>(col, C, k, col0, tl.z, tl.w, acc);
-        store_vec<VEC>(seg_part + (int64_t)(-tl.y - 1) * k + col0, acc);
+        sum_edges<VEC>(col, C, k, col0, off[r], off[r + 1], acc);
+        row_update<VEC>(r, acc, seed, fz, X, Cn, outdeg, k, col0, teleport, damp, err);
       }
     }
   }
@@ -196,53 +192,91 @@ __global__ __launch_bounds__(kPBlock) void ppr_pull_kernel(
   }
 }
 
-// Hub rows: segment partials added in segment order, then the row update.
-template <int VEC>
-__global__ __launch_bounds__(kPBlock) void ppr_hub_kernel(
-    const int32_t* __restrict__ hub_row, const int32_t* __restrict__ hub_slot,
-    const int32_t* __restrict__ hub_nseg, int n_hubs, const float* __restrict__ seg_part,
-    const int32_t* __restrict__ outdeg, const int32_t* __restrict__ seeds,
-    const int32_t* __restrict__ frozen, int ka, int64_t k, float* __restrict__ X,
-    float* __restrict__ C0, float* __restrict__ C1, double* __restrict__ hub_err,
-    const PprState* __restrict__ st, float teleport, float damp) {
+// Hub rows: workgroup (h, g) adds hub row hub_row[h] for columns [64 g, 64 g +
+// 64) in edge order, then runs their row updates.  Lane c of wave 0 adds
+// column 64 g + c chunk by chunk from LDS (its own dependent chain: the
+// scalar loop's order and rounding) while waves 1..3 gather the next chunk,
+// kHubEdgesW edges each: one coalesced 256-byte load of C[u][64 g .. 64 g +
+// 63] per edge.  hub_err row h receives the row's per-column |change|.
+__global__ __launch_bounds__(kPBlock) void ppr_hub_chain_kernel(
+    const int32_t* __restrict__ hub_row, const int32_t* __restrict__ off,
+    const int32_t* __restrict__ col, const int32_t* __restrict__ outdeg,
+    const int32_t* __restrict__ seeds, const int32_t* __restrict__ frozen, int ka, int64_t k,
+    float* __restrict__ X, float* __restrict__ C0, float* __restrict__ C1,
+    double* __restrict__ hub_err, const PprState* __restrict__ st, float teleport, float damp) {
   if (st->done) return;
+  __shared__ float buf[2][kHubChunkE * 64];
   const int odd = st->iter & 1;
+  const float* __restrict__ C = odd ? C1 : C0;
   float* __restrict__ Cn = odd ? C0 : C1;
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int col0 = blockIdx.y * 64 * VEC + lane * VEC;
-  int seed[VEC], fz[VEC];
+  const int h = blockIdx.x;
+  const int cc = blockIdx.y * 64 + lane;  // this lane's column (< k: k is a multiple of 64)
+  const int row = hub_row[h];
+  const int e0 = off[row];
+  const int ne = off[row + 1] - e0;
+  const int nchunks = (ne + kHubChunkE - 1) / kHubChunkE;
+  const bool running = blockIdx.y * 64 < ka;  // uniform: a group past the running columns idles
+  // gathering waves: edges [c*kHubChunkE + (wid-1)*kHubEdgesW, ... + kHubEdgesW) of chunk c;
+  // the source ids of chunk c+1 are loaded one chunk ahead
+  const int gw = wid - 1;
+  auto load_ids = [&](int c) {
+    const int e = c * kHubChunkE + gw * kHubEdgesW + lane;
+    return (lane < kHubEdgesW && e < ne) ? col[e0 + e] : 0;
+  };
+  auto produce = [&](int c, int ids, float* dst) {
+    const int base = c * kHubChunkE + gw * kHubEdgesW;
+    const int cnt = min(kHubEdgesW, ne - base);
+    float v[kHubEdgesW];
 #pragma unroll
-  for (int j = 0; j < VEC; j++) {
-    seed[j] = seeds[col0 + j];
-    fz[j] = frozen[col0 + j];
-  }
-  double err[VEC];
+    for (int i = 0; i < kHubEdgesW; i++) {
+      const int u = __builtin_amdgcn_readlane(ids, i);  // 0 past the row: a valid address
+      v[i] = C[(int64_t)u * k + cc];
+    }
 #pragma unroll
-  for (int j = 0; j < VEC; j++) err[j] = 0.0;
-  if (col0 < ka) {
-    for (int h = blockIdx.x * kPWaves + wid; h < n_hubs; h += gridDim.x * kPWaves) {
-      float acc[VEC];
+    for (int i = 0; i < kHubEdgesW; i++)
+      if (i < cnt) dst[(gw * kHubEdgesW + i) * 64 + lane] = v[i];
+  };
+  float s = 0.0f;
+  if (running) {
+    int ids = 0;
+    if (wid != 0 && nchunks > 0) {
+      ids = load_ids(0);
+      const int nxt = nchunks > 1 ? load_ids(1) : 0;
+      produce(0, ids, buf[0]);
+      ids = nxt;
+    }
+    __syncthreads();
+    for (int c = 0; c < nchunks; c++) {
+      if (wid == 0) {
+        const float* __restrict__ b = buf[c & 1];
+        const int nc = min(kHubChunkE, ne - c * kHubChunkE);
+        int e = 0;
+        for (; e + 8 <= nc; e += 8) {
+          float a[8];
 #pragma unroll
-      for (int j = 0; j < VEC; j++) acc[j] = 0.0f;
-      const int s0 = hub_slot[h], ns = hub_nseg[h];
-      for (int q = 0; q < ns; q++) {
-        const Vec<VEC> p = load_vec<VEC>(seg_part + (int64_t)(s0 + q) * k + col0);
+          for (int q = 0; q < 8; q++) a[q] = b[(e + q) * 64 + lane];
 #pragma unroll
-        for (int j = 0; j < VEC; j++) acc[j] += p.v[j];
+          for (int q = 0; q < 8; q++) s += a[q];
+        }
+        for (; e < nc; e++) s += b[e * 64 + lane];
+      } else if (c + 1 < nchunks) {
+        const int nxt = c + 2 < nchunks ? load_ids(c + 2) : 0;  // in flight across the barrier
+        produce(c + 1, ids, buf[(c + 1) & 1]);
+        ids = nxt;
       }
-      row_update<VEC>(hub_row[h], acc, seed, fz, X, Cn, outdeg, k, col0, teleport, damp, err);
+      __syncthreads();
     }
   }
-  __shared__ double s_err[kPWaves][64 * VEC];
-#pragma unroll
-  for (int j = 0; j < VEC; j++) s_err[wid][lane * VEC + j] = err[j];
-  __syncthreads();
-  for (int c = threadIdx.x; c < 64 * VEC; c += kPBlock) {
-    double s = 0.0;
-#pragma unroll
-    for (int w = 0; w < kPWaves; w++) s += s_err[w][c];
-    hub_err[(int64_t)blockIdx.x * k + blockIdx.y * 64 * VEC + c] = s;
+  if (wid == 0) {
+    double err[1] = {0.0};
+    if (cc < ka) {
+      const float acc[1] = {s};
+      const int sd[1] = {seeds[cc]}, fz[1] = {frozen[cc]};
+      row_update<1>(row, acc, sd, fz, X, Cn, outdeg, k, cc, teleport, damp, err);
+    }
+    hub_err[(int64_t)h * k + cc] = err[0];
   }
 }
 
@@ -390,8 +424,16 @@ struct gd_ppr_plan {
   const int32_t* outdeg = nullptr;
   Relabel rl;
   Schedule sched;
-  int n_tiles = 0, n_hubs = 0, grid = 0, hub_grid = 0;
-  DeviceBuffer tiles, hub_row, hub_slot, hub_nseg, seg_part, err_part, hub_err, fin, col_err,
+  int n_tiles = 0, n_hubs = 0, grid = 0;
+  // hub rows run on the plan's stream beside the tiles on `side` (fork / join)
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  ~gd_ppr_plan() {
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
+    if (side) (void)hipStreamDestroy(side);
+  }
+  DeviceBuffer tiles, hub_row, err_part, hub_err, fin, col_err,
       C0, C1, Xint, seeds, state, errors, frozen, col_iters, newcol, cperm;
   int errors_cap = 0;
   bool column_stop = true;
@@ -429,25 +471,24 @@ int ppr_build(gd_ppr_plan* p, const int32_t* d_off, const int32_t* d_col,
                               p->stream));
     GD_HIP_TRY(hipStreamSynchronize(p->stream));
   }
-  // wavefront tiles: <= kTileEdges edges / kTileRowsP rows; rows > kSegEdges split
+  // wavefront tiles: <= kTileEdges edges / kTileRowsP rows; rows > kSegEdges are
+  // hubs (ppr_hub_chain_kernel), longest first
   Schedule& s = p->sched;
   {
     Schedule hubs;
     rc = build_schedule(m, p->nnz, h_off.data(), kSegEdges, m, &hubs);
     if (rc) return rc;
     s.hub_row = hubs.hub_row;
-    s.hub_slot = hubs.hub_slot;
-    s.hub_nseg = hubs.hub_nseg;
-    s.n_slots = hubs.n_slots;
     s.max_row_len = hubs.max_row_len;
+    std::stable_sort(s.hub_row.begin(), s.hub_row.end(), [&](int32_t a, int32_t b) {
+      return h_off[a + 1] - h_off[a] > h_off[b + 1] - h_off[b];
+    });
     // re-tile the non-hub rows with the smaller wavefront tiles
     Schedule small;
     rc = build_schedule(m, p->nnz, h_off.data(), kTileEdges, kTileRowsP, &small);
     if (rc) return rc;
-    // heaviest work first: hub segments, then single rows of (kTileEdges,
-    // kSegEdges] edges, then the packed row tiles
-    for (const Tile& t : hubs.tiles)
-      if (t.y < 0) s.tiles.push_back(t);
+    // heaviest work first: single rows of (kTileEdges, kSegEdges] edges, then
+    // the packed row tiles
     for (size_t h = 0; h < small.hub_row.size(); h++) {
       const int r = small.hub_row[h];
       const int d = h_off[r + 1] - h_off[r];
@@ -462,15 +503,11 @@ int ppr_build(gd_ppr_plan* p, const int32_t* d_off, const int32_t* d_col,
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   p->grid = std::max(1, std::min(cus * 8, (p->n_tiles + kPWaves - 1) / kPWaves));
-  p->hub_grid = std::max(1, std::min(cus * 2, (p->n_hubs + kPWaves - 1) / kPWaves));
   const size_t mk = (size_t)m * p->k;
   rc = p->tiles.alloc(sizeof(Tile) * p->n_tiles);
   if (!rc) rc = p->hub_row.alloc(4 * (size_t)std::max(1, p->n_hubs));
-  if (!rc) rc = p->hub_slot.alloc(4 * (size_t)std::max(1, p->n_hubs));
-  if (!rc) rc = p->hub_nseg.alloc(4 * (size_t)std::max(1, p->n_hubs));
-  if (!rc) rc = p->seg_part.alloc(4 * (size_t)std::max(1, s.n_slots) * p->k);
   if (!rc) rc = p->err_part.alloc(8 * (size_t)p->grid * p->k);
-  if (!rc) rc = p->hub_err.alloc(8 * (size_t)p->hub_grid * p->k);
+  if (!rc) rc = p->hub_err.alloc(8 * (size_t)std::max(1, p->n_hubs) * p->k);
   if (!rc) rc = p->fin.alloc(8 * (size_t)2 * kSlices * p->k);
   if (!rc) rc = p->col_err.alloc(8 * (size_t)p->k);
   if (!rc) rc = p->C0.alloc(4 * mk);
@@ -488,12 +525,10 @@ int ppr_build(gd_ppr_plan* p, const int32_t* d_off, const int32_t* d_col,
   if (p->n_hubs) {
     GD_HIP_TRY(hipMemcpyAsync(p->hub_row.ptr, s.hub_row.data(), 4 * p->n_hubs,
                               hipMemcpyHostToDevice, p->stream));
-    GD_HIP_TRY(hipMemcpyAsync(p->hub_slot.ptr, s.hub_slot.data(), 4 * p->n_hubs,
-                              hipMemcpyHostToDevice, p->stream));
-    GD_HIP_TRY(hipMemcpyAsync(p->hub_nseg.ptr, s.hub_nseg.data(), 4 * p->n_hubs,
-                              hipMemcpyHostToDevice, p->stream));
+    GD_HIP_TRY(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+    GD_HIP_TRY(hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming));
+    GD_HIP_TRY(hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming));
   }
-  GD_HIP_TRY(hipMemsetAsync(p->hub_err.ptr, 0, 8 * (size_t)p->hub_grid * p->k, p->stream));
   GD_HIP_TRY(hipStreamSynchronize(p->stream));
   return GD_OK;
 }
@@ -508,22 +543,32 @@ void ppr_launch_iteration(gd_ppr_plan* p, float* X, PprState* st, float teleport
   float* C0 = p->C0.as<float>();
   float* C1 = p->C1.as<float>();
   const int32_t* frozen = p->frozen.as<int32_t>();
-  hipLaunchKernelGGL(ppr_pull_kernel<VEC>, dim3(p->grid, chunks), dim3(kPBlock), 0, p->stream,
+  // hub rows first, on the plan's stream, so their workgroups are placed before
+  // the tiles fill the CUs; the tiles fork onto the side stream and join back
+  hipStream_t ts = p->stream;
+  if (p->n_hubs) {
+    (void)hipEventRecord(p->fork_ev, p->stream);
+    (void)hipStreamWaitEvent(p->side, p->fork_ev, 0);
+    ts = p->side;
+    hipLaunchKernelGGL(ppr_hub_chain_kernel, dim3(p->n_hubs, (unsigned)(kact / 64)),
+                       dim3(kPBlock), 0, p->stream, p->hub_row.as<int32_t>(), p->off, p->col,
+                       p->outdeg, p->seeds.as<int32_t>(), frozen, ka, (int64_t)p->k, X, C0, C1,
+                       p->hub_err.as<double>(), st, teleport, damp);
+  }
+  hipLaunchKernelGGL(ppr_pull_kernel<VEC>, dim3(p->grid, chunks), dim3(kPBlock), 0, ts,
                      p->tiles.as<Tile>(), p->n_tiles, p->off, p->col, p->outdeg,
                      p->seeds.as<int32_t>(), frozen, ka, (int64_t)p->k, X, C0, C1,
-                     p->seg_part.as<float>(), p->err_part.as<double>(), st, teleport, damp);
-  if (p->n_hubs)
-    hipLaunchKernelGGL(ppr_hub_kernel<VEC>, dim3(p->hub_grid, chunks), dim3(kPBlock), 0,
-                       p->stream, p->hub_row.as<int32_t>(), p->hub_slot.as<int32_t>(),
-                       p->hub_nseg.as<int32_t>(), p->n_hubs, p->seg_part.as<float>(), p->outdeg,
-                       p->seeds.as<int32_t>(), frozen, ka, (int64_t)p->k, X, C0, C1,
-                       p->hub_err.as<double>(), st, teleport, damp);
+                     p->err_part.as<double>(), st, teleport, damp);
+  if (p->n_hubs) {
+    (void)hipEventRecord(p->join_ev, p->side);
+    (void)hipStreamWaitEvent(p->stream, p->join_ev, 0);
+  }
   const dim3 sg((unsigned)((kact + kPBlock - 1) / kPBlock), kSlices);
   double* fin = p->fin.as<double>();
   hipLaunchKernelGGL(ppr_err_slices_kernel, sg, dim3(kPBlock), 0, p->stream,
                      p->err_part.as<double>(), p->grid, (int64_t)p->k, kact, fin, st);
   hipLaunchKernelGGL(ppr_err_slices_kernel, sg, dim3(kPBlock), 0, p->stream,
-                     p->hub_err.as<double>(), p->hub_grid, (int64_t)p->k, kact,
+                     p->hub_err.as<double>(), p->n_hubs, (int64_t)p->k, kact,
                      fin + (size_t)kSlices * p->k, st);
   hipLaunchKernelGGL(ppr_finalize_kernel, dim3(1), dim3(kPBlock), 0, p->stream, fin,
                      (int64_t)p->k, kact, p->col_err.as<double>(), p->frozen.as<int32_t>(),

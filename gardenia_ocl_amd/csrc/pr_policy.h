@@ -47,7 +47,10 @@ struct PrPolicy {
   // hub rows in the reference's order and rounding: its sequential FP32 sum of
   // a 10^5-edge row drifts by ~1e-4 relative, which a tree sum does not match
   static constexpr bool kExactHubs = true;
-  static constexpr bool kSeqAll = false;
+  // every row of a row tile summed by its owner thread in edge order (long
+  // rows by a pipelined one-lane chain): with the exact hubs, every row of
+  // the plan is the reference's, bit for bit
+  static constexpr bool kSeqAll = true;
   struct Args {
     float* scores;
     float* c0;

@@ -78,9 +78,60 @@ __device__ __forceinline__ T block_sum(T v, T* s_tmp) {
   return t;
 }
 
-// Sequential sum s + v[b] + ... + v[e-1] from LDS, 4 reads in flight.
+// s + v[0] + ... + v[n-1] in order with FP32 rounding, by the calling lane,
+// from 16-byte aligned LDS: 4*kQ values per step, the next step's LDS reads in
+// flight while the current step is added (the adds are one dependent chain).
+template <int kQ>  // float4 reads per step
+__device__ __forceinline__ float lane_seq_sum(float s, const float* __restrict__ v, int n) {
+  const float4* __restrict__ v4 = reinterpret_cast<const float4*>(v);
+  const int nb = n / (4 * kQ);
+  float4 A[kQ], B[kQ];
+  auto load = [&](float4(&R)[kQ], int blk) {
+#pragma unroll
+    for (int k = 0; k < kQ; k++) R[k] = v4[blk * kQ + k];
+  };
+  auto add = [&](const float4(&R)[kQ]) {
+#pragma unroll
+    for (int k = 0; k < kQ; k++) {
+      s += R[k].x;
+      s += R[k].y;
+      s += R[k].z;
+      s += R[k].w;
+    }
+  };
+  if (nb > 0) {
+    load(A, 0);
+    int b = 0;
+    // scheduling barriers keep each block's reads issued before the previous
+    // block's adds, so the adds wait only for reads issued one block earlier
+    for (; b + 2 <= nb; b += 2) {
+      load(B, b + 1);
+      __builtin_amdgcn_sched_barrier(0);
+      add(A);
+      __builtin_amdgcn_sched_barrier(0);
+      load(A, min(b + 2, nb - 1));  // unconditional (clamped): A and B keep their registers
+      __builtin_amdgcn_sched_barrier(0);
+      add(B);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (b < nb) add(A);
+  }
+  for (int i = nb * 4 * kQ; i < n; i++) s += v[i];
+  return s;
+}
+
+// Sequential sum s + v[b] + ... + v[e-1] from LDS (`v` 16-byte aligned) by
+// the calling lane, in order with FP32 rounding.  More than kChainRow terms:
+// scalar adds up to a 16-byte boundary, then float4 reads 16 terms ahead of
+// the dependent adds (lane_seq_sum); fewer: 4 reads in flight.
+constexpr int kChainRow = 32;
+
 __device__ __forceinline__ float seq_sum(float s, const float* __restrict__ v, int b, int e) {
   int k = b;
+  if (e - b > kChainRow) {
+    for (; k & 3; k++) s += v[k];
+    return lane_seq_sum<4>(s, v + k, e - k);
+  }
   for (; k + 4 <= e; k += 4) {
     const float a0 = v[k], a1 = v[k + 1], a2 = v[k + 2], a3 = v[k + 3];
     s += a0;
@@ -144,7 +195,7 @@ __device__ __forceinline__ float segment_sum(const int32_t* __restrict__ col,
 // LDS of one row tile.
 template <int IPT>
 struct RowTileLds {
-  float val[kBlock * IPT];    // gathered per-edge values
+  alignas(16) float val[kBlock * IPT];  // gathered per-edge values
   int off[kTileRows + 1];     // row offsets, rebased to the tile on use
   float longsum[kTileRows];   // wavefront sums of rows longer than kSeqRow
 };
@@ -630,48 +681,6 @@ __device__ __forceinline__ float exact_seq_sum(const int32_t* __restrict__ col, 
 // term (a dependent v_add_f32 every ~4.8 cycles when the terms sit in
 // registers; the LDS reads feeding it take the rest), independent of the
 // terms' signs, ties or magnitudes.
-
-// s + v[0] + ... + v[n-1] in order with FP32 rounding, by the calling lane,
-// from 16-byte aligned LDS: 4*kQ values per step, the next step's LDS reads in
-// flight while the current step is added (the adds are one dependent chain).
-template <int kQ>  // float4 reads per step
-__device__ __forceinline__ float lane_seq_sum(float s, const float* __restrict__ v, int n) {
-  const float4* __restrict__ v4 = reinterpret_cast<const float4*>(v);
-  const int nb = n / (4 * kQ);
-  float4 A[kQ], B[kQ];
-  auto load = [&](float4(&R)[kQ], int blk) {
-#pragma unroll
-    for (int k = 0; k < kQ; k++) R[k] = v4[blk * kQ + k];
-  };
-  auto add = [&](const float4(&R)[kQ]) {
-#pragma unroll
-    for (int k = 0; k < kQ; k++) {
-      s += R[k].x;
-      s += R[k].y;
-      s += R[k].z;
-      s += R[k].w;
-    }
-  };
-  if (nb > 0) {
-    load(A, 0);
-    int b = 0;
-    // scheduling barriers keep each block's reads issued before the previous
-    // block's adds, so the adds wait only for reads issued one block earlier
-    for (; b + 2 <= nb; b += 2) {
-      load(B, b + 1);
-      __builtin_amdgcn_sched_barrier(0);
-      add(A);
-      __builtin_amdgcn_sched_barrier(0);
-      load(A, min(b + 2, nb - 1));  // unconditional (clamped): A and B keep their registers
-      __builtin_amdgcn_sched_barrier(0);
-      add(B);
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    if (b < nb) add(A);
-  }
-  for (int i = nb * 4 * kQ; i < n; i++) s += v[i];
-  return s;
-}
 
 // s0 + the gathered values of edges [e0, e0+ne) in edge order, rounded as the
 // sequential FP32 loop, with the workgroup split by role: lane 0 of wave 0

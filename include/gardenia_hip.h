@@ -51,7 +51,7 @@ typedef struct gd_pr_stats {
   float pull_ms;        /* summed pull-kernel time (profile != 0), else 0 */
   int pull_launches;    /* pull kernels timed */
   int near_tie;         /* 1 if one of the last two L1 changes lies within
-                           5e-8 + 1e-6*tol of tol: the measured difference between
+                           1e-7 + 1e-6*tol of tol: the measured difference between
                            this library's and the reference's L1 values (rows of
                            65-4096 in-edges are tree-summed), where a stop decision
                            could fall one iteration apart */

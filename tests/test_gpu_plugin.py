@@ -86,14 +86,9 @@ def test_cpp_prsolver_prints_the_reference_lines(golden, device, ds):
     ref_errs = arrays[f"{key}/pr_directed/errors_printed"]
     assert printed == meta["datasets"][ds]["pr_directed"]["iterations_printed"], text
     assert [i for i, _ in lines] == list(range(1, len(ref_errs) + 1)), text
-    # %lf prints 6 decimals; rows of > 64 in-edges are tree-summed (4.mtx has some)
-    slack = 5.1e-7 if in_csr.degree.max() <= 64 else 1.6e-6
-    np.testing.assert_allclose([e for _, e in lines], ref_errs, atol=slack)
-    ref = arrays[f"{key}/pr_directed/scores"]
-    if in_csr.degree.max() <= 64:
-        np.testing.assert_array_equal(scores, ref)
-    else:
-        np.testing.assert_allclose(scores, ref, rtol=1e-6, atol=0)
+    # %lf prints 6 decimals
+    np.testing.assert_allclose([e for _, e in lines], ref_errs, atol=5.1e-7)
+    np.testing.assert_array_equal(scores, arrays[f"{key}/pr_directed/scores"])
 
 
 def test_cpp_prsolver_iteration_rule_when_not_converged(device):

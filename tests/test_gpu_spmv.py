@@ -208,3 +208,22 @@ def test_spmv_sliced_refresh_values_and_fallback(device):
     assert plan.mode == "tiles"
     np.testing.assert_array_equal(
         y, oracle.spmv(m, un.row_offsets, un.column_indices, Ax, xx, np.zeros(m, np.float32)))
+
+
+@pytest.mark.parametrize("mode", ["auto", "tiles"])
+def test_spmv_er22_config_size_bit_exact(device, mode):
+    """The SpMV workload at its measured size: ER 2^22 x 2^22, 32 uniform
+    columns per row (134,217,728 nonzeros), Ax and x ~ U(-1,1), y0 = 0, the
+    bench's seeds; y += A x bit-identical to the oracle's restatement of
+    src/spmv/omp_base.cc:17-28 in both modes (auto picks the XCD-sliced mode
+    here)."""
+    er = graph.er_matrix(22, 32, seed=1)
+    assert er.nnz == 32 << 22
+    x = graph.uniform_pm1(er.m, 2)
+    y, plan, _ = _gpu_spmv_mode(device, er, er.weight, x, np.zeros(er.m, np.float32), mode,
+                                reps=2)
+    assert plan.mode == ("sliced" if mode == "auto" else "tiles")
+    ref = np.zeros(er.m, np.float32)
+    for _ in range(2):
+        ref = oracle.spmv(er.m, er.row_offsets, er.column_indices, er.weight, x, ref)
+    np.testing.assert_array_equal(y, ref)
