@@ -172,6 +172,8 @@ SIGNATURES = {
     "gd_pr_plan_destroy": (_i, [_P]),
     "gd_pr_solve": (_i, [_i, _i, _P, _P, _P, _P, _P, _P, ctypes.POINTER(PrOptions),
                          ctypes.POINTER(PrStats), _P]),
+    "gd_pr_run_noalloc": (_i, [_i, _i64, _P, _P, _P, _P, ctypes.POINTER(PrOptions), _P,
+                               ctypes.POINTER(PrStats), _P]),
     "gd_ppr_default_plan_config": (None, [ctypes.POINTER(PprPlanConfig)]),
     "gd_ppr_plan_create": (_i, [_i, _i64, _P, _P, _P, _i, _P, ctypes.POINTER(PprPlanConfig),
                                 ctypes.POINTER(_P)]),

@@ -170,6 +170,26 @@ int gd_pr_solve(int m, int nnz, const int32_t *in_row_offsets, const int32_t *in
                 const int32_t *degree, float *scores, const gd_pr_options *opts,
                 gd_pr_stats *stats, double *errors);
 
+/* Pull PageRank that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * iteration, options, stats and L1 log as gd_pr_plan_run over the same
+ * device-resident in-CSR, out-degree and scores, with every row summed in
+ * CSR order (the reference's sequential FP32 sum) and no plan: the contrib
+ * vectors, L1 partials, L1 log and iteration state live in a static device
+ * array of about 16 MB per GPU, held until the process ends.  Limits:
+ * m <= GD_NOALLOC_MAX_VERTICES, max_iter <= GD_NOALLOC_MAX_ITER (else
+ * GD_ERR_INVALID, before any device work).  The CSR is validated on the device
+ * first (GD_ERR_INVALID for bad offsets or columns).  Calls that overlap on
+ * one GPU share the array and race.  Synchronises `stream`; stats->solve_ms
+ * is host wall time of the call. */
+#define GD_NOALLOC_MAX_VERTICES (1 << 21)
+#define GD_NOALLOC_MAX_ITER 4096
+int gd_pr_run_noalloc(int m, int64_t nnz, const int32_t *d_in_row_offsets,
+                      const int32_t *d_in_column_indices, const int32_t *d_out_degree,
+                      float *d_scores, const gd_pr_options *opts, void *stream,
+                      gd_pr_stats *stats, double *errors);
+
 /* ------------------------------------------------------------ batched personalised PageRank */
 
 /* k personalised PageRank vectors at once (no counterpart in the reference,

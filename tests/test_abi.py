@@ -111,3 +111,20 @@ def test_oracle_is_not_imported_by_the_product():
                 text = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in text and "liboracle" not in text, f
 
+
+def test_noalloc_limits_rejected_before_device():
+    """gd_pr_run_noalloc rejects sizes beyond its static scratch before any
+    device work (so also here, without a GPU)."""
+    from gardenia_ocl_amd import _native as N
+    from gardenia_ocl_amd.pr import make_options
+
+    lib = N.lib()
+    o = make_options(tol=1e-3, max_iter=10)
+    big = (1 << 21) + 1
+    assert lib.gd_pr_run_noalloc(big, 0, None, None, None, None, ctypes.byref(o), None, None,
+                                 None) == N.GD_ERR_INVALID
+    assert b"GD_NOALLOC_MAX_VERTICES" in lib.gd_last_error()
+    o.max_iter = 4097
+    assert lib.gd_pr_run_noalloc(4, 0, None, None, None, None, ctypes.byref(o), None, None,
+                                 None) == N.GD_ERR_INVALID
+    assert b"max_iter" in lib.gd_last_error()
