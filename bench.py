@@ -395,7 +395,7 @@ def bench_small(scale, tol, max_iter, seed, reps=7):
     return rec, (s.cpu().numpy(), errs, its)
 
 
-def bench_ppr(scale, k, tol, seed, fp64_seeds):
+def bench_ppr(scale, k, tol, seed, fp64_seeds, hub_cols=0):
     """Batched PPR (k seeds in one SpMM-style pull) vs k looped single-seed
     runs of the plain pull kernel, both to the same per-seed L1 tolerance."""
     import numpy as np
@@ -412,7 +412,7 @@ def bench_ppr(scale, k, tol, seed, fp64_seeds):
     col = torch.from_numpy(ic.column_indices).to(dev)
     deg = torch.from_numpy(od).to(dev)
     tseeds = torch.from_numpy(seeds).to(dev)
-    bp = PersonalizedPageRankPlan(off, col, deg, k)
+    bp = PersonalizedPageRankPlan(off, col, deg, k, hub_cols=hub_cols)
     X = bp.one_hot(seeds)
     bp.run(tseeds, X, tol=tol, max_iter=2)  # warm-up
     X = bp.one_hot(seeds)
@@ -458,6 +458,7 @@ def bench_ppr(scale, k, tol, seed, fp64_seeds):
             "speedup": round(t_looped / t_batched, 2), "batched_iterations": bst["iterations"],
             "batched_column_iterations_total": col_its,
             "looped_iterations_total": its, "k_chunk": bp.info()["k_chunk"],
+            "hub_cols": bp.info()["hub_cols"],
             "l1_vs_fp64_batched": l1_b, "l1_vs_fp64_looped": l1_l,
             "fp64_seeds_checked": fp64_seeds}
 
@@ -694,17 +695,19 @@ def main():
     ap.add_argument("--nt", type=int, default=-1, help="non-temporal level 0/1/2 (-1 auto)")
     ap.add_argument("--persistent", choices=["auto", "on", "off"], default="auto")
     ap.add_argument("--sort-tiles", choices=["auto", "on", "off"], default="auto")
+    ap.add_argument("--block", type=int, default=0, help="threads per row tile (256/1024, 0=auto)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the tol-1e-6 run, SpMV (ER 2^22 x 32), batched-PPR (RMAT-22, "
                          "k=256), multi-graph and adjoint figures")
     ap.add_argument("--ppr-scale", type=int, default=22)
     ap.add_argument("--ppr-k", type=int, default=256)
     ap.add_argument("--ppr-tol", type=float, default=1e-6)
+    ap.add_argument("--ppr-hub-cols", type=int, default=0, help="PPR hub-row column group (16/32/64)")
     ap.add_argument("--spmv-log2", type=int, default=22)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (scales rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
-    ap.add_argument("--only", choices=["symgs", "csr_build", "spmv", "small"], default=None,
+    ap.add_argument("--only", choices=["symgs", "csr_build", "spmv", "small", "ppr"], default=None,
                     help="run just this secondary figure and print its JSON (for profiling)")
     ap.add_argument("--tol-run", type=float, default=1e-6,
                     help="L1 tolerance of the extra run to convergence (with its FP64 L1)")
@@ -726,6 +729,10 @@ def main():
         return
     if args.only == "small":
         print(json.dumps({"small": bench_small(16, 1e-4, 100, args.seed)[0]}), flush=True)
+        return
+    if args.only == "ppr":
+        print(json.dumps({"ppr": bench_ppr(args.ppr_scale, args.ppr_k, args.ppr_tol, args.seed, 0,
+                                           args.ppr_hub_cols)}), flush=True)
         return
     if args.only == "spmv":
         print(json.dumps({"spmv": bench_spmv(args.spmv_log2, 32, 50, args.seed)[0]}), flush=True)
@@ -761,7 +768,8 @@ def main():
     plan = PageRankPlan(off, col, deg, stream=stream, relabel=relabel, items_per_thread=args.ipt,
                         nontemporal=None if args.nt < 0 else args.nt,
                         persistent={"auto": None, "on": True, "off": False}[args.persistent],
-                        sort_tiles={"auto": None, "on": True, "off": False}[args.sort_tiles])
+                        sort_tiles={"auto": None, "on": True, "off": False}[args.sort_tiles],
+                        block=args.block)
     info = plan.info()
     log(f"plan: {info}")
     init = float(np.float32(1.0) / np.float32(V))
@@ -795,7 +803,8 @@ def main():
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(args.scale),
         "kernel": (f"{'pull_persistent_kernel' if info['persistent'] else 'pull_tile_kernel'}"
                    f"<PrPolicy, {info['items_per_thread']}, {info['nontemporal']}, "
-                   f"{'true' if info['sort_tiles'] else 'false'}>"),
+                   f"{'true' if info['sort_tiles'] else 'false'}"
+                   f"{', ' + str(info['block']) if info['block'] != 256 else ''}>"),
         "avg_launch_ms": round(avg_pull_ms, 5),
         "algorithmic_bytes_per_launch": bytes_iter,
     }
@@ -867,6 +876,7 @@ def main():
                    "relabel": bool(info["relabel"]), "items_per_thread": info["items_per_thread"],
                    "nontemporal": info["nontemporal"], "persistent": bool(info["persistent"]),
                    "pull_grid": info["pull_grid"], "sort_tiles": bool(info["sort_tiles"]),
+                   "block": info["block"],
                    "plan_ms": round(info["plan_ms"], 1)},
         "roofline": roof, "cpu_baseline": cpu, "production": production,
     }

@@ -74,7 +74,7 @@ class PrStats(ctypes.Structure):
 class PrPlanConfig(ctypes.Structure):
     _fields_ = [("relabel", ctypes.c_int), ("items_per_thread", ctypes.c_int),
                 ("nontemporal", ctypes.c_int), ("persistent", ctypes.c_int),
-                ("sort_tiles", ctypes.c_int)]
+                ("sort_tiles", ctypes.c_int), ("block", ctypes.c_int)]
 
 
 class PrPlanInfo(ctypes.Structure):
@@ -90,6 +90,7 @@ class PrPlanInfo(ctypes.Structure):
         ("pull_grid", ctypes.c_int32),
         ("plan_ms", ctypes.c_float),
         ("sort_tiles", ctypes.c_int32),
+        ("block", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -115,7 +116,8 @@ class DeviceCsr(ctypes.Structure):
 
 class PprPlanConfig(ctypes.Structure):
     _fields_ = [("relabel", ctypes.c_int), ("k_chunk", ctypes.c_int),
-                ("column_stop", ctypes.c_int), ("compact", ctypes.c_int)]
+                ("column_stop", ctypes.c_int), ("compact", ctypes.c_int),
+                ("hub_cols", ctypes.c_int)]
 
 
 class PprPlanInfo(ctypes.Structure):
@@ -127,6 +129,7 @@ class PprPlanInfo(ctypes.Structure):
         ("relabel", ctypes.c_int32),
         ("grid", ctypes.c_int32),
         ("plan_ms", ctypes.c_float),
+        ("hub_cols", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:

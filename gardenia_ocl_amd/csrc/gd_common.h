@@ -48,12 +48,11 @@ struct Tile {
 // than exact_len is instead one whole-row hub tile (summed in edge order by one
 // workgroup); the longest of them come first, the rest are spread evenly
 // among the row tiles.
-// Whole-row exact hubs (exact_hub_kernel): rows of up to kScanRow edges are
-// added by one lane from LDS chunks of kHubChunk edges, each chunk stored
-// sorted by source (the plan's tile sort); longer rows by the workgroup's
-// binade scan, stored in edge order.
+// Whole-row exact hubs of PageRank-like plans are summed by the workgroup's
+// binade scan (exact_row_sum); long rows of the SpMV / SymGS / many-graph /
+// no-allocation paths by one lane from LDS chunks of kHubChunk edges
+// (hub_chain_sum).
 constexpr int kHubChunk = 1536;    // 192 gathering lanes x 8 edges
-constexpr int kScanRow = 1 << 17;
 
 struct Schedule {
   std::vector<Tile> tiles;

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "gardenia_hip.h"
@@ -17,11 +18,20 @@
 
 namespace gd {
 
+// Policies that may run row tiles on workgroups wider than kBlock threads
+// (static constexpr bool kWideTiles = true); the others always use kBlock.
+template <class P, class = void>
+struct wide_tiles : std::false_type {};
+template <class P>
+struct wide_tiles<P, std::void_t<decltype(P::kWideTiles)>> : std::bool_constant<P::kWideTiles> {};
+
 struct IterEngine {
   int m = 0;
   int64_t nnz = 0;
   hipStream_t stream = nullptr;
   int ipt = 8;               // edges per thread per tile (8 or 16)
+  int block = kBlock;        // threads per row-tile workgroup: kBlock, or 1024 (wide_tiles
+                             // policies): tiles of block*ipt edges and 4*block rows
   int nt = 1;                // cache-hint level (see gd_pr_plan_config.nontemporal)
   bool persistent = true;
   int exact_len = 0;         // > 0: rows longer than this are whole-row exact hub tiles
@@ -35,10 +45,6 @@ struct IterEngine {
   Schedule sched;
   int n_tiles = 0, n_hubs = 0, n_hub_tiles = 0, fin_blocks = 1;
   bool fused = false;  // the pull launches finish each iteration (no finalize launch)
-  // Hub chains read 16 float4 ahead (176 VGPRs) when the longest chained hub
-  // row, not the row tiles, bounds the pull; else 8 ahead (132 VGPRs), which
-  // lets a hub workgroup share a CU with four row-tile waves per SIMD.
-  bool deep_hub_chain = false;
   DeviceBuffer tiles, hub_tiles, hub_row, hub_slot, hub_nseg, hub_part, err_part, fin_part, state,
       errors;
   int errors_cap = 0;
@@ -102,7 +108,8 @@ struct IterEngine {
     GD_HIP_TRY(hipMemcpyAsync(h_off.data(), off, 4 * ((size_t)m + 1), hipMemcpyDeviceToHost,
                               stream));
     GD_HIP_TRY(hipStreamSynchronize(stream));
-    rc = build_schedule(m, nnz, h_off.data(), kBlock * ipt, kTileRows, &sched, exact_len);
+    rc = build_schedule(m, nnz, h_off.data(), block * ipt, kRowsPerThread * block, &sched,
+                        exact_len);
     if (rc) return rc;
     n_hubs = (int)sched.hub_row.size();
     rc = pos.alloc(sort_tiles ? 2 * (size_t)std::max<int64_t>(nnz, 1) : 2);
@@ -114,8 +121,8 @@ struct IterEngine {
         if (rc) return rc;
         dst = col_sorted.as<int32_t>();
       }
-      // exact hub rows: sorted in the chunks their kernel adds, or kept in order
-      rc = build_tile_sort(sched, kBlock * ipt, col, dst, pos.as<uint16_t>(), stream,
+      // exact hub rows are kept in edge order
+      rc = build_tile_sort(sched, block * ipt, col, dst, pos.as<uint16_t>(), stream,
                            exact_len > 0);
       if (rc) return rc;
       col = dst;
@@ -128,15 +135,6 @@ struct IterEngine {
     sched.tiles = row_tiles;
     n_tiles = (int)row_tiles.size();
     n_hub_tiles = (int)hubs.size();
-    {
-      // Estimates measured on the MI355X: ~3.5 ns per chained term, row tiles
-      // at ~5 ps per edge (~200 G gathered edges/s).
-      int64_t chain = 0, tile_edges = 0;
-      for (const Tile& t : hubs)
-        if (t.w - t.z <= kScanRow) chain = std::max<int64_t>(chain, t.w - t.z);
-      for (const Tile& t : row_tiles) tile_edges += t.w - t.z;
-      deep_hub_chain = chain * 3500 > tile_edges * 5;
-    }
     if (n_hub_tiles) {
       rc = hub_tiles.alloc(sizeof(Tile) * n_hub_tiles);
       if (rc) return rc;
@@ -186,7 +184,8 @@ struct IterEngine {
     // small iterations end in their last pull workgroup (fused_finish): no hub
     // segments to close, one finalize workgroup's worth of partials
     const int parts = (n_tiles > 0 ? pull_grid : 0) + n_hub_tiles;
-    fused = !persistent && (exact_len > 0 || n_hubs == 0) && parts > 0 && parts < 2 * kBlock;
+    fused = !persistent && block == kBlock && (exact_len > 0 || n_hubs == 0) && parts > 0 &&
+            parts < 2 * kBlock;
     return GD_OK;
   }
 
@@ -204,6 +203,13 @@ struct IterEngine {
     return FinArgs{hub_row.as<int32_t>(), hub_slot.as<int32_t>(), hub_nseg.as<int32_t>(),
                    exact_len > 0 ? 0 : n_hubs, (n_tiles > 0 ? pull_grid : 0) + n_hub_tiles,
                    errors.as<double>(), fin_part.as<double>()};
+  }
+
+  template <class P, int IPT, int NT, bool SORTED>
+  void launch_wide(const typename P::Args& a, const IterArgs& it, hipStream_t ts) {
+    if constexpr (wide_tiles<P>::value)
+      hipLaunchKernelGGL((pull_tile_kernel<P, IPT, NT, SORTED, 1024>), dim3(n_tiles), dim3(1024),
+                         0, ts, a, it);
   }
 
   template <class P, int IPT, int NT, bool SORTED>
@@ -226,19 +232,16 @@ struct IterEngine {
       ts = side;
     }
     if (n_hub_tiles) {
-      if constexpr (P::kExactHubs) {
-        if (deep_hub_chain)
-          hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED, 16>), dim3(n_hub_tiles),
-                             dim3(kBlock), 0, stream, a, it);
-        else
-          hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED, 8>), dim3(n_hub_tiles),
-                             dim3(kBlock), 0, stream, a, it);
-      }
+      if constexpr (P::kExactHubs)
+        hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED>), dim3(n_hub_tiles),
+                           dim3(kBlock), 0, stream, a, it);
     }
     if (n_tiles > 0) {
       if (persistent)
         hipLaunchKernelGGL((pull_persistent_kernel<P, IPT, NT, SORTED>), dim3(pull_grid),
                            dim3(kBlock), 0, ts, a, it);
+      else if (wide_tiles<P>::value && block == 1024)
+        launch_wide<P, IPT, NT, SORTED>(a, it, ts);
       else
         hipLaunchKernelGGL((pull_tile_kernel<P, IPT, NT, SORTED>), dim3(n_tiles), dim3(kBlock), 0,
                            ts, a, it);

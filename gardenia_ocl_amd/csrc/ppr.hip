@@ -10,7 +10,7 @@
 // Each column of every row is summed sequentially in CSR order, the scalar
 // personalised loop's order and FP32 rounding, so every column is the
 // single-seed run's bit for bit.  Rows longer than kSegEdges (hubs) run
-// beside the tiles: one workgroup per (hub row, 64 columns), three waves
+// beside the tiles: one workgroup per (hub row, 16 columns), three waves
 // gathering chunks of the row into LDS while the fourth adds them in edge
 // order, one column per lane.  The per-iteration update is base + d*sum with
 // two roundings (built with -ffp-contract=off).
@@ -46,8 +46,6 @@ namespace {
 constexpr int kPBlock = 256;
 constexpr int kPWaves = kPBlock / 64;
 constexpr int kSegEdges = 4096;   // rows longer than this are hubs (ppr_hub_chain_kernel)
-constexpr int kHubEdgesW = 32;    // edges a gathering wave loads per hub chunk
-constexpr int kHubChunkE = 3 * kHubEdgesW;  // edges per hub chunk (three gathering waves)
 constexpr int kTileEdges = 1024;  // edges per wavefront tile
 constexpr int kTileRowsP = 256;   // rows per wavefront tile
 constexpr int kUnroll = 8;        // gathers in flight per lane
@@ -192,91 +190,113 @@ __global__ __launch_bounds__(kPBlock) void ppr_pull_kernel(
   }
 }
 
-// Hub rows: workgroup (h, g) adds hub row hub_row[h] for columns [64 g, 64 g +
-// 64) in edge order, then runs their row updates.  Lane c of wave 0 adds
-// column 64 g + c chunk by chunk from LDS (its own dependent chain: the
-// scalar loop's order and rounding) while waves 1..3 gather the next chunk,
-// kHubEdgesW edges each: one coalesced 256-byte load of C[u][64 g .. 64 g +
-// 63] per edge.  hub_err row h receives the row's per-column |change|.
+// Hub rows: workgroup b adds hub row hub_row[b / G] for the kHubCols columns
+// (16, 32 or 64: gd_ppr_plan_config.hub_cols) of group g = b % G (all groups
+// of the longest row are dispatched first) in
+// edge order, then runs their row updates.  Lane c < kHubCols of wave 0 adds
+// column kHubCols*g + c chunk by chunk from LDS (its own dependent chain: the
+// scalar loop's order and rounding) while waves 1..3 gather the next chunk.
+// A gathering instruction loads 64/kHubCols edges' column groups at once
+// (lane l: edge l / kHubCols, column l % kHubCols), so each gathering wave
+// keeps kHubLoads * 64/kHubCols edges in flight: the chain of a 10^5-edge row
+// is bound by load latency, and narrower column groups put more of its edges
+// in flight (and spread the row over more workgroups).  hub_err row b / G
+// receives the row's per-column |change|.
+constexpr int kHubLoads = 32;  // gathering instructions per wave per chunk
+
+template <int kHubCols>  // columns per workgroup: 16, 32 or 64
 __global__ __launch_bounds__(kPBlock) void ppr_hub_chain_kernel(
-    const int32_t* __restrict__ hub_row, const int32_t* __restrict__ off,
+    const int32_t* __restrict__ hub_row, int groups, const int32_t* __restrict__ off,
     const int32_t* __restrict__ col, const int32_t* __restrict__ outdeg,
     const int32_t* __restrict__ seeds, const int32_t* __restrict__ frozen, int ka, int64_t k,
     float* __restrict__ X, float* __restrict__ C0, float* __restrict__ C1,
     double* __restrict__ hub_err, const PprState* __restrict__ st, float teleport, float damp) {
+  constexpr int kHubEpi = 64 / kHubCols;           // edges per gathering instruction
+  constexpr int kHubEdgesW = kHubLoads * kHubEpi;  // edges per gathering wave per chunk
+  constexpr int kHubChunkE = 3 * kHubEdgesW;       // edges per chunk
+  constexpr int kHubIdRegs = (kHubEdgesW + 63) / 64;
   if (st->done) return;
-  __shared__ float buf[2][kHubChunkE * 64];
+  __shared__ float buf[2][kHubChunkE * kHubCols];
   const int odd = st->iter & 1;
   const float* __restrict__ C = odd ? C1 : C0;
   float* __restrict__ Cn = odd ? C0 : C1;
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int h = blockIdx.x;
-  const int cc = blockIdx.y * 64 + lane;  // this lane's column (< k: k is a multiple of 64)
+  const int h = blockIdx.x / groups;
+  const int g0 = (blockIdx.x % groups) * kHubCols;  // first column of the group (< k)
   const int row = hub_row[h];
   const int e0 = off[row];
   const int ne = off[row + 1] - e0;
   const int nchunks = (ne + kHubChunkE - 1) / kHubChunkE;
-  const bool running = blockIdx.y * 64 < ka;  // uniform: a group past the running columns idles
-  // gathering waves: edges [c*kHubChunkE + (wid-1)*kHubEdgesW, ... + kHubEdgesW) of chunk c;
-  // the source ids of chunk c+1 are loaded one chunk ahead
-  const int gw = wid - 1;
-  auto load_ids = [&](int c) {
-    const int e = c * kHubChunkE + gw * kHubEdgesW + lane;
-    return (lane < kHubEdgesW && e < ne) ? col[e0 + e] : 0;
-  };
-  auto produce = [&](int c, int ids, float* dst) {
-    const int base = c * kHubChunkE + gw * kHubEdgesW;
-    const int cnt = min(kHubEdgesW, ne - base);
-    float v[kHubEdgesW];
+  const bool running = g0 < ka;  // uniform: a group past the running columns idles
+  const int gw = wid - 1;  // gathering wave 0..2
+  const int my_col = g0 + (lane % kHubCols);
+  // source ids of this wave's edges of chunk c: lane l holds edge l + 64 q
+  auto load_ids = [&](int c, int (&ids)[kHubIdRegs]) {
 #pragma unroll
-    for (int i = 0; i < kHubEdgesW; i++) {
-      const int u = __builtin_amdgcn_readlane(ids, i);  // 0 past the row: a valid address
-      v[i] = C[(int64_t)u * k + cc];
+    for (int q = 0; q < kHubIdRegs; q++) {
+      const int e = c * kHubChunkE + gw * kHubEdgesW + q * 64 + lane;
+      ids[q] = (q * 64 + lane < kHubEdgesW && e < ne) ? col[e0 + e] : 0;
+    }
+  };
+  auto produce = [&](int c, const int (&ids)[kHubIdRegs], float* dst) {
+    const int base = c * kHubChunkE + gw * kHubEdgesW;  // first edge of this wave's part
+    const int cnt = min(kHubEdgesW, ne - base);
+    float v[kHubLoads];
+#pragma unroll
+    for (int i = 0; i < kHubLoads; i++) {
+      const int e = i * kHubEpi + lane / kHubCols;  // this lane's edge in the wave's part
+      const int u = __shfl(ids[(i * kHubEpi) / 64], e % 64, 64);  // 0 past the row: valid
+      v[i] = C[(int64_t)u * k + my_col];
     }
 #pragma unroll
-    for (int i = 0; i < kHubEdgesW; i++)
-      if (i < cnt) dst[(gw * kHubEdgesW + i) * 64 + lane] = v[i];
+    for (int i = 0; i < kHubLoads; i++)
+      if (i * kHubEpi + lane / kHubCols < cnt)
+        dst[(gw * kHubEdgesW) * kHubCols + i * 64 + lane] = v[i];
   };
   float s = 0.0f;
   if (running) {
-    int ids = 0;
+    int ids[kHubIdRegs], nxt[kHubIdRegs];
     if (wid != 0 && nchunks > 0) {
-      ids = load_ids(0);
-      const int nxt = nchunks > 1 ? load_ids(1) : 0;
+      load_ids(0, ids);
+      if (nchunks > 1) load_ids(1, nxt);
       produce(0, ids, buf[0]);
-      ids = nxt;
+#pragma unroll
+      for (int q = 0; q < kHubIdRegs; q++) ids[q] = nxt[q];
     }
     __syncthreads();
     for (int c = 0; c < nchunks; c++) {
       if (wid == 0) {
-        const float* __restrict__ b = buf[c & 1];
-        const int nc = min(kHubChunkE, ne - c * kHubChunkE);
-        int e = 0;
-        for (; e + 8 <= nc; e += 8) {
-          float a[8];
+        if (lane < kHubCols) {
+          const float* __restrict__ b = buf[c & 1] + lane;
+          const int nc = min(kHubChunkE, ne - c * kHubChunkE);
+          int e = 0;
+          for (; e + 8 <= nc; e += 8) {
+            float a[8];
 #pragma unroll
-          for (int q = 0; q < 8; q++) a[q] = b[(e + q) * 64 + lane];
+            for (int q = 0; q < 8; q++) a[q] = b[(e + q) * kHubCols];
 #pragma unroll
-          for (int q = 0; q < 8; q++) s += a[q];
+            for (int q = 0; q < 8; q++) s += a[q];
+          }
+          for (; e < nc; e++) s += b[e * kHubCols];
         }
-        for (; e < nc; e++) s += b[e * 64 + lane];
       } else if (c + 1 < nchunks) {
-        const int nxt = c + 2 < nchunks ? load_ids(c + 2) : 0;  // in flight across the barrier
+        if (c + 2 < nchunks) load_ids(c + 2, nxt);  // in flight across the barrier
         produce(c + 1, ids, buf[(c + 1) & 1]);
-        ids = nxt;
+#pragma unroll
+        for (int q = 0; q < kHubIdRegs; q++) ids[q] = nxt[q];
       }
       __syncthreads();
     }
   }
-  if (wid == 0) {
+  if (wid == 0 && lane < kHubCols) {
     double err[1] = {0.0};
-    if (cc < ka) {
+    if (my_col < ka) {
       const float acc[1] = {s};
-      const int sd[1] = {seeds[cc]}, fz[1] = {frozen[cc]};
-      row_update<1>(row, acc, sd, fz, X, Cn, outdeg, k, cc, teleport, damp, err);
+      const int sd[1] = {seeds[my_col]}, fz[1] = {frozen[my_col]};
+      row_update<1>(row, acc, sd, fz, X, Cn, outdeg, k, my_col, teleport, damp, err);
     }
-    hub_err[(int64_t)h * k + cc] = err[0];
+    hub_err[(int64_t)h * k + my_col] = err[0];
   }
 }
 
@@ -416,6 +436,7 @@ using namespace gd;
 
 struct gd_ppr_plan {
   int m = 0, k = 0, kc = 64;
+  int hub_cols = 64;  // columns per hub-row workgroup
   int64_t nnz = 0;
   hipStream_t stream = nullptr;
   bool relabel = false;
@@ -550,10 +571,23 @@ void ppr_launch_iteration(gd_ppr_plan* p, float* X, PprState* st, float teleport
     (void)hipEventRecord(p->fork_ev, p->stream);
     (void)hipStreamWaitEvent(p->side, p->fork_ev, 0);
     ts = p->side;
-    hipLaunchKernelGGL(ppr_hub_chain_kernel, dim3(p->n_hubs, (unsigned)(kact / 64)),
-                       dim3(kPBlock), 0, p->stream, p->hub_row.as<int32_t>(), p->off, p->col,
-                       p->outdeg, p->seeds.as<int32_t>(), frozen, ka, (int64_t)p->k, X, C0, C1,
-                       p->hub_err.as<double>(), st, teleport, damp);
+    const int groups = (int)(kact / p->hub_cols);
+    const dim3 hg((unsigned)(p->n_hubs * groups));
+    if (p->hub_cols == 16)
+      hipLaunchKernelGGL(ppr_hub_chain_kernel<16>, hg, dim3(kPBlock), 0, p->stream,
+                         p->hub_row.as<int32_t>(), groups, p->off, p->col, p->outdeg,
+                         p->seeds.as<int32_t>(), frozen, ka, (int64_t)p->k, X, C0, C1,
+                         p->hub_err.as<double>(), st, teleport, damp);
+    else if (p->hub_cols == 32)
+      hipLaunchKernelGGL(ppr_hub_chain_kernel<32>, hg, dim3(kPBlock), 0, p->stream,
+                         p->hub_row.as<int32_t>(), groups, p->off, p->col, p->outdeg,
+                         p->seeds.as<int32_t>(), frozen, ka, (int64_t)p->k, X, C0, C1,
+                         p->hub_err.as<double>(), st, teleport, damp);
+    else
+      hipLaunchKernelGGL(ppr_hub_chain_kernel<64>, hg, dim3(kPBlock), 0, p->stream,
+                         p->hub_row.as<int32_t>(), groups, p->off, p->col, p->outdeg,
+                         p->seeds.as<int32_t>(), frozen, ka, (int64_t)p->k, X, C0, C1,
+                         p->hub_err.as<double>(), st, teleport, damp);
   }
   hipLaunchKernelGGL(ppr_pull_kernel<VEC>, dim3(p->grid, chunks), dim3(kPBlock), 0, ts,
                      p->tiles.as<Tile>(), p->n_tiles, p->off, p->col, p->outdeg,
@@ -641,6 +675,7 @@ void gd_ppr_default_plan_config(gd_ppr_plan_config* c) {
   c->k_chunk = 0;
   c->column_stop = -1;
   c->compact = -1;
+  c->hub_cols = 0;
 }
 
 int gd_ppr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* d_col,
@@ -660,6 +695,8 @@ int gd_ppr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* 
   GD_CHECK_ARG(c.k_chunk == 0 || c.k_chunk == 64 || c.k_chunk == 128 || c.k_chunk == 256,
                "k_chunk must be 0, 64, 128 or 256");
   GD_CHECK_ARG(c.k_chunk == 0 || k % c.k_chunk == 0, "k must be a multiple of k_chunk");
+  GD_CHECK_ARG(c.hub_cols == 0 || c.hub_cols == 16 || c.hub_cols == 32 || c.hub_cols == 64,
+               "hub_cols must be 0, 16, 32 or 64");
   if (gd_device_count() <= 0) {
     set_error("no HIP device visible");
     return GD_ERR_NO_DEVICE;
@@ -674,6 +711,7 @@ int gd_ppr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* 
   p->column_stop = c.column_stop != 0;
   // LDS of the in-place column permutation: kPWaves rows of k floats <= 64 KB
   p->compact = p->column_stop && c.compact != 0 && k <= 4096;
+  p->hub_cols = c.hub_cols ? c.hub_cols : 64;
   int rc = GD_OK;
   if (m > 0) {
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -705,6 +743,7 @@ int gd_ppr_plan_info(const gd_ppr_plan* p, gd_ppr_plan_info_t* info) {
   info->relabel = p->relabel ? 1 : 0;
   info->grid = p->grid;
   info->plan_ms = p->plan_ms;
+  info->hub_cols = p->hub_cols;
   return GD_OK;
 }
 

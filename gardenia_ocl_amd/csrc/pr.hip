@@ -6,10 +6,10 @@
 // convergence flag); see pull_iter.h / iter_engine.h.  contrib is
 // double-buffered so the in-place score update never races the gather.
 //
-// Arithmetic matches the reference bit for bit on rows of <= 64 in-edges:
-// base + kDamp*sum is rounded twice (this file is built with
-// -ffp-contract=off), the division is IEEE (correctly rounded), and the row sum
-// is accumulated sequentially in CSR order.
+// Arithmetic matches the reference bit for bit on every row: base + kDamp*sum
+// is rounded twice (this file is built with -ffp-contract=off), the division
+// is IEEE (correctly rounded), and every row sum is accumulated sequentially
+// in CSR order (row tiles: the owner thread; longer rows: exact_hub_kernel).
 
 #include <hip/hip_runtime.h>
 
@@ -118,6 +118,7 @@ void gd_pr_default_plan_config(gd_pr_plan_config* c) {
   c->nontemporal = -1;
   c->persistent = -1;
   c->sort_tiles = -1;
+  c->block = 0;
 }
 
 int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t* d_col,
@@ -138,6 +139,7 @@ int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t
   GD_CHECK_ARG(c.relabel >= -1 && c.relabel <= 1, "relabel must be -1, 0 or 1");
   GD_CHECK_ARG(c.nontemporal >= -1 && c.nontemporal <= 2, "nontemporal must be -1, 0, 1 or 2");
   GD_CHECK_ARG(c.persistent >= -1 && c.persistent <= 1, "persistent must be -1, 0 or 1");
+  GD_CHECK_ARG(c.block == 0 || c.block == 256 || c.block == 1024, "block must be 0, 256 or 1024");
   if (gd_device_count() <= 0) {
     set_error("no HIP device visible");
     return GD_ERR_NO_DEVICE;
@@ -154,11 +156,12 @@ int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t
   p->eng.nt = c.nontemporal < 0 ? 0 : c.nontemporal;
   p->eng.persistent = c.persistent == 1;
   p->eng.sort_tiles = c.sort_tiles != 0;
-  // Rows longer than a tile (256*ipt edges) are whole-row hubs summed in the
-  // reference's order and rounding: its sequential FP32 sum drifts on long
-  // rows (3e-4 relative on the 238,668-edge RMAT-24 row), and matching it
-  // keeps every score within 1e-5 of the reference; shorter rows stay
-  // tree-summed (<= 4e-6 difference on RMAT-24).
+  p->eng.block = (c.block == 1024 && !p->eng.persistent) ? 1024 : kBlock;
+  // Rows longer than 256*ipt edges are whole-row hubs summed in the
+  // reference's order and rounding (exact_hub_kernel); shorter rows by their
+  // owner thread in a row tile: every score is the reference's bit for bit
+  // (its sequential FP32 sum drifts by 3e-4 relative on the 238,668-edge
+  // RMAT-24 row, so matching its rounding matters).
   p->eng.exact_len = kBlock * p->eng.ipt;
   int rc = GD_OK;
   if (m > 0) {
@@ -200,6 +203,7 @@ int gd_pr_plan_info(const gd_pr_plan* p, gd_pr_plan_info_t* info) {
   info->pull_grid = p->eng.pull_grid;
   info->plan_ms = p->plan_ms;
   info->sort_tiles = p->eng.sort_tiles ? 1 : 0;
+  info->block = p->eng.block;
   return GD_OK;
 }
 

@@ -51,6 +51,8 @@ struct PrPolicy {
   // rows by a pipelined one-lane chain): with the exact hubs, every row of
   // the plan is the reference's, bit for bit
   static constexpr bool kSeqAll = true;
+  // row tiles may run on 1024-thread workgroups (16K-edge tiles sorted by source)
+  static constexpr bool kWideTiles = true;
   struct Args {
     float* scores;
     float* c0;

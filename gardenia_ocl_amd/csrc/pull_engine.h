@@ -1,6 +1,7 @@
 // pull_engine.h -- device building blocks of the CSR pull engine (wave64).
 //
-// A workgroup of kBlock threads owns one Tile.  Latency is paid once per tile:
+// A workgroup of NB threads (kBlock = 256 by default) owns one Tile of up to
+// NB*IPT edges and 4*NB rows.  Latency is paid once per tile:
 // every load the tile needs (edge indices, row offsets, the epilogue's
 // per-row operands) is issued unconditionally up front -- addresses of idle
 // lanes are clamped into the tile instead of branching -- and only then does
@@ -12,7 +13,7 @@
 //     sum goes back through LDS to the owning thread for the epilogue.
 // Hub segments gather kBlock*IPT edges of one long row into a partial sum.
 //
-// Row r of the tile is owned by thread (r - r0) % kBlock, so per-row reads and
+// Row r of the tile is owned by thread (r - r0) % NB, so per-row reads and
 // writes are coalesced.
 #ifndef GD_PULL_ENGINE_H_
 #define GD_PULL_ENGINE_H_
@@ -27,9 +28,9 @@ namespace gd {
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kWaves = kBlock / kWave;
-constexpr int kTileRows = 1024;
-constexpr int kRowsPerThread = kTileRows / kBlock;
-constexpr int kOffLoads = (kTileRows + 1 + kBlock - 1) / kBlock;
+constexpr int kRowsPerThread = 4;  // rows per thread of a row tile (any block size)
+constexpr int kTileRows = kRowsPerThread * kBlock;
+constexpr int kOffLoads = kRowsPerThread + 1;  // ceil((rows + 1) / threads)
 constexpr int kSeqRow = 64;
 
 // Loads / stores of arrays streamed once per launch.  With NT the accesses
@@ -145,34 +146,34 @@ __device__ __forceinline__ float seq_sum(float s, const float* __restrict__ v, i
 
 // Loads the tile's edge indices col[e0 + tid + j*kBlock] (clamped into the
 // tile so all IPT loads issue back to back).  Requires ne >= 1.
-template <int IPT, bool NT>
+template <int IPT, bool NT, int NB = kBlock>
 __device__ __forceinline__ void load_indices(const int32_t* __restrict__ col, int e0, int ne,
                                              int (&idx)[IPT]) {
   const int last = ne - 1;
 #pragma unroll
   for (int j = 0; j < IPT; j++)
-    idx[j] = ld_stream<NT>(col + e0 + min((int)threadIdx.x + j * kBlock, last));
+    idx[j] = ld_stream<NT>(col + e0 + min((int)threadIdx.x + j * NB, last));
 }
 
 // Tiles whose edges are stored sorted by source (SORTED) carry, per edge, its
 // offset in the tile: pos[] (16 bits).  The gather functor always sees the
 // edge's CSR index.
-template <int IPT, bool NT>
+template <int IPT, bool NT, int NB = kBlock>
 __device__ __forceinline__ void load_positions(const uint16_t* __restrict__ pos, int e0, int ne,
                                                int (&p)[IPT]) {
   const int last = ne - 1;
 #pragma unroll
   for (int j = 0; j < IPT; j++)
-    p[j] = (int)ld_stream<NT>(pos + e0 + min((int)threadIdx.x + j * kBlock, last));
+    p[j] = (int)ld_stream<NT>(pos + e0 + min((int)threadIdx.x + j * NB, last));
 }
 
-template <int IPT, bool SORTED, class G>
+template <int IPT, bool SORTED, class G, int NB = kBlock>
 __device__ __forceinline__ void gather_values(const G& g, int e0, int ne, const int (&idx)[IPT],
                                               const int (&p)[IPT], float (&v)[IPT]) {
   const int last = ne - 1;
 #pragma unroll
   for (int j = 0; j < IPT; j++)
-    v[j] = g.value(e0 + (SORTED ? p[j] : min((int)threadIdx.x + j * kBlock, last)), idx[j]);
+    v[j] = g.value(e0 + (SORTED ? p[j] : min((int)threadIdx.x + j * NB, last)), idx[j]);
 }
 
 // Sum of the gathered values of a hub segment [e0, e0+ne), ne >= 1: per-thread
@@ -192,18 +193,18 @@ __device__ __forceinline__ float segment_sum(const int32_t* __restrict__ col,
   return block_sum<kWaves>(s, s_tmp);
 }
 
-// LDS of one row tile.
-template <int IPT>
+// LDS of one row tile of a workgroup of NB threads.
+template <int IPT, int NB = kBlock>
 struct RowTileLds {
-  alignas(16) float val[kBlock * IPT];  // gathered per-edge values
-  int off[kTileRows + 1];     // row offsets, rebased to the tile on use
-  float longsum[kTileRows];   // wavefront sums of rows longer than kSeqRow
+  alignas(16) float val[NB * IPT];           // gathered per-edge values
+  int off[kRowsPerThread * NB + 1];          // row offsets, rebased to the tile on use
+  float longsum[kRowsPerThread * NB];        // wavefront sums of rows longer than kSeqRow
 };
 
 // Registers holding the loads of one tile that do not depend on a gather:
 // its edge indices, and for a row tile its row offsets and each owned row's
 // epilogue operands.  Issued as early as possible (see pr_pull_persistent).
-template <int IPT, class Pre>
+template <int IPT, class Pre, int NB = kBlock>
 struct TileRegs {
   int idx[IPT];
   int pos[IPT];  // sorted tiles only
@@ -217,43 +218,43 @@ struct TileRegs {
 // clamped: an empty tile re-reads its first edge, a hub segment reads the
 // offsets and operands of its own row.
 // `e_max` is the last valid index of `col` (the arrays hold >= 1 element).
-template <int IPT, bool NT, bool SORTED, class E>
+template <int IPT, bool NT, bool SORTED, class E, int NB = kBlock>
 __device__ __forceinline__ void issue_tile_loads(const Tile& t, const int32_t* __restrict__ off,
                                                  const int32_t* __restrict__ col,
                                                  const uint16_t* __restrict__ pos, int e_max,
-                                                 const E& e, TileRegs<IPT, typename E::Pre>& R) {
+                                                 const E& e, TileRegs<IPT, typename E::Pre, NB>& R) {
   const int tid = threadIdx.x;
   const int last = max(t.w - t.z - 1, 0);
 #pragma unroll
   for (int j = 0; j < IPT; j++)
-    R.idx[j] = ld_stream<NT>(col + min(t.z + min(tid + j * kBlock, last), e_max));
+    R.idx[j] = ld_stream<NT>(col + min(t.z + min(tid + j * NB, last), e_max));
   if constexpr (SORTED) {
 #pragma unroll
     for (int j = 0; j < IPT; j++)
-      R.pos[j] = (int)ld_stream<NT>(pos + min(t.z + min(tid + j * kBlock, last), e_max));
+      R.pos[j] = (int)ld_stream<NT>(pos + min(t.z + min(tid + j * NB, last), e_max));
   }
   const int nr = (t.y >= 0) ? t.y - t.x : 1;
 #pragma unroll
   for (int q = 0; q < kOffLoads; q++)
-    R.offv[q] = ld_stream<NT>(off + t.x + min(tid + q * kBlock, nr));
+    R.offv[q] = ld_stream<NT>(off + t.x + min(tid + q * NB, nr));
 #pragma unroll
-  for (int q = 0; q < kRowsPerThread; q++) R.pre[q] = e.pre(t.x + min(tid + q * kBlock, nr - 1));
+  for (int q = 0; q < kRowsPerThread; q++) R.pre[q] = e.pre(t.x + min(tid + q * NB, nr - 1));
 }
 
 // Phases C-E of a row tile whose gathered values are already in lds.val and
 // rebased offsets in lds.off (after a barrier): reduce rows, epilogue.
 // SEQ_ALL: every row, whatever its length, is summed by its owner thread in
 // CSR order (no wavefront tree): bit-identical sums for any row the tile holds.
-template <int IPT, class E, bool SEQ_ALL = false>
+template <int IPT, class E, bool SEQ_ALL = false, int NB = kBlock>
 __device__ __forceinline__ double reduce_row_tile(int r0, int nr, const E& e,
                                                   const typename E::Pre (&pre)[kRowsPerThread],
-                                                  RowTileLds<IPT>& lds) {
+                                                  RowTileLds<IPT, NB>& lds) {
   const int tid = threadIdx.x;
   float sum[kRowsPerThread];
   bool have_long = false;
 #pragma unroll
   for (int q = 0; q < kRowsPerThread; q++) {
-    const int i = tid + q * kBlock;
+    const int i = tid + q * NB;
     sum[q] = 0.0f;
     if (i < nr) {
       const int b = lds.off[i], en = lds.off[i + 1];
@@ -265,7 +266,7 @@ __device__ __forceinline__ double reduce_row_tile(int r0, int nr, const E& e,
   }
   if (!SEQ_ALL && __syncthreads_or(have_long)) {
     const int lane = tid & (kWave - 1), wid = tid / kWave;
-    for (int c = wid * kWave; c < nr; c += kBlock) {
+    for (int c = wid * kWave; c < nr; c += NB) {
       const int i = c + lane;
       const bool lr = (i < nr) && (lds.off[i + 1] - lds.off[i] > kSeqRow);
       uint64_t mask = __ballot(lr);
@@ -291,31 +292,31 @@ __device__ __forceinline__ double reduce_row_tile(int r0, int nr, const E& e,
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kRowsPerThread; q++) {
-      const int i = tid + q * kBlock;
+      const int i = tid + q * NB;
       if (i < nr && lds.off[i + 1] - lds.off[i] > kSeqRow) sum[q] = e.init(pre[q]) + lds.longsum[i];
     }
   }
   double acc = 0.0;
 #pragma unroll
   for (int q = 0; q < kRowsPerThread; q++) {
-    const int i = tid + q * kBlock;
+    const int i = tid + q * NB;
     if (i < nr) acc += e.finish(r0 + i, sum[q], pre[q]);
   }
   return acc;
 }
 
 // Stage a row tile's gathered values and rebased offsets into LDS.
-template <int IPT, bool SORTED, class Pre>
-__device__ __forceinline__ void stage_row_tile(const Tile& t, const TileRegs<IPT, Pre>& R,
-                                               const float (&v)[IPT], RowTileLds<IPT>& lds) {
+template <int IPT, bool SORTED, class Pre, int NB = kBlock>
+__device__ __forceinline__ void stage_row_tile(const Tile& t, const TileRegs<IPT, Pre, NB>& R,
+                                               const float (&v)[IPT], RowTileLds<IPT, NB>& lds) {
   const int tid = threadIdx.x;
   const int nr = t.y - t.x, ne = t.w - t.z;
 #pragma unroll
   for (int q = 0; q < kOffLoads; q++)
-    if (tid + q * kBlock <= nr) lds.off[tid + q * kBlock] = R.offv[q] - t.z;
+    if (tid + q * NB <= nr) lds.off[tid + q * NB] = R.offv[q] - t.z;
 #pragma unroll
   for (int j = 0; j < IPT; j++)
-    if (tid + j * kBlock < ne) lds.val[SORTED ? R.pos[j] : tid + j * kBlock] = v[j];
+    if (tid + j * NB < ne) lds.val[SORTED ? R.pos[j] : tid + j * NB] = v[j];
 }
 
 // One row tile: rows [r0, r1) whose edges are [e0, e0+ne).  The epilogue E
@@ -325,89 +326,64 @@ __device__ __forceinline__ void stage_row_tile(const Tile& t, const TileRegs<IPT
 //   float init(const Pre&)   value the row sum starts from
 //   double finish(int row, float sum, const Pre&)   writes the row, returns
 //                            its contribution to the tile's double accumulator
-template <int IPT, bool NT, bool SORTED, class G, class E, bool SEQ_ALL = false>
+template <int IPT, bool NT, bool SORTED, class G, class E, bool SEQ_ALL = false,
+          int NB = kBlock>
 __device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
                                            const int32_t* __restrict__ col,
                                            const uint16_t* __restrict__ pos, int e_max, int r0,
                                            int r1, int e0, int ne, const G& g, const E& e,
-                                           RowTileLds<IPT>& lds) {
+                                           RowTileLds<IPT, NB>& lds) {
   const Tile t{r0, r1, e0, e0 + ne};
-  TileRegs<IPT, typename E::Pre> R;
-  issue_tile_loads<IPT, NT, SORTED>(t, off, col, pos, e_max, e, R);
+  TileRegs<IPT, typename E::Pre, NB> R;
+  issue_tile_loads<IPT, NT, SORTED, E, NB>(t, off, col, pos, e_max, e, R);
   float v[IPT];
-  if (ne > 0) gather_values<IPT, SORTED>(g, e0, ne, R.idx, R.pos, v);
-  stage_row_tile<IPT, SORTED>(t, R, v, lds);
+  if (ne > 0) gather_values<IPT, SORTED, G, NB>(g, e0, ne, R.idx, R.pos, v);
+  stage_row_tile<IPT, SORTED, typename E::Pre, NB>(t, R, v, lds);
   __syncthreads();
-  return reduce_row_tile<IPT, E, SEQ_ALL>(r0, r1 - r0, e, R.pre, lds);
+  return reduce_row_tile<IPT, E, SEQ_ALL, NB>(r0, r1 - r0, e, R.pre, lds);
 }
 
 // ---------------------------------------------------------------------------
-// Sequential-order sums of long rows.
+// Sequential-order sums of long rows by a workgroup.
 //
-// exact_seq_sum returns s0 + v[0] + v[1] + ... + v[n-1] rounded exactly as the
-// sequential FP32 loop `s = s0; for (i) s += v[i];` rounds it (the reference's
-// per-row loop, src/pr/omp_base.cc:34-37), computed by the whole workgroup:
+// exact_row_sum returns s0 + v[0] + v[1] + ... + v[n-1] rounded exactly as
+// the sequential FP32 loop `s = s0; for (i) s += v[i];` rounds it (the
+// reference's per-row loop, src/pr/omp_base.cc:34-37), computed by the whole
+// workgroup, kScanChunk terms at a time (thread t owns terms [16t, 16t+16)):
 //   While the running sum s lies in one binade, 2^k <= |s| < 2^(k+1) with
 //   ulp u = 2^(k-23), and the exact s + v[i] stays strictly inside it, the
 //   rounded step is s + q_i*u with q_i = v[i]/u rounded to the nearest
 //   integer: s advances by an integer number of ulps.  A positive term whose
 //   fraction is exactly 1/2 (a tie) rounds s to even: that step is the map
-//   x -> (x + floor(q)) rounded up to even, and the workgroup scans the steps
-//   as maps (TieMap: additions and round-to-even steps compose into maps of
-//   the same form), so runs of equal terms that tie at every step -- many
-//   sources with the same score and degree -- do not stop the scan.  s
-//   advances over every step up to the first one that leaves the binade, is a
-//   negative tie or inf / NaN, or starts from a subnormal / non-finite s;
-//   that step is one ordinary FP32 addition, and the scan restarts after it.
-//   From s == +0 the sum jumps to the first nonzero term.  For sums of
-//   positive terms (PageRank) the restarts are the ~log2(n) binade crossings.
+//   x -> (x + floor(q)) rounded up to even.  Steps are maps of the running
+//   value that compose into maps of the same form (TieMap), so one workgroup
+//   prefix over the maps gives every running value of the chunk, ties
+//   included.  s advances over every step up to the first one that leaves
+//   the binade, is a negative tie or inf / NaN; that step is one ordinary
+//   FP32 addition and the scan restarts after it in the new binade.  From
+//   s == +0 the sum jumps to the first nonzero term; from a subnormal, -0 or
+//   non-finite s it takes one ordinary addition.  For sums of positive terms
+//   (PageRank) the restarts are the ~log2(n) binade crossings, most of them
+//   in the first terms of a row, which one lane adds serially (kExactHead).
+//   The next chunk's terms are gathered into registers while a chunk is
+//   scanned, so the gathers' latency overlaps the scan.
+// A host emulation of these steps (integer and float forms of q) matched the
+// sequential loop bit for bit on rows built to hit ties, binade crossings,
+// zeros, signs, subnormals and wide ranges.
 // ---------------------------------------------------------------------------
 
-constexpr int kExactHead = 256;  // terms of a row added serially before the scan
-
-struct ExactLds {
-  int32_t wsum[kWaves];
-  int32_t wtie[kWaves];
-  int32_t wb[kWaves];
-  int wmin[kWaves];
-  float s;
-  int tie;  // the step that stopped the scan was a positive tie
-};
-
-// Exclusive prefix over the workgroup (lanes, then waves) and the total, in
-// wrapping 32-bit arithmetic.  Only prefixes of steps that all stay inside the
-// binade are used, and those are exact (each below 2^25 in magnitude).
-__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t& total, ExactLds& x) {
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  uint32_t inc = (uint32_t)v;
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    const uint32_t t = (uint32_t)__shfl_up((int)inc, o, kWave);
-    if (lane >= o) inc += t;
-  }
-  if (lane == kWave - 1) x.wsum[wid] = (int32_t)inc;
-  __syncthreads();
-  uint32_t before = 0, tot = 0;
-#pragma unroll
-  for (int w = 0; w < kWaves; w++) {
-    const uint32_t ws = (uint32_t)x.wsum[w];
-    before += (w < wid) ? ws : 0u;
-    tot += ws;
-  }
-  __syncthreads();
-  total = (int32_t)tot;
-  return (int32_t)(before + (inc - (uint32_t)v));
-}
+constexpr int kScanPT = 16;                   // terms per thread per chunk
+constexpr int kScanChunk = kBlock * kScanPT;  // 4096
+constexpr int kExactHead = 256;               // terms of a row added serially first
+constexpr int kNoBad = 0x7fffffff;
 
 // A run of steps in integer ulps as one map of the running value x:
 //   t == 0:  x -> x + a
 //   t == 1:  x -> y + (y & 1) + b,  y = x + a
-// A regular step q is (0, q, 0).  A positive term that is exactly half an ulp
-// above an integer f rounds to even: x -> (x + f) rounded up to even, the map
-// (1, f, 0).  After such a step the value is even, so what follows it only
-// depends on constants: maps compose into maps of the same form, and a scan
-// over them gives every prefix exactly (wrapping 32-bit arithmetic keeps the
-// parity).  Negative ties stay irregular.
+// A regular step q is (0, q, 0); a positive tie above floor f is (1, f, 0).
+// After a round-to-even step the value is even, so what follows it only
+// depends on constants: maps compose into maps of the same form (wrapping
+// 32-bit arithmetic keeps the parity).
 struct TieMap {
   uint32_t t, a, b;
 };
@@ -424,248 +400,243 @@ __device__ __forceinline__ uint32_t tie_apply(const TieMap& m, uint32_t x) {
   return m.t ? y + (y & 1u) + m.b : y;
 }
 
-// Exclusive prefix (lanes, then waves, in order) of the maps and their total.
-__device__ __forceinline__ TieMap block_excl_scan_map(TieMap v, TieMap& total, ExactLds& x) {
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  TieMap inc = v;
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    TieMap left;
-    left.t = (uint32_t)__shfl_up((int)inc.t, o, kWave);
-    left.a = (uint32_t)__shfl_up((int)inc.a, o, kWave);
-    left.b = (uint32_t)__shfl_up((int)inc.b, o, kWave);
-    if (lane >= o) inc = tie_compose(left, inc);
-  }
-  // exclusive within the wave: the inclusive value of the previous lane
-  TieMap ex;
-  ex.t = (uint32_t)__shfl_up((int)inc.t, 1, kWave);
-  ex.a = (uint32_t)__shfl_up((int)inc.a, 1, kWave);
-  ex.b = (uint32_t)__shfl_up((int)inc.b, 1, kWave);
-  if (lane == 0) ex = TieMap{0u, 0u, 0u};
-  if (lane == kWave - 1) {
-    x.wtie[wid] = (int32_t)inc.t;
-    x.wsum[wid] = (int32_t)inc.a;
-    x.wb[wid] = (int32_t)inc.b;
-  }
-  __syncthreads();
-  TieMap before{0u, 0u, 0u}, tot{0u, 0u, 0u};
-#pragma unroll
-  for (int w = 0; w < kWaves; w++) {
-    const TieMap wm{(uint32_t)x.wtie[w], (uint32_t)x.wsum[w], (uint32_t)x.wb[w]};
-    if (w < wid) before = tie_compose(before, wm);
-    tot = tie_compose(tot, wm);
-  }
-  __syncthreads();
-  total = tot;
-  return tie_compose(before, ex);
+// The step of term v in ulps of the binade with exponent ex: T = v / u
+// exactly (a power-of-two scaling; inf when v is far above the binade),
+// rounded to the nearest integer, |q| clamped to 2^26 (such a step leaves the
+// binade anyway).  irregular: a negative tie, or T not finite.
+__device__ __forceinline__ TieMap step_map(float v, int ex, bool& irregular) {
+  const float T = ldexpf(v, 23 - ex);
+  const float fl = floorf(T);
+  const float fr = T - fl;  // exact
+  const float qf = fminf(fmaxf(fr > 0.5f ? fl + 1.0f : fl, -0x1p26f), 0x1p26f);
+  const bool half = fr == 0.5f;
+  irregular = (half && v < 0.0f) || !isfinite(T);
+  return TieMap{(half && v > 0.0f) ? 1u : 0u, (uint32_t)(int32_t)qf, 0u};
 }
 
-__device__ __forceinline__ int block_min(int v, ExactLds& x) {
+// Inclusive prefix of the maps over the 64 lanes of a wavefront: DPP row
+// shifts inside each row of 16 lanes, then row broadcasts across rows (lanes
+// whose source is outside the row read the identity map (0, 0, 0)).
+#define GD_MAP_SCAN_STEP(CTRL, ROWS)                                                          \
+  {                                                                                           \
+    TieMap l;                                                                                 \
+    l.t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)inc.t, CTRL, ROWS, 0xf, false);       \
+    l.a = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)inc.a, CTRL, ROWS, 0xf, false);       \
+    l.b = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)inc.b, CTRL, ROWS, 0xf, false);       \
+    inc = tie_compose(l, inc);                                                                \
+  }
+
+__device__ __forceinline__ TieMap wave_incl_scan_map(TieMap inc) {
+  GD_MAP_SCAN_STEP(0x111, 0xf)  // row_shr:1
+  GD_MAP_SCAN_STEP(0x112, 0xf)  // row_shr:2
+  GD_MAP_SCAN_STEP(0x114, 0xf)  // row_shr:4
+  GD_MAP_SCAN_STEP(0x118, 0xf)  // row_shr:8
+  GD_MAP_SCAN_STEP(0x142, 0xa)  // row_bcast:15 -> rows 1, 3
+  GD_MAP_SCAN_STEP(0x143, 0xc)  // row_bcast:31 -> rows 2, 3
+  return inc;
+}
+#undef GD_MAP_SCAN_STEP
+
+struct ExactLds {
+  uint32_t wt[kWaves], wa[kWaves], wb[kWaves];  // wave totals of the maps
+  int wbad[kWaves];                             // wave's first irregular term
+  float s;
+};
+
+// First value over the workgroup of a per-thread int that increases with the
+// thread index where present (kNoBad elsewhere): ballot per wave, then waves
+// in order.  Uniform result.
+__device__ __forceinline__ int block_first(int v, ExactLds& x) {
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
-  if (lane == 0) x.wmin[wid] = v;
+  const uint64_t m = __ballot(v != kNoBad);
+  if (lane == 0) x.wbad[wid] = m ? __builtin_amdgcn_readlane(v, __builtin_ctzll(m)) : kNoBad;
   __syncthreads();
-  int r = x.wmin[0];
+  int r = kNoBad;
 #pragma unroll
-  for (int w = 1; w < kWaves; w++) r = min(r, x.wmin[w]);
-  __syncthreads();
+  for (int w = kWaves - 1; w >= 0; w--) r = (x.wbad[w] != kNoBad) ? x.wbad[w] : r;
   return r;
 }
 
-// v in units of u = 2^(ex-23), rounded to the nearest integer, from the bits of
-// v (|q| clamped to 2^26: such a step leaves the binade anyway).  tie: v is
-// positive and its fraction is exactly 1/2 (the result holds the floor; the
-// step rounds to even, map (1, q, 0)).  irregular: a negative tie, or v is inf
-// or NaN.
-__device__ __forceinline__ int32_t ulps_of_tie(float v, int ex, bool& tie, bool& irregular) {
-  const uint32_t b = __float_as_uint(v);
-  const int eb = (int)((b >> 23) & 0xffu);
-  const uint32_t mv = eb ? ((b & 0x7fffffu) | 0x800000u) : (b & 0x7fffffu);  // v = mv*2^(ev-23)
-  const int d = ex - (eb ? eb - 127 : -126);                                  // q = mv / 2^d
-  int32_t q;
-  bool half = false;
-  if (d <= 0) {
-    q = (d <= -3) ? (1 << 26) : (int32_t)min((uint32_t)1 << 26, mv << (-d));
-  } else if (d >= 26) {
-    q = 0;
-  } else {
-    const uint32_t hf = 1u << (d - 1), rem = mv & ((1u << d) - 1u);
-    half = (rem == hf);
-    q = (int32_t)((mv >> d) + (rem > hf ? 1u : 0u));
-  }
-  const bool neg = (b >> 31) != 0;
-  tie = half && !neg;
-  irregular = (half && neg) || eb == 0xff;
-  return neg ? -q : q;
-}
-
-// Position of chunk element i in the padded LDS buffer: thread t reads its IPT
-// consecutive elements at stride IPT+1 words (no bank conflicts).
-template <int IPT>
-__device__ __forceinline__ int exact_pos(int i) {
-  return i + i / IPT;
-}
-
-// One chunk of a sequential-order sum: nc values already in LDS (val, padded
-// layout exact_pos), added to s in index order with sequential FP32 rounding.
-// `head`: add the first kExactHead values serially first (the start of a row,
-// where s crosses a binade at almost every doubling).  All threads return the
-// same s.  Ends with the workgroup synchronised (val may be rewritten).
-// tie_mode (uniform, kept across the chunks of a row): once a positive tie has
-// stopped the scan, the row is scanned as maps (TieMap), which go through
-// ties; before that, the plain integer scan (cheaper) treats a tie as a stop.
-template <int IPT>
-__device__ float exact_scan_chunk(const float* __restrict__ val, int nc, bool head, float s,
-                                  ExactLds& x, bool& tie_mode) {
+// The scan of one chunk: nc terms in LDS (`val`, thread t's terms at 16t),
+// added to s from term `start` on.  All threads return the same s.  Ends with
+// the workgroup synchronised.
+__device__ __forceinline__ float exact_scan_chunk(const float* __restrict__ val, int nc,
+                                                  int start, float s, ExactLds& x) {
   const int tid = threadIdx.x;
-  const int lo = tid * IPT, hi = min(lo + IPT, nc);  // this thread's elements
-  int start = 0;
-  if (head) {
-    const int ns = min(nc, kExactHead);
-    if (tid == 0) {
-      // software-pipelined: the next batch's LDS loads are in flight while the
-      // current batch is added (the adds are one dependent chain)
-      constexpr int B = 16;
-      float a = s;
-      int i = 0;
-      if (ns >= B) {
-        float t[B];
+  const int lane = tid & (kWave - 1), wid = tid / kWave;
+  const int lo = tid * kScanPT;
+  float v[kScanPT];
+  {
+    const float4* v4 = reinterpret_cast<const float4*>(val + lo);
 #pragma unroll
-        for (int k = 0; k < B; k++) t[k] = val[exact_pos<IPT>(k)];
-        for (i = B; i + B <= ns; i += B) {
-          float u[B];
-#pragma unroll
-          for (int k = 0; k < B; k++) u[k] = val[exact_pos<IPT>(i + k)];
-#pragma unroll
-          for (int k = 0; k < B; k++) a += t[k];
-#pragma unroll
-          for (int k = 0; k < B; k++) t[k] = u[k];
-        }
-#pragma unroll
-        for (int k = 0; k < B; k++) a += t[k];
-      }
-      for (; i < ns; i++) a += val[exact_pos<IPT>(i)];
-      x.s = a;
+    for (int k = 0; k < kScanPT / 4; k++) {
+      const float4 t = v4[k];
+      v[4 * k] = t.x;
+      v[4 * k + 1] = t.y;
+      v[4 * k + 2] = t.z;
+      v[4 * k + 3] = t.w;
     }
-    __syncthreads();
-    s = x.s;
-    __syncthreads();
-    start = ns;
   }
   while (start < nc) {  // s and start are uniform across the workgroup
-    if (!(fabsf(s) >= 0x1p-125f) || !isfinite(s)) {  // zero, subnormal, inf/NaN
-      if (s == 0.0f && !signbit(s)) {
-        // +0 + (+-0) == +0: jump to the first nonzero term (x != 0: +0 + x == x)
-        int nz = 0x7fffffff;
-        for (int i = max(lo, start); i < hi; i++)
-          if (val[exact_pos<IPT>(i)] != 0.0f) {
-            nz = i;
-            break;
-          }
-        const int jz = block_min(nz, x);
-        if (jz == 0x7fffffff) {
+    const uint32_t sb = __float_as_uint(s);
+    const int eb = (int)((sb >> 23) & 0xffu);
+    if (eb <= 1 || eb == 0xff) {  // zero, subnormal, [2^-126, 2^-125), inf / NaN
+      if (sb == 0u) {
+        // +0 + (+-0) == +0: jump to the first nonzero term (+0 + v == v)
+        int nz = kNoBad;
+#pragma unroll
+        for (int k = kScanPT - 1; k >= 0; k--)
+          if (lo + k >= start && lo + k < nc && v[k] != 0.0f) nz = lo + k;
+        const int jz = block_first(nz, x);
+        if (jz == kNoBad) {
           start = nc;
         } else {
-          s = val[exact_pos<IPT>(jz)];
+          s = val[jz];
           start = jz + 1;
         }
+        __syncthreads();  // wbad is reused
         continue;
       }
-      s = s + val[exact_pos<IPT>(start)];
+      s = s + val[start];
       start++;
       continue;
     }
-    const int ex = ilogbf(s);
-    // s = P * 2^(ex-23) with 2^23 <= |P| < 2^24
-    const int32_t P = (int32_t)(__float_as_uint(s) & 0x7fffffu) | 0x800000;
-    const int32_t Ps = signbit(s) ? -P : P;
-    int32_t run, end;  // the value before this thread's first step; after the last step
-    if (tie_mode) {
-      TieMap L{0u, 0u, 0u};
-      for (int i = max(lo, start); i < hi; i++) {
-        bool tie, irr;
-        const uint32_t q = (uint32_t)ulps_of_tie(val[exact_pos<IPT>(i)], ex, tie, irr);
-        L = tie_compose(L, TieMap{tie ? 1u : 0u, q, 0u});
-      }
-      TieMap total;
-      run = (int32_t)tie_apply(block_excl_scan_map(L, total, x), (uint32_t)Ps);
-      end = (int32_t)tie_apply(total, (uint32_t)Ps);
-    } else {
-      uint32_t L = 0;
-      for (int i = max(lo, start); i < hi; i++) {
-        bool tie, irr;
-        L += (uint32_t)ulps_of_tie(val[exact_pos<IPT>(i)], ex, tie, irr);
-      }
-      int32_t total;
-      run = (int32_t)((uint32_t)Ps + (uint32_t)block_excl_scan((int32_t)L, total, x));
-      end = (int32_t)((uint32_t)Ps + (uint32_t)total);
+    const int ex = eb - 127;
+    const int32_t P = (int32_t)(sb & 0x7fffffu) | 0x800000;
+    const uint32_t Ps = (uint32_t)((sb >> 31) ? -P : P);
+    // this thread's steps (identity outside [start, nc)) and their composition
+    uint32_t q[kScanPT];
+    uint32_t ties = 0u, irrs = 0u;
+    TieMap L{0u, 0u, 0u};
+#pragma unroll
+    for (int k = 0; k < kScanPT; k++) {
+      const bool act = lo + k >= start && lo + k < nc;
+      bool irr;
+      const TieMap mk = step_map(v[k], ex, irr);
+      q[k] = act ? mk.a : 0u;
+      ties |= (act && mk.t) ? (1u << k) : 0u;
+      irrs |= (act && irr) ? (1u << k) : 0u;
+      L = tie_compose(L, TieMap{(ties >> k) & 1u, q[k], 0u});
     }
-    int bad = 0x7fffffff;
-    bool bad_tie = false;
-    for (int i = max(lo, start); i < hi; i++) {
-      bool tie, irr;
-      const uint32_t q = (uint32_t)ulps_of_tie(val[exact_pos<IPT>(i)], ex, tie, irr);
-      const uint32_t y = (uint32_t)run + q;
-      const int32_t r = (int32_t)(tie ? y + (y & 1u) : y);
-      if (irr || (tie && !tie_mode) || !(abs(r) > (1 << 23) && abs(r) < (1 << 24))) {
-        bad = i;
-        bad_tie = tie && !tie_mode;
-        break;
-      }
-      run = r;
+    // workgroup prefix of the maps: waves in order, lanes in order
+    const TieMap inc = wave_incl_scan_map(L);
+    if (lane == kWave - 1) {
+      x.wt[wid] = inc.t;
+      x.wa[wid] = inc.a;
+      x.wb[wid] = inc.b;
     }
-    const int jb = block_min(bad, x);
-    if (jb == 0x7fffffff) {
+    TieMap ex_lane;
+    ex_lane.t = (uint32_t)__shfl_up((int)inc.t, 1, kWave);
+    ex_lane.a = (uint32_t)__shfl_up((int)inc.a, 1, kWave);
+    ex_lane.b = (uint32_t)__shfl_up((int)inc.b, 1, kWave);
+    if (lane == 0) ex_lane = TieMap{0u, 0u, 0u};
+    __syncthreads();
+    TieMap before{0u, 0u, 0u}, total{0u, 0u, 0u};
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+      const TieMap wm{x.wt[w], x.wa[w], x.wb[w]};
+      if (w < wid) before = tie_compose(before, wm);
+      total = tie_compose(total, wm);
+    }
+    // running values of this thread's terms, in order, up to the first one
+    // whose step is irregular or leaves the binade
+    uint32_t run = tie_apply(tie_compose(before, ex_lane), Ps);
+    int bad = kNoBad;
+    uint32_t run_bad = 0u;
+#pragma unroll
+    for (int k = 0; k < kScanPT; k++) {
+      if (bad == kNoBad && lo + k >= start && lo + k < nc) {
+        const uint32_t y = run + q[k];
+        const int32_t r = (int32_t)(((ties >> k) & 1u) ? y + (y & 1u) : y);
+        if (((irrs >> k) & 1u) || !(abs(r) > (1 << 23) && abs(r) < (1 << 24))) {
+          bad = lo + k;
+          run_bad = run;
+        } else {
+          run = (uint32_t)r;
+        }
+      }
+    }
+    const int jb = block_first(bad, x);
+    if (jb == kNoBad) {
       // exact: an integer below 2^24 in magnitude times a power of two
-      s = ldexpf((float)end, ex - 23);
+      s = ldexpf((float)(int32_t)tie_apply(total, Ps), ex - 23);
       start = nc;
+      __syncthreads();  // wt/wa/wb and wbad are reused
     } else {
-      if (bad == jb) {
-        x.s = ldexpf((float)run, ex - 23) + val[exact_pos<IPT>(jb)];  // FP32 step
-        x.tie = bad_tie ? 1 : 0;
-      }
+      if (bad == jb) x.s = ldexpf((float)(int32_t)run_bad, ex - 23) + val[jb];  // FP32 step
       __syncthreads();
       s = x.s;
-      tie_mode = tie_mode || x.tie != 0;
-      __syncthreads();
       start = jb + 1;
+      __syncthreads();
     }
   }
-  __syncthreads();
   return s;
 }
 
-// Sum of the gathered values of edges [e0, e0+ne) in edge order, rounded as the
-// sequential FP32 loop starting from s.  All threads return the same value.
-// `val` holds kBlock*IPT + kBlock floats (the padded layout exact_pos).
-template <int IPT, bool NT, bool SORTED, class G>
-__device__ __forceinline__ float exact_seq_sum(const int32_t* __restrict__ col, const uint16_t* __restrict__ pos,
-                               int e_max, int e0, int ne, const G& g, float s,
-                               float* __restrict__ val, ExactLds& x) {
-  constexpr int C = kBlock * IPT;
+// s0 + the gathered values of edges [e0, e0+ne) in edge order, rounded as the
+// sequential FP32 loop starting from s0 (all threads return it).  `buf0`,
+// `buf1`: two 16-byte aligned LDS buffers of kScanChunk floats.  Chunk c is
+// gathered as thread t's terms t + 256 j (coalesced index loads), stored at
+// their position in the chunk, and scanned as thread t's terms 16t..16t+15;
+// chunk c+1's gathers and chunk c+2's index loads are in flight meanwhile.
+template <bool NT, class G>
+__device__ float exact_row_sum(const int32_t* __restrict__ col, int e_max, int e0, int ne,
+                               const G& g, float s0, float* __restrict__ buf0,
+                               float* __restrict__ buf1, ExactLds& x) {
   const int tid = threadIdx.x;
-  bool tie_mode = false;
-  for (int c0 = 0; c0 < ne; c0 += C) {
-    const int nc = min(C, ne - c0);
-    int idx[IPT], p[IPT];
-    float v[IPT];
+  const int nchunks = (ne + kScanChunk - 1) / kScanChunk;
+  int idx[kScanPT];
+  float vv[kScanPT];
+  auto elem = [&](int c, int j) {  // edge offset in the row, clamped into it
+    return min(c * kScanChunk + tid + j * kBlock, ne - 1);
+  };
+  auto load_idx = [&](int c) {
 #pragma unroll
-    for (int j = 0; j < IPT; j++)
-      idx[j] = ld_stream<NT>(col + min(e0 + c0 + min(tid + j * kBlock, nc - 1), e_max));
-    if constexpr (SORTED) {  // chunks are sorted separately: offsets are chunk-relative
+    for (int j = 0; j < kScanPT; j++) idx[j] = ld_stream<NT>(col + min(e0 + elem(c, j), e_max));
+  };
+  auto gather = [&](int c) {
 #pragma unroll
-      for (int j = 0; j < IPT; j++)
-        p[j] = (int)ld_stream<NT>(pos + min(e0 + c0 + min(tid + j * kBlock, nc - 1), e_max));
+    for (int j = 0; j < kScanPT; j++) vv[j] = g.value(e0 + elem(c, j), idx[j]);
+  };
+  float s = s0;
+  if (nchunks == 0) return s;
+  load_idx(0);
+  gather(0);
+  if (nchunks > 1) load_idx(1);
+  for (int c = 0; c < nchunks; c++) {
+    float* cur = (c & 1) ? buf1 : buf0;
+    const int nc = min(kScanChunk, ne - c * kScanChunk);
+#pragma unroll
+    for (int j = 0; j < kScanPT; j++)
+      if (tid + j * kBlock < nc) cur[tid + j * kBlock] = vv[j];
+    if (c + 1 < nchunks) {
+      gather(c + 1);                         // in flight during the scan of chunk c
+      if (c + 2 < nchunks) load_idx(c + 2);  // likewise
     }
-#pragma unroll
-    for (int j = 0; j < IPT; j++)
-      v[j] = g.value(e0 + c0 + (SORTED ? p[j] : min(tid + j * kBlock, nc - 1)), idx[j]);
-#pragma unroll
-    for (int j = 0; j < IPT; j++)
-      if (tid + j * kBlock < nc) val[exact_pos<IPT>(SORTED ? p[j] : tid + j * kBlock)] = v[j];
     __syncthreads();
-    s = exact_scan_chunk<IPT>(val, nc, c0 == 0, s, x, tie_mode);
+    int start = 0;
+    if (c == 0) {
+      // the first terms, where s doubles every few steps: one lane, in order
+      const int nh = min(nc, kExactHead);
+      if (tid == 0) {
+        float a = s;
+        int i = 0;
+        for (; i + 16 <= nh; i += 16) {
+          float t[16];
+#pragma unroll
+          for (int k = 0; k < 16; k++) t[k] = cur[i + k];
+#pragma unroll
+          for (int k = 0; k < 16; k++) a += t[k];
+        }
+        for (; i < nh; i++) a += cur[i];
+        x.s = a;
+      }
+      __syncthreads();
+      s = x.s;
+      start = nh;
+      __syncthreads();
+    }
+    s = exact_scan_chunk(cur, nc, start, s, x);
   }
   return s;
 }

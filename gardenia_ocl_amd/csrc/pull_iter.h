@@ -118,38 +118,43 @@ __device__ __forceinline__ void fused_finish(const IterArgs& it, double* s_err, 
   }
 }
 
-// One workgroup per tile.  SORTED: each tile's edges are stored in source
-// order with their tile offsets (tile_sort.h); sums are still in CSR order.
+// One workgroup of NB threads per tile (NB*IPT edges, 4*NB rows).  SORTED:
+// each tile's edges are stored in source order with their tile offsets
+// (tile_sort.h); sums are still in CSR order.  Larger workgroups sort larger
+// tiles, so more lanes of a gather instruction share a cache line.
 // tile_body: row tile b of the iteration (the caller checked st->done).
-template <class P, int IPT, int NT, bool SORTED>
+template <class P, int IPT, int NT, bool SORTED, int NB = kBlock>
 __device__ __forceinline__ void tile_body(const typename P::Args& a, const IterArgs& it, int b) {
+  static_assert(NB == kBlock || P::kExactHubs, "hub segments need kBlock-thread tiles");
   const int odd = it.st->iter & 1;
-  __shared__ RowTileLds<IPT> lds;
-  __shared__ double s_err[kWaves];
+  __shared__ RowTileLds<IPT, NB> lds;
+  __shared__ double s_err[NB / kWave];
   const Tile t = it.tiles[b];
   const auto g = P::template gather<NT>(a, odd);
   double err = 0.0;
   if (t.y >= 0) {
     const auto e = P::template epilogue<NT>(a, odd);
-    err = row_tile<IPT, (NT > 0), SORTED, decltype(g), decltype(e), P::kSeqAll>(
+    err = row_tile<IPT, (NT > 0), SORTED, decltype(g), decltype(e), P::kSeqAll, NB>(
         it.off, it.col, it.pos, it.e_max, t.x, t.y, t.z, t.w - t.z, g, e, lds);
   } else if constexpr (!P::kExactHubs) {
     const float p = segment_sum<IPT, (NT > 0), SORTED>(it.col, it.pos, t.z, t.w - t.z, g, lds.val);
     if (threadIdx.x == 0) it.hub_part[-t.y - 1] = p;
   }
-  const double tot = block_sum<kWaves>(err, s_err);
+  const double tot = block_sum<NB / kWave>(err, s_err);
   if (threadIdx.x == 0) it.err_part[b] = tot;
-  if (it.fuse_parts > 0) {
-    __shared__ int s_last;
-    fused_finish(it, s_err, &s_last);
+  if constexpr (NB == kBlock) {
+    if (it.fuse_parts > 0) {
+      __shared__ int s_last;
+      fused_finish(it, s_err, &s_last);
+    }
   }
 }
 
-template <class P, int IPT, int NT, bool SORTED>
-__global__ __launch_bounds__(kBlock) void pull_tile_kernel(const typename P::Args a,
-                                                           const IterArgs it) {
+template <class P, int IPT, int NT, bool SORTED, int NB = kBlock>
+__global__ __launch_bounds__(NB) void pull_tile_kernel(const typename P::Args a,
+                                                       const IterArgs it) {
   if (it.st->done) return;
-  tile_body<P, IPT, NT, SORTED>(a, it, blockIdx.x);
+  tile_body<P, IPT, NT, SORTED, NB>(a, it, blockIdx.x);
 }
 
 // Persistent, software-pipelined form: workgroup b walks tiles b, b+G, ...;
@@ -208,24 +213,16 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
 }
 
 // Whole-row hubs of an exact-hub policy: one workgroup per hub row adds the
-// row in edge order with the reference's sequential rounding, then runs the
-// row's epilogue.  Rows of up to kScanRow edges: one lane's dependent chain
-// (hub_chain_sum, fed by the other three waves), the shortest path to the
-// bits for a row alone on the GPU; longer rows: the workgroup's binade scan
-// (exact_seq_sum), whose cost per term holds up beside the row tiles where a
-// chain's doubles (their waves take the CU's issue slots and LDS).  A kernel
-// of its own, launched before and beside the row-tile launch; its registers
-// are kept near a row tile's so that a hub workgroup fits on a CU beside four
-// of them.  With sorted tiles the chained rows are stored source-sorted per
-// kHubChunk edges (fewer L2 requests per gather), scanned rows in edge order.
-// KQ: float4 LDS reads ahead in a hub chain (IterEngine::deep_hub_chain).
-// hub_body: exact hub h of the iteration (the caller checked st->done).
-template <class P, int IPT, int NT, bool SORTED, int KQ>
+// row in edge order with the reference's sequential rounding (exact_row_sum:
+// the workgroup's binade scan, the next chunk's gathers in flight during the
+// scan), then runs the row's epilogue.  A kernel of its own, launched before
+// and beside the row-tile launch, longest row first.  Hub rows are stored in
+// edge order.  hub_body: exact hub h of the iteration (the caller checked
+// st->done).
+template <class P, int IPT, int NT, bool SORTED>
 __device__ __forceinline__ void hub_body(const typename P::Args& a, const IterArgs& it, int h) {
   const int odd = it.st->iter & 1;
-  constexpr int kBuf = (2 * kHubChunk > kBlock * IPT + kBlock ? 2 * kHubChunk
-                                                              : kBlock * IPT + kBlock) + 4;
-  __shared__ __align__(16) float buf[kBuf];
+  __shared__ __align__(16) float buf[2 * kScanChunk];
   __shared__ double s_err[kWaves];
   __shared__ ExactLds xs;
   const Tile t = it.hub_tiles[h];
@@ -233,13 +230,8 @@ __device__ __forceinline__ void hub_body(const typename P::Args& a, const IterAr
   const auto e = P::template epilogue<NT>(a, odd);
   const auto pre = e.pre(t.x);
   const int ne = t.w - t.z;
-  float sum;
-  if (ne > kScanRow)
-    sum = exact_seq_sum<IPT, (NT > 0), false>(it.col, nullptr, it.e_max, t.z, ne, g, e.init(pre),
-                                              buf, xs);
-  else
-    sum = hub_chain_sum<KQ, SORTED>(it.col, it.pos, it.e_max, t.z, ne, g, e.init(pre), buf,
-                                    buf + kHubChunk, buf + 2 * kHubChunk);
+  const float sum = exact_row_sum<(NT > 0)>(it.col, it.e_max, t.z, ne, g, e.init(pre), buf,
+                                            buf + kScanChunk, xs);
   double err = 0.0;
   if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
   const double tot = block_sum<kWaves>(err, s_err);
@@ -250,25 +242,24 @@ __device__ __forceinline__ void hub_body(const typename P::Args& a, const IterAr
   }
 }
 
-template <class P, int IPT, int NT, bool SORTED, int KQ>
+template <class P, int IPT, int NT, bool SORTED>
 __global__ __launch_bounds__(kBlock) void exact_hub_kernel(const typename P::Args a,
                                                            const IterArgs it) {
   if (it.st->done) return;
-  hub_body<P, IPT, NT, SORTED, KQ>(a, it, blockIdx.x);
+  hub_body<P, IPT, NT, SORTED>(a, it, blockIdx.x);
 }
 
 // Small iterations (it.fuse_parts > 0) in one launch: workgroups [0,
 // n_hub_tiles) take the exact hubs (longest first), the rest the row tiles,
 // and the last workgroup to finish closes the iteration (fused_finish) --
-// one kernel per iteration, no fork, join or finalize.  Registers are the hub
-// body's; with under 2*kBlock workgroups occupancy does not bound them.
+// one kernel per iteration, no fork, join or finalize.
 template <class P, int IPT, int NT, bool SORTED>
 __global__ __launch_bounds__(kBlock) void pull_small_kernel(const typename P::Args a,
                                                             const IterArgs it) {
   if (it.st->done) return;
   if constexpr (P::kExactHubs) {
     if ((int)blockIdx.x < it.n_hub_tiles) {
-      hub_body<P, IPT, NT, SORTED, 16>(a, it, blockIdx.x);
+      hub_body<P, IPT, NT, SORTED>(a, it, blockIdx.x);
       return;
     }
   }

@@ -17,7 +17,7 @@
 
 namespace gd {
 
-constexpr int kSortMax = 4096;  // longest sorted segment (one LDS bitonic sort)
+constexpr int kSortMax = 16384;  // longest sorted segment (one LDS bitonic sort)
 
 // Sorts the edges of every work segment of `s` by source: row tiles, and hub
 // tiles in chunks of `chunk` edges (whole-row exact hubs) or as they are
@@ -25,9 +25,8 @@ constexpr int kSortMax = 4096;  // longest sorted segment (one LDS bitonic sort)
 // segment in source order (ties kept in edge order), its source and its
 // offset in the segment.  col and col_sorted may be the same array.
 // exact_hubs (a schedule built with exact_len > 0, whose hub tiles are whole
-// rows): hub rows of up to kScanRow edges are sorted in chunks of kHubChunk
-// edges (the chunks their kernel adds), longer ones stay in edge order
-// (col_sorted = col and pos = offset there).
+// rows): hub rows stay in edge order (col_sorted = col and pos = offset
+// there).
 int build_tile_sort(const Schedule& s, int chunk, const int32_t* d_col, int32_t* d_col_sorted,
                     uint16_t* d_pos, hipStream_t stream, bool exact_hubs = false);
 

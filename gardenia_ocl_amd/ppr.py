@@ -23,7 +23,7 @@ class PersonalizedPageRankPlan:
     """
 
     def __init__(self, in_row_offsets, in_column_indices, out_degree, k, stream=None, *,
-                 relabel=None, k_chunk=0, column_stop=True, compact=True):
+                 relabel=None, k_chunk=0, column_stop=True, compact=True, hub_cols=0):
         """column_stop: each column stops at its own first iteration with L1
         change < tol (as k single-seed runs would); False: all columns run
         until the largest change is < tol.  compact: move stopped columns
@@ -45,6 +45,7 @@ class PersonalizedPageRankPlan:
         cfg.k_chunk = int(k_chunk)
         cfg.column_stop = 1 if column_stop else 0
         cfg.compact = 1 if compact else 0
+        cfg.hub_cols = int(hub_cols)
         h = ctypes.c_void_p()
         N.call("gd_ppr_plan_create", self.m, self.nnz, in_row_offsets.data_ptr(),
                in_column_indices.data_ptr(), out_degree.data_ptr(), self.k,

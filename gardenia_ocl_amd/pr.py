@@ -133,10 +133,11 @@ class PageRankPlan:
 
     def __init__(self, in_row_offsets, in_column_indices, out_degree, stream=None, *,
                  relabel=None, items_per_thread=0, nontemporal=None, persistent=None,
-                 sort_tiles=None):
+                 sort_tiles=None, block=0):
         """relabel / nontemporal / persistent / sort_tiles: None = library default,
         True / False force it (sort_tiles: gather each tile's edges in source
-        order; same results)."""
+        order; same results).  block: threads per row-tile workgroup (256 or
+        1024; 0 = library default); same results."""
         import torch
 
         for t in (in_row_offsets, in_column_indices, out_degree):
@@ -159,6 +160,7 @@ class PageRankPlan:
             cfg.persistent = 1 if persistent else 0
         if sort_tiles is not None:
             cfg.sort_tiles = 1 if sort_tiles else 0
+        cfg.block = int(block)
         h = ctypes.c_void_p()
         N.call("gd_pr_plan_create_ex", self.m, self.nnz, in_row_offsets.data_ptr(),
                in_column_indices.data_ptr(), out_degree.data_ptr(),

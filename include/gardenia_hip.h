@@ -81,6 +81,11 @@ typedef struct gd_pr_plan_config {
                            value placed back at its CSR position in LDS: same results,
                            fewer L2 requests; costs 2 bytes per edge of plan memory and of
                            index stream; 0: CSR order; -1: auto (1) */
+  int block;            /* threads per row-tile workgroup: 256 (tiles of 256*items_per_thread
+                           edges) or 1024 (four times larger tiles: with sort_tiles the
+                           gathers of a tile are sorted over 4x more edges, so more lanes of
+                           a gather instruction share a cache line; one workgroup per CU);
+                           0: auto (256); the persistent grid always uses 256 */
 } gd_pr_plan_config;
 
 void gd_pr_default_plan_config(gd_pr_plan_config *cfg);
@@ -100,6 +105,7 @@ typedef struct gd_pr_plan_info_t {
   int32_t pull_grid;      /* workgroups per pull launch */
   float plan_ms;          /* plan construction time (validation, relabel, schedule) */
   int32_t sort_tiles;     /* 1 if tiles are gathered in source order */
+  int32_t block;          /* threads per row-tile workgroup */
 } gd_pr_plan_info_t;
 
 /* Plan over a device-resident in-CSR (rows = destinations, columns = sources)
@@ -209,6 +215,11 @@ typedef struct gd_ppr_plan_config {
   int compact;     /* 1 (default, -1; needs k <= 4096): stopped columns are moved behind
                       the running ones between host checks, so their lanes stop gathering;
                       0: stopped columns stay in place (masked) */
+  int hub_cols;    /* columns per workgroup of a hub row (> 4096 in-edges, added in edge
+                      order per column): 16, 32 or 64; narrower groups keep more of the
+                      row's edges in flight per workgroup, wider ones gather each
+                      edge's columns in fewer requests; 0: auto (64, the fastest of
+                      the three for RMAT-22 at k = 256) */
 } gd_ppr_plan_config;
 
 void gd_ppr_default_plan_config(gd_ppr_plan_config *cfg);
@@ -221,6 +232,7 @@ typedef struct gd_ppr_plan_info_t {
   int32_t relabel;
   int32_t grid; /* persistent workgroups per column chunk */
   float plan_ms;
+  int32_t hub_cols;
 } gd_ppr_plan_info_t;
 
 /* k must be a positive multiple of 64.  Arrays are borrowed, as for gd_pr_plan_create. */

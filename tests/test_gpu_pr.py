@@ -23,7 +23,7 @@ def _key(ds):
 
 
 def _gpu_pr(device, in_csr, outdeg, tol, max_iter, seed_vertex=-1, init=None, chunk=0,
-            relabel=None, ipt=0, persistent=None):
+            relabel=None, ipt=0, persistent=None, block=0, sort_tiles=None):
     import torch
 
     m = in_csr.m
@@ -34,7 +34,7 @@ def _gpu_pr(device, in_csr, outdeg, tol, max_iter, seed_vertex=-1, init=None, ch
         init = np.full(m, np.float32(1.0) / np.float32(m), np.float32)
     scores = torch.from_numpy(np.array(init, np.float32)).to(device)
     plan = PageRankPlan(off, col, deg, relabel=relabel, items_per_thread=ipt,
-                        persistent=persistent)
+                        persistent=persistent, block=block, sort_tiles=sort_tiles)
     stats, errs = plan.run(scores, tol=tol, max_iter=max_iter, seed_vertex=seed_vertex,
                            chunk=chunk)
     out = scores.cpu().numpy()
@@ -414,3 +414,19 @@ def test_pr_persistent_tiny_and_empty_tail(device):
                                     persistent=persistent)
         base = (np.float32(1.0) - np.float32(0.85)) / np.float32(m)  # (1.0f - kDamp) / m
         np.testing.assert_array_equal(s, np.full(m, base, np.float32))
+
+
+@pytest.mark.parametrize("scale,relabel,ipt,sort", [(16, True, 16, True), (18, True, 16, True),
+                                                    (18, False, 8, False), (20, True, 16, True)])
+def test_pr_wide_tiles_bit_exact(device, scale, relabel, ipt, sort):
+    """Row tiles on 1024-thread workgroups (tiles of 1024*ipt edges and 4096
+    rows, sorted by source over the whole tile): every score and L1 value is
+    the oracle's, as with 256-thread tiles."""
+    out_csr, in_csr = graph.rmat(scale, 16, seed=3)
+    s, stats, errs, info = _gpu_pr(device, in_csr, out_csr.degree, 1e-5, 100, relabel=relabel,
+                                   ipt=ipt, block=1024, sort_tiles=sort)
+    assert info["block"] == 1024 and info["sort_tiles"] == int(sort)
+    ref, it, ref_errs = oracle.pr(in_csr.m, in_csr.row_offsets, in_csr.column_indices,
+                                  out_csr.degree, tol=1e-5, max_iter=100)
+    np.testing.assert_array_equal(s, ref)
+    assert_same_iterations(stats["iterations"], errs, it, ref_errs, 1e-5)
