@@ -374,7 +374,7 @@ __device__ __forceinline__ double row_tile(const int32_t* __restrict__ off,
 
 constexpr int kScanPT = 16;                   // terms per thread per chunk
 constexpr int kScanChunk = kBlock * kScanPT;  // 4096
-constexpr int kExactHead = 256;               // terms of a row added serially first
+constexpr int kExactHead = kScanChunk;       // terms of a row added by one lane first
 constexpr int kNoBad = 0x7fffffff;
 
 // A run of steps in integer ulps as one map of the running value x:
@@ -400,18 +400,20 @@ __device__ __forceinline__ uint32_t tie_apply(const TieMap& m, uint32_t x) {
   return m.t ? y + (y & 1u) + m.b : y;
 }
 
-// The step of term v in ulps of the binade with exponent ex: T = v / u
-// exactly (a power-of-two scaling; inf when v is far above the binade),
-// rounded to the nearest integer, |q| clamped to 2^26 (such a step leaves the
-// binade anyway).  irregular: a negative tie, or T not finite.
-__device__ __forceinline__ TieMap step_map(float v, int ex, bool& irregular) {
+// The step of term v (sign bit `neg`) in ulps of the binade with exponent ex:
+// T = v / u exactly (a power-of-two scaling; inf when v is far above the
+// binade), rounded to the nearest integer, |q| clamped to 2^26 (such a step
+// leaves the binade anyway).  irregular: a negative tie, or T not finite.
+// The sign comes in as an integer so that no per-term comparison of v is
+// loop-invariant (hoisted lane masks would crowd the scalar registers).
+__device__ __forceinline__ TieMap step_map(float v, uint32_t neg, int ex, uint32_t& irregular) {
   const float T = ldexpf(v, 23 - ex);
   const float fl = floorf(T);
   const float fr = T - fl;  // exact
   const float qf = fminf(fmaxf(fr > 0.5f ? fl + 1.0f : fl, -0x1p26f), 0x1p26f);
-  const bool half = fr == 0.5f;
-  irregular = (half && v < 0.0f) || !isfinite(T);
-  return TieMap{(half && v > 0.0f) ? 1u : 0u, (uint32_t)(int32_t)qf, 0u};
+  const uint32_t half = fr == 0.5f;  // T == +-0 gives fr == 0: no tie
+  irregular = (half & neg) | (uint32_t)!isfinite(T);
+  return TieMap{half & (neg ^ 1u), (uint32_t)(int32_t)qf, 0u};
 }
 
 // Inclusive prefix of the maps over the 64 lanes of a wavefront: DPP row
@@ -443,6 +445,13 @@ struct ExactLds {
   float s;
 };
 
+// Workgroup-uniform values kept in scalar registers: branches on them are
+// scalar branches, not exec-mask updates.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // First value over the workgroup of a per-thread int that increases with the
 // thread index where present (kNoBad elsewhere): ballot per wave, then waves
 // in order.  Uniform result.
@@ -454,12 +463,13 @@ __device__ __forceinline__ int block_first(int v, ExactLds& x) {
   int r = kNoBad;
 #pragma unroll
   for (int w = kWaves - 1; w >= 0; w--) r = (x.wbad[w] != kNoBad) ? x.wbad[w] : r;
-  return r;
+  return uniform(r);
 }
 
 // The scan of one chunk: nc terms in LDS (`val`, thread t's terms at 16t),
 // added to s from term `start` on.  All threads return the same s.  Ends with
-// the workgroup synchronised.
+// the workgroup synchronised.  A thread's terms in [start, nc) are the set
+// bits of its mask `act`; the per-term checks are branch-free selects.
 __device__ __forceinline__ float exact_scan_chunk(const float* __restrict__ val, int nc,
                                                   int start, float s, ExactLds& x) {
   const int tid = threadIdx.x;
@@ -477,46 +487,62 @@ __device__ __forceinline__ float exact_scan_chunk(const float* __restrict__ val,
       v[4 * k + 3] = t.w;
     }
   }
-  while (start < nc) {  // s and start are uniform across the workgroup
+  const uint32_t below_nc = (1u << min(max(nc - lo, 0), kScanPT)) - 1u;
+  uint32_t negs = 0u, nonzero = 0u;  // per-term bits, computed once
+#pragma unroll
+  for (int k = 0; k < kScanPT; k++) {
+    const uint32_t b = __float_as_uint(v[k]);
+    negs |= (b >> 31) << k;
+    nonzero |= (uint32_t)((b << 1) != 0u) << k;
+  }
+  start = uniform(start);
+  s = uniform(s);
+  while (start < nc) {
     const uint32_t sb = __float_as_uint(s);
     const int eb = (int)((sb >> 23) & 0xffu);
     if (eb <= 1 || eb == 0xff) {  // zero, subnormal, [2^-126, 2^-125), inf / NaN
       if (sb == 0u) {
         // +0 + (+-0) == +0: jump to the first nonzero term (+0 + v == v)
-        int nz = kNoBad;
-#pragma unroll
-        for (int k = kScanPT - 1; k >= 0; k--)
-          if (lo + k >= start && lo + k < nc && v[k] != 0.0f) nz = lo + k;
+        const uint32_t cand =
+            nonzero & below_nc & ~((1u << min(max(start - lo, 0), kScanPT)) - 1u);
+        const int nz = cand ? lo + __builtin_ctz(cand) : kNoBad;
         const int jz = block_first(nz, x);
         if (jz == kNoBad) {
           start = nc;
         } else {
-          s = val[jz];
+          s = uniform(val[jz]);
           start = jz + 1;
         }
         __syncthreads();  // wbad is reused
         continue;
       }
-      s = s + val[start];
+      s = uniform(s + val[start]);
       start++;
       continue;
     }
     const int ex = eb - 127;
     const int32_t P = (int32_t)(sb & 0x7fffffu) | 0x800000;
     const uint32_t Ps = (uint32_t)((sb >> 31) ? -P : P);
-    // this thread's steps (identity outside [start, nc)) and their composition
+    const uint32_t act = below_nc & ~((1u << min(max(start - lo, 0), kScanPT)) - 1u);
+    // this thread's steps (identity outside act) and their composition
     uint32_t q[kScanPT];
     uint32_t ties = 0u, irrs = 0u;
     TieMap L{0u, 0u, 0u};
+    if (act) {  // whole waves before a restart skip this
 #pragma unroll
-    for (int k = 0; k < kScanPT; k++) {
-      const bool act = lo + k >= start && lo + k < nc;
-      bool irr;
-      const TieMap mk = step_map(v[k], ex, irr);
-      q[k] = act ? mk.a : 0u;
-      ties |= (act && mk.t) ? (1u << k) : 0u;
-      irrs |= (act && irr) ? (1u << k) : 0u;
-      L = tie_compose(L, TieMap{(ties >> k) & 1u, q[k], 0u});
+      for (int k = 0; k < kScanPT; k++) {
+        const uint32_t a = (act >> k) & 1u;  // integer masks: no lane-mask registers
+        uint32_t irr;
+        const TieMap mk = step_map(v[k], (negs >> k) & 1u, ex, irr);
+        q[k] = mk.a & (0u - a);
+        const uint32_t tk = mk.t & a;
+        ties |= tk << k;
+        irrs |= (irr & a) << k;
+        L = tie_compose(L, TieMap{tk, q[k], 0u});
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kScanPT; k++) q[k] = 0u;
     }
     // workgroup prefix of the maps: waves in order, lanes in order
     const TieMap inc = wave_incl_scan_map(L);
@@ -539,33 +565,34 @@ __device__ __forceinline__ float exact_scan_chunk(const float* __restrict__ val,
       total = tie_compose(total, wm);
     }
     // running values of this thread's terms, in order, up to the first one
-    // whose step is irregular or leaves the binade
+    // whose step is irregular or leaves the binade (|r| in (2^23, 2^24))
     uint32_t run = tie_apply(tie_compose(before, ex_lane), Ps);
     int bad = kNoBad;
-    uint32_t run_bad = 0u;
+    uint32_t run_bad = 0u, alive = 1u;
 #pragma unroll
     for (int k = 0; k < kScanPT; k++) {
-      if (bad == kNoBad && lo + k >= start && lo + k < nc) {
-        const uint32_t y = run + q[k];
-        const int32_t r = (int32_t)(((ties >> k) & 1u) ? y + (y & 1u) : y);
-        if (((irrs >> k) & 1u) || !(abs(r) > (1 << 23) && abs(r) < (1 << 24))) {
-          bad = lo + k;
-          run_bad = run;
-        } else {
-          run = (uint32_t)r;
-        }
-      }
+      const uint32_t y = run + q[k];
+      const uint32_t r = y + (y & (ties >> k) & 1u);  // a tie rounds to even
+      const uint32_t ar = (uint32_t)abs((int32_t)r);
+      // ar in (2^23, 2^24), and a regular step
+      const uint32_t ok = (uint32_t)(ar - 0x800001u < 0x7fffffu) & ~(irrs >> k) & 1u;
+      const uint32_t lv = (act >> k) & alive;
+      const uint32_t fail = lv & (ok ^ 1u);
+      run_bad = fail ? run : run_bad;
+      bad = fail ? lo + k : bad;
+      alive &= fail ^ 1u;
+      run = (lv & ok) ? r : run;
     }
     const int jb = block_first(bad, x);
     if (jb == kNoBad) {
       // exact: an integer below 2^24 in magnitude times a power of two
-      s = ldexpf((float)(int32_t)tie_apply(total, Ps), ex - 23);
+      s = uniform(ldexpf((float)(int32_t)tie_apply(total, Ps), ex - 23));
       start = nc;
       __syncthreads();  // wt/wa/wb and wbad are reused
     } else {
       if (bad == jb) x.s = ldexpf((float)(int32_t)run_bad, ex - 23) + val[jb];  // FP32 step
       __syncthreads();
-      s = x.s;
+      s = uniform(x.s);
       start = jb + 1;
       __syncthreads();
     }
@@ -579,6 +606,9 @@ __device__ __forceinline__ float exact_scan_chunk(const float* __restrict__ val,
 // gathered as thread t's terms t + 256 j (coalesced index loads), stored at
 // their position in the chunk, and scanned as thread t's terms 16t..16t+15;
 // chunk c+1's gathers and chunk c+2's index loads are in flight meanwhile.
+// The first kExactHead terms are one lane's chain (lane_seq_sum): there the
+// running sum crosses binades every few terms, and each crossing would cost
+// the scan a restart.
 template <bool NT, class G>
 __device__ float exact_row_sum(const int32_t* __restrict__ col, int e_max, int e0, int ne,
                                const G& g, float s0, float* __restrict__ buf0,
@@ -616,21 +646,10 @@ __device__ float exact_row_sum(const int32_t* __restrict__ col, int e_max, int e
     __syncthreads();
     int start = 0;
     if (c == 0) {
-      // the first terms, where s doubles every few steps: one lane, in order
+      // the first terms, where s crosses a binade every few steps: one lane's
+      // pipelined chain, in order
       const int nh = min(nc, kExactHead);
-      if (tid == 0) {
-        float a = s;
-        int i = 0;
-        for (; i + 16 <= nh; i += 16) {
-          float t[16];
-#pragma unroll
-          for (int k = 0; k < 16; k++) t[k] = cur[i + k];
-#pragma unroll
-          for (int k = 0; k < 16; k++) a += t[k];
-        }
-        for (; i < nh; i++) a += cur[i];
-        x.s = a;
-      }
+      if (tid == 0) x.s = lane_seq_sum<4>(s, cur, nh);
       __syncthreads();
       s = x.s;
       start = nh;

@@ -124,10 +124,10 @@ __device__ __forceinline__ void fused_finish(const IterArgs& it, double* s_err, 
 // tiles, so more lanes of a gather instruction share a cache line.
 // tile_body: row tile b of the iteration (the caller checked st->done).
 template <class P, int IPT, int NT, bool SORTED, int NB = kBlock>
-__device__ __forceinline__ void tile_body(const typename P::Args& a, const IterArgs& it, int b) {
+__device__ __forceinline__ void tile_body(const typename P::Args& a, const IterArgs& it, int b,
+                                          RowTileLds<IPT, NB>& lds) {
   static_assert(NB == kBlock || P::kExactHubs, "hub segments need kBlock-thread tiles");
   const int odd = it.st->iter & 1;
-  __shared__ RowTileLds<IPT, NB> lds;
   __shared__ double s_err[NB / kWave];
   const Tile t = it.tiles[b];
   const auto g = P::template gather<NT>(a, odd);
@@ -154,7 +154,8 @@ template <class P, int IPT, int NT, bool SORTED, int NB = kBlock>
 __global__ __launch_bounds__(NB) void pull_tile_kernel(const typename P::Args a,
                                                        const IterArgs it) {
   if (it.st->done) return;
-  tile_body<P, IPT, NT, SORTED, NB>(a, it, blockIdx.x);
+  __shared__ RowTileLds<IPT, NB> lds;
+  tile_body<P, IPT, NT, SORTED, NB>(a, it, blockIdx.x, lds);
 }
 
 // Persistent, software-pipelined form: workgroup b walks tiles b, b+G, ...;
@@ -219,12 +220,18 @@ __global__ __launch_bounds__(kBlock) void pull_persistent_kernel(const typename 
 // and beside the row-tile launch, longest row first.  Hub rows are stored in
 // edge order.  hub_body: exact hub h of the iteration (the caller checked
 // st->done).
+struct HubLds {
+  alignas(16) float buf[2 * kScanChunk];  // two chunks of the row, double-buffered
+  ExactLds xs;
+};
+
 template <class P, int IPT, int NT, bool SORTED>
-__device__ __forceinline__ void hub_body(const typename P::Args& a, const IterArgs& it, int h) {
+__device__ __forceinline__ void hub_body(const typename P::Args& a, const IterArgs& it, int h,
+                                         HubLds& lds) {
   const int odd = it.st->iter & 1;
-  __shared__ __align__(16) float buf[2 * kScanChunk];
   __shared__ double s_err[kWaves];
-  __shared__ ExactLds xs;
+  float* const buf = lds.buf;
+  ExactLds& xs = lds.xs;
   const Tile t = it.hub_tiles[h];
   const auto g = P::template gather<NT>(a, odd);
   const auto e = P::template epilogue<NT>(a, odd);
@@ -242,28 +249,36 @@ __device__ __forceinline__ void hub_body(const typename P::Args& a, const IterAr
   }
 }
 
+// Held to 3 waves per SIMD (<= 168 VGPRs, none spilled): the hub workgroups
+// share the CUs with the row tiles running beside them.
 template <class P, int IPT, int NT, bool SORTED>
-__global__ __launch_bounds__(kBlock) void exact_hub_kernel(const typename P::Args a,
-                                                           const IterArgs it) {
+__global__ __launch_bounds__(kBlock, 3) void exact_hub_kernel(const typename P::Args a,
+                                                              const IterArgs it) {
   if (it.st->done) return;
-  hub_body<P, IPT, NT, SORTED>(a, it, blockIdx.x);
+  __shared__ HubLds lds;
+  hub_body<P, IPT, NT, SORTED>(a, it, blockIdx.x, lds);
 }
 
 // Small iterations (it.fuse_parts > 0) in one launch: workgroups [0,
 // n_hub_tiles) take the exact hubs (longest first), the rest the row tiles,
 // and the last workgroup to finish closes the iteration (fused_finish) --
 // one kernel per iteration, no fork, join or finalize.
+// The two bodies share one LDS allocation.
 template <class P, int IPT, int NT, bool SORTED>
-__global__ __launch_bounds__(kBlock) void pull_small_kernel(const typename P::Args a,
-                                                            const IterArgs it) {
+__global__ __launch_bounds__(kBlock, 3) void pull_small_kernel(const typename P::Args a,
+                                                               const IterArgs it) {
   if (it.st->done) return;
+  __shared__ union {
+    RowTileLds<IPT> tile;
+    HubLds hub;
+  } lds;
   if constexpr (P::kExactHubs) {
     if ((int)blockIdx.x < it.n_hub_tiles) {
-      hub_body<P, IPT, NT, SORTED>(a, it, blockIdx.x);
+      hub_body<P, IPT, NT, SORTED>(a, it, blockIdx.x, lds.hub);
       return;
     }
   }
-  tile_body<P, IPT, NT, SORTED>(a, it, blockIdx.x - it.n_hub_tiles);
+  tile_body<P, IPT, NT, SORTED>(a, it, blockIdx.x - it.n_hub_tiles, lds.tile);
 }
 
 // Hub-row epilogues (segment partials summed in segment order) and the L1
