@@ -454,11 +454,15 @@ __device__ __forceinline__ float uniform(float v) {
 
 // First value over the workgroup of a per-thread int that increases with the
 // thread index where present (kNoBad elsewhere): ballot per wave, then waves
-// in order.  Uniform result.
+// in order.  Uniform result.  The cross-lane read runs with the whole wave
+// active: inside `if (lane == 0)` the compiler may compute v in lane 0 only
+// (it is used nowhere else), and the read would return another lane's stale
+// register.
 __device__ __forceinline__ int block_first(int v, ExactLds& x) {
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const uint64_t m = __ballot(v != kNoBad);
-  if (lane == 0) x.wbad[wid] = m ? __builtin_amdgcn_readlane(v, __builtin_ctzll(m)) : kNoBad;
+  const int first = m ? __builtin_amdgcn_readlane(v, __builtin_ctzll(m)) : kNoBad;
+  if (lane == 0) x.wbad[wid] = first;
   __syncthreads();
   int r = kNoBad;
 #pragma unroll
@@ -511,7 +515,7 @@ __device__ __forceinline__ float exact_scan_chunk(const float* __restrict__ val,
           start = nc;
         } else {
           s = uniform(val[jz]);
-          start = jz + 1;
+          start = max(jz, start) + 1;  // every pass advances: the loop ends
         }
         __syncthreads();  // wbad is reused
         continue;
@@ -565,7 +569,9 @@ __device__ __forceinline__ float exact_scan_chunk(const float* __restrict__ val,
       total = tie_compose(total, wm);
     }
     // running values of this thread's terms, in order, up to the first one
-    // whose step is irregular or leaves the binade (|r| in (2^23, 2^24))
+    // whose step is irregular or leaves the binade: |r| in (2^23, 2^24), or
+    // |r| == 2^23 (|s| == 2^k exactly) left unchanged by a zero term or one of
+    // its own sign, whose exact sum cannot have come from below 2^k
     uint32_t run = tie_apply(tie_compose(before, ex_lane), Ps);
     int bad = kNoBad;
     uint32_t run_bad = 0u, alive = 1u;
@@ -574,8 +580,9 @@ __device__ __forceinline__ float exact_scan_chunk(const float* __restrict__ val,
       const uint32_t y = run + q[k];
       const uint32_t r = y + (y & (ties >> k) & 1u);  // a tie rounds to even
       const uint32_t ar = (uint32_t)abs((int32_t)r);
-      // ar in (2^23, 2^24), and a regular step
-      const uint32_t ok = (uint32_t)(ar - 0x800001u < 0x7fffffu) & ~(irrs >> k) & 1u;
+      const uint32_t away = ((nonzero >> k) ^ 1u) | ((negs >> k) ^ (run >> 31) ^ 1u);
+      const uint32_t stay = (uint32_t)(r == run) & (uint32_t)(ar == 0x800000u) & away;
+      const uint32_t ok = ((uint32_t)(ar - 0x800001u < 0x7fffffu) | stay) & ~(irrs >> k) & 1u;
       const uint32_t lv = (act >> k) & alive;
       const uint32_t fail = lv & (ok ^ 1u);
       run_bad = fail ? run : run_bad;
@@ -593,7 +600,7 @@ __device__ __forceinline__ float exact_scan_chunk(const float* __restrict__ val,
       if (bad == jb) x.s = ldexpf((float)(int32_t)run_bad, ex - 23) + val[jb];  // FP32 step
       __syncthreads();
       s = uniform(x.s);
-      start = jb + 1;
+      start = max(jb, start) + 1;
       __syncthreads();
     }
   }
