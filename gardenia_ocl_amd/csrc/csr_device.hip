@@ -19,11 +19,12 @@
 //      the row (graph_io.h:48-61);
 //   5. row offsets from the row boundaries of the sorted records, degree =
 //      row length (graph_io.h:72-78, :289-293).
-// Each step is a hipCUB primitive or a one-pass kernel; the edge list never
+// Each step is a rocPRIM primitive or a one-pass kernel; the edge list never
 // leaves HBM.
 
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -162,12 +163,12 @@ int compact(int64_t n, DeviceBuffer& key, DeviceBuffer& w, const DeviceBuffer& k
   if (!rc) rc = w2.alloc(4 * cap);
   if (rc) return rc;
   size_t tb = 0;
-  GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, keep.as<int32_t>(), pos.as<int32_t>(),
-                                              (int)n, s));
+  GD_HIP_TRY(rocprim::exclusive_scan(nullptr, tb, keep.as<int32_t>(), pos.as<int32_t>(), 0,
+                                     (size_t)n, rocprim::plus<int32_t>(), s));
   rc = tmp.alloc(std::max<size_t>(tb, 1));
   if (rc) return rc;
-  GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.ptr, tb, keep.as<int32_t>(), pos.as<int32_t>(),
-                                              (int)n, s));
+  GD_HIP_TRY(rocprim::exclusive_scan(tmp.ptr, tb, keep.as<int32_t>(), pos.as<int32_t>(), 0,
+                                     (size_t)n, rocprim::plus<int32_t>(), s));
   hipLaunchKernelGGL(scatter_kept_kernel, dim3(grid_for(n)), dim3(kB), 0, s, n,
                      keep.as<int32_t>(), pos.as<int32_t>(), key.as<uint64_t>(), w.as<float>(),
                      key2.as<uint64_t>(), w2.as<float>());
@@ -220,12 +221,12 @@ int build(int m, int64_t n, const int32_t* d_src, const int32_t* d_dst, const fl
     hipLaunchKernelGGL(count_kernel, dim3(grid_for(n)), dim3(kB), 0, s, n, d_src, d_dst, sym,
                        mirror, cnt.as<int64_t>());
     size_t tb = 0;
-    GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.as<int64_t>(), at.as<int64_t>(),
-                                                (int)n, s));
+    GD_HIP_TRY(rocprim::exclusive_scan(nullptr, tb, cnt.as<int64_t>(), at.as<int64_t>(),
+                                       (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), s));
     rc = tmp.alloc(std::max<size_t>(tb, 1));
     if (rc) return rc;
-    GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.ptr, tb, cnt.as<int64_t>(), at.as<int64_t>(),
-                                                (int)n, s));
+    GD_HIP_TRY(rocprim::exclusive_scan(tmp.ptr, tb, cnt.as<int64_t>(), at.as<int64_t>(),
+                                       (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), s));
     int64_t last[2] = {0, 0};
     rc = sync_read(at.as<int64_t>() + (n - 1), &last[0], 8, s);
     if (!rc) rc = sync_read(cnt.as<int64_t>() + (n - 1), &last[1], 8, s);
@@ -253,14 +254,14 @@ int build(int m, int64_t n, const int32_t* d_src, const int32_t* d_dst, const fl
     while (bits < 31 && ((int64_t)1 << bits) < m) bits++;
     const int end_bit = sorted ? 32 + bits : bits;
     size_t tb = 0;
-    GD_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key0.as<uint64_t>(),
-                                                  key.as<uint64_t>(), w0.as<float>(),
-                                                  w.as<float>(), (int)cap, 0, end_bit, s));
+    GD_HIP_TRY(rocprim::radix_sort_pairs(nullptr, tb, key0.as<uint64_t>(), key.as<uint64_t>(),
+                                         w0.as<float>(), w.as<float>(), (size_t)cap, 0, end_bit,
+                                         s));
     rc = tmp.alloc(std::max<size_t>(tb, 1));
     if (rc) return rc;
-    GD_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tb, key0.as<uint64_t>(),
-                                                  key.as<uint64_t>(), w0.as<float>(),
-                                                  w.as<float>(), (int)cap, 0, end_bit, s));
+    GD_HIP_TRY(rocprim::radix_sort_pairs(tmp.ptr, tb, key0.as<uint64_t>(), key.as<uint64_t>(),
+                                         w0.as<float>(), w.as<float>(), (size_t)cap, 0, end_bit,
+                                         s));
     GD_HIP_TRY(hipStreamSynchronize(s));  // scratch is freed on scope exit
   }
   at.release();

@@ -14,7 +14,8 @@
 // at the start of a run and back at the end.
 
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "gd_common.h"
 #include "relabel.h"
@@ -86,23 +87,25 @@ int build_relabel(int m, int64_t nnz, const int32_t* d_off, const int32_t* d_col
   // equal degrees keep increasing ids.
   hipLaunchKernelGGL(iota_kernel, dim3(grid_for(m)), dim3(256), 0, stream, ids.as<int32_t>(), m);
   size_t sort_bytes = 0, scan_bytes = 0;
-  GD_HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(
-      nullptr, sort_bytes, d_outdeg, r->outdeg.as<int32_t>(), ids.as<int32_t>(),
-      r->perm.as<int32_t>(), m, 0, 32, stream));
-  GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, len.as<int32_t>(),
-                                             r->off.as<int32_t>(), m + 1, stream));
+  GD_HIP_TRY(rocprim::radix_sort_pairs_desc(nullptr, sort_bytes, d_outdeg,
+                                             r->outdeg.as<int32_t>(), ids.as<int32_t>(),
+                                             r->perm.as<int32_t>(), (size_t)m, 0, 32, stream));
+  GD_HIP_TRY(rocprim::exclusive_scan(nullptr, scan_bytes, len.as<int32_t>(),
+                                     r->off.as<int32_t>(), 0, (size_t)m + 1,
+                                     rocprim::plus<int32_t>(), stream));
   rc = tmp.alloc(std::max(sort_bytes, scan_bytes));
   if (rc) return rc;
   // (the sorted keys land in r->outdeg temporarily; overwritten below)
-  GD_HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(
-      tmp.ptr, sort_bytes, d_outdeg, r->outdeg.as<int32_t>(), ids.as<int32_t>(),
-      r->perm.as<int32_t>(), m, 0, 32, stream));
+  GD_HIP_TRY(rocprim::radix_sort_pairs_desc(tmp.ptr, sort_bytes, d_outdeg,
+                                             r->outdeg.as<int32_t>(), ids.as<int32_t>(),
+                                             r->perm.as<int32_t>(), (size_t)m, 0, 32, stream));
   hipLaunchKernelGGL(relabel_rows_kernel, dim3(grid_for(m)), dim3(256), 0, stream, m,
                      r->perm.as<int32_t>(), d_off, d_outdeg, r->inv.as<int32_t>(),
                      len.as<int32_t>(), r->outdeg.as<int32_t>());
   GD_HIP_TRY(hipGetLastError());
-  GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.ptr, scan_bytes, len.as<int32_t>(),
-                                             r->off.as<int32_t>(), m + 1, stream));
+  GD_HIP_TRY(rocprim::exclusive_scan(tmp.ptr, scan_bytes, len.as<int32_t>(),
+                                     r->off.as<int32_t>(), 0, (size_t)m + 1,
+                                     rocprim::plus<int32_t>(), stream));
   hipLaunchKernelGGL(relabel_cols_kernel, dim3(grid_for(m)), dim3(256), 0, stream, m,
                      r->perm.as<int32_t>(), r->inv.as<int32_t>(), d_off, d_col,
                      r->off.as<int32_t>(), r->col.as<int32_t>());

@@ -13,12 +13,13 @@
 
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "gardenia_hip.h"
 #include "gd_common.h"
 #include "pull_engine.h"
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 namespace gd {
 
@@ -238,14 +239,14 @@ int build_sliced(gd_spmv_plan* p) {
     return GD_ERR_INVALID;
   }
   size_t bytes = 0;
-  GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt.as<int32_t>(),
-                                             p->soff.as<int32_t>(), rows + 1, p->stream));
+  GD_HIP_TRY(rocprim::exclusive_scan(nullptr, bytes, cnt.as<int32_t>(), p->soff.as<int32_t>(), 0,
+                                     (size_t)rows + 1, rocprim::plus<int32_t>(), p->stream));
   rc = tmp.alloc(bytes);
   if (rc) return rc;
-  for (int q = 0; q < kSlices; q++)
-    GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(
-        tmp.ptr, bytes, cnt.as<int32_t>() + (size_t)q * (rows + 1),
-        p->soff.as<int32_t>() + (size_t)q * (rows + 1), rows + 1, p->stream));
+  for (int q = 0; q < kSlices; q++)  // each slice's row offsets from 0
+    GD_HIP_TRY(rocprim::exclusive_scan(tmp.ptr, bytes, cnt.as<int32_t>() + (size_t)q * (rows + 1),
+                                       p->soff.as<int32_t>() + (size_t)q * (rows + 1), 0,
+                                       (size_t)rows + 1, rocprim::plus<int32_t>(), p->stream));
   std::vector<int32_t> tot(kSlices);
   for (int q = 0; q < kSlices; q++)
     GD_HIP_TRY(hipMemcpyAsync(&tot[q], p->soff.as<int32_t>() + (size_t)q * (rows + 1) + rows, 4,
@@ -447,6 +448,7 @@ int gd_spmv_solve(int m, int nnz, const int32_t* Ap, const int32_t* Aj, const fl
   rc = gd_spmv_plan_create_ex(m, m, nnz, dAp.as<int32_t>(), dAj.as<int32_t>(), dAx.as<float>(),
                               nullptr, GD_SPMV_AUTO, &plan);
   if (rc) return rc;
+  std::unique_ptr<gd_spmv_plan> owner(plan);  // destroyed on every return below
   EventPair evp;
   GD_HIP_TRY(evp.create());
   hipEvent_t t0 = evp.a, t1 = evp.b;
@@ -457,7 +459,6 @@ int gd_spmv_solve(int m, int nnz, const int32_t* Ap, const int32_t* Aj, const fl
   float ms = 0.0f;
   GD_HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
   if (kernel_ms) *kernel_ms = ms;
-  gd_spmv_plan_destroy(plan);
   if (rc) return rc;
   if (m > 0) GD_HIP_TRY(hipMemcpy(y, dy.ptr, 4 * (size_t)m, hipMemcpyDeviceToHost));
   return GD_OK;

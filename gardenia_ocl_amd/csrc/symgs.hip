@@ -24,7 +24,7 @@
 // symgs_color_kernel; the stream orders the colors.
 
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -269,13 +269,13 @@ int plan_build(gd_symgs_plan* p, int m, int64_t nnz, const int32_t* d_Ap, const 
   }
   {
     size_t tb = 0;
-    GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, len.as<int32_t>(),
-                                                p->off.as<int32_t>(), (int)np1, s));
+    GD_HIP_TRY(rocprim::exclusive_scan(nullptr, tb, len.as<int32_t>(), p->off.as<int32_t>(), 0,
+                                       (size_t)np1, rocprim::plus<int32_t>(), s));
     DeviceBuffer tmp;
     rc = tmp.alloc(std::max<size_t>(tb, 1));
     if (rc) return rc;
-    GD_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.ptr, tb, len.as<int32_t>(),
-                                                p->off.as<int32_t>(), (int)np1, s));
+    GD_HIP_TRY(rocprim::exclusive_scan(tmp.ptr, tb, len.as<int32_t>(), p->off.as<int32_t>(), 0,
+                                       (size_t)np1, rocprim::plus<int32_t>(), s));
     GD_HIP_TRY(hipStreamSynchronize(s));
   }
   std::vector<int32_t> h_off(np1);

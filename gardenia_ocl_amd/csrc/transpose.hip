@@ -8,7 +8,7 @@
 // permutation that moves per-edge values between the two orders.
 
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "gd_common.h"
 #include "transpose.h"
@@ -82,14 +82,14 @@ int build_transpose(int rows, int cols, int64_t nnz, const int32_t* d_off, const
   int end_bit = 1;
   while ((1ll << end_bit) < (int64_t)cols + 1 && end_bit < 32) end_bit++;
   size_t bytes = 0;
-  GD_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, d_col, keys.as<int32_t>(),
-                                                ids.as<int32_t>(), t->perm.as<int32_t>(),
-                                                (int)nnz, 0, end_bit, stream));
+  GD_HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, d_col, keys.as<int32_t>(),
+                                       ids.as<int32_t>(), t->perm.as<int32_t>(), (size_t)nnz, 0,
+                                       end_bit, stream));
   rc = tmp.alloc(bytes);
   if (rc) return rc;
-  GD_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, bytes, d_col, keys.as<int32_t>(),
-                                                ids.as<int32_t>(), t->perm.as<int32_t>(),
-                                                (int)nnz, 0, end_bit, stream));
+  GD_HIP_TRY(rocprim::radix_sort_pairs(tmp.ptr, bytes, d_col, keys.as<int32_t>(),
+                                       ids.as<int32_t>(), t->perm.as<int32_t>(), (size_t)nnz, 0,
+                                       end_bit, stream));
   // columns with no edges before the first key
   hipLaunchKernelGGL(fill_int_kernel, dim3(grid64(cols + 1)), dim3(256), 0, stream,
                      t->off.as<int32_t>(), cols + 1, 0);

@@ -210,6 +210,33 @@ def test_spmv_sliced_refresh_values_and_fallback(device):
         y, oracle.spmv(m, un.row_offsets, un.column_indices, Ax, xx, np.zeros(m, np.float32)))
 
 
+def test_spmv_sliced_wide_matrix_many_slices(device):
+    """2^16 rows over 2^24 columns (64 MB of x: 32 column slices of 2 MB),
+    rows of 0..64 sorted columns spread over the slices, some empty: every row
+    is the oracle's sequential sum, y carried from slice to slice."""
+    rng = np.random.default_rng(31)
+    rows, cols = 1 << 16, 1 << 24
+    lens = rng.integers(0, 65, rows)
+    lens[rng.random(rows) < 0.05] = 0
+    off = np.zeros(rows + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    col = np.empty(int(off[-1]), np.int32)
+    for r in range(rows):
+        n = int(lens[r])
+        if n:
+            col[off[r]:off[r + 1]] = np.sort(rng.integers(0, cols - 64, n)) + np.arange(n)
+    csr = graph.Csr(rows, off.astype(np.int32), col, lens.astype(np.int32))
+    Ax = graph.uniform_pm1(csr.column_indices.shape[0], 32)
+    x = graph.uniform_pm1(cols, 33)
+    y0 = graph.uniform_pm1(rows, 34)
+    y, plan, _ = _gpu_spmv_mode(device, csr, Ax, x, y0, "auto", reps=2)
+    assert plan.mode == "sliced"
+    ref = y0
+    for _ in range(2):
+        ref = oracle.spmv(rows, csr.row_offsets, csr.column_indices, Ax, x, ref)
+    np.testing.assert_array_equal(y, ref)
+
+
 @pytest.mark.parametrize("mode", ["auto", "tiles"])
 def test_spmv_er22_config_size_bit_exact(device, mode):
     """The SpMV workload at its measured size: ER 2^22 x 2^22, 32 uniform
