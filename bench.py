@@ -22,6 +22,9 @@ import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# the autotuner's choices for the workloads below on the MI355X
+# (python -m gardenia_ocl_amd.autotune --scale 24 --ppr-k 256 --multigraph 64)
+TUNED = os.path.join(ROOT, "gardenia_ocl_amd", "tuned_mi355x.json")
 sys.path.insert(0, ROOT)
 
 METRIC = "PR GTEPS + % HBM peak at RMAT-24; batched PPR vs looped plain PR speedup"
@@ -280,27 +283,32 @@ def fp64_pagerank(off, col, outdeg, *, damping=0.85, seeds=None, tol=1e-13, max_
     V = int(outdeg.shape[0])
     inv = 1.0 / outdeg.to(torch.float64).clamp(min=1)
     A = torch.sparse_csr_tensor(off, col, inv[col.long()], (V, V))
+
+    def solve(base, x):  # one column: torch's CSR x vector product
+        it = 0
+        while it < max_iter:
+            xn = base + damping * (A @ x)
+            err = float((xn - x).abs().sum())
+            x = xn
+            it += 1
+            if err < tol:
+                break
+        return x[:, 0], it
+
     if seeds is None:
         base = torch.full((V, 1), (1.0 - damping) / V, dtype=torch.float64, device=off.device)
         x = torch.full((V, 1), 1.0 / V, dtype=torch.float64, device=off.device)
-    else:
-        k = len(seeds)
-        sd = torch.as_tensor(seeds, dtype=torch.int64, device=off.device)
-        ar = torch.arange(k, device=off.device)
-        base = torch.zeros((V, k), dtype=torch.float64, device=off.device)
-        base[sd, ar] = 1.0 - damping
-        x = torch.zeros((V, k), dtype=torch.float64, device=off.device)
-        x[sd, ar] = 1.0
-    it = 0
-    while it < max_iter:
-        xn = base + damping * (A @ x)
-        err = float((xn - x).abs().sum(0).max())
-        x = xn
-        it += 1
-        if err < tol:
-            break
-    del A
-    return (x[:, 0] if seeds is None else x), it
+        return solve(base, x)
+    cols, its = [], 0
+    for sd in seeds:
+        base = torch.zeros((V, 1), dtype=torch.float64, device=off.device)
+        base[int(sd), 0] = 1.0 - damping
+        x = torch.zeros((V, 1), dtype=torch.float64, device=off.device)
+        x[int(sd), 0] = 1.0
+        xs, it = solve(base, x)
+        cols.append(xs)
+        its = max(its, it)
+    return torch.stack(cols, 1), its
 
 
 def bench_tol_run(plan, off, col, deg, tol, max_iter):
@@ -395,9 +403,11 @@ def bench_small(scale, tol, max_iter, seed, reps=7):
     return rec, (s.cpu().numpy(), errs, its)
 
 
-def bench_ppr(scale, k, tol, seed, fp64_seeds, hub_cols=0):
+def bench_ppr(scale, k, tol, seed, fp64_seeds, hub_cols=0, tune_cache=None):
     """Batched PPR (k seeds in one SpMM-style pull) vs k looped single-seed
-    runs of the plain pull kernel, both to the same per-seed L1 tolerance."""
+    runs of the plain pull kernel, both to the same per-seed L1 tolerance.
+    The batched plan takes the tuned configuration for this graph and k from
+    ``tune_cache`` when it holds one (hub_cols > 0 overrides it)."""
     import numpy as np
     import torch
 
@@ -412,20 +422,34 @@ def bench_ppr(scale, k, tol, seed, fp64_seeds, hub_cols=0):
     col = torch.from_numpy(ic.column_indices).to(dev)
     deg = torch.from_numpy(od).to(dev)
     tseeds = torch.from_numpy(seeds).to(dev)
-    bp = PersonalizedPageRankPlan(off, col, deg, k, hub_cols=hub_cols)
+    ppr_kw, source = {}, "library defaults"
+    if tune_cache:
+        from gardenia_ocl_amd import autotune
+
+        tuned = autotune.ppr_config(tune_cache, off, col, deg, k)
+        if tuned:
+            ppr_kw, source = dict(tuned), f"autotuned ({os.path.relpath(tune_cache, ROOT)})"
+    if hub_cols:
+        ppr_kw["hub_cols"], source = hub_cols, "command line"
+    log(f"batched ppr: RMAT-{scale} built, plan {ppr_kw or 'defaults'} ...")
+    bp = PersonalizedPageRankPlan(off, col, deg, k, **ppr_kw)
     X = bp.one_hot(seeds)
     bp.run(tseeds, X, tol=tol, max_iter=2)  # warm-up
     X = bp.one_hot(seeds)
     torch.cuda.synchronize()
+    log("batched ppr: plan and warm-up done; timed run ...")
     t0 = time.perf_counter()
     bst, _, _ = bp.run(tseeds, X, tol=tol, max_iter=300)
     torch.cuda.synchronize()
     t_batched = time.perf_counter() - t0
+    log(f"batched ppr: {t_batched:.3f} s, {bst['iterations']} iterations; single-seed plan ...")
     pp = PageRankPlan(off, col, deg)
     s = torch.zeros(V, dtype=torch.float32, device=dev)
+    s[int(seeds[0])] = 1.0
     pp.run(s, tol=tol, max_iter=2, seed_vertex=int(seeds[0]))  # warm-up
     looped = []
     its = 0
+    log("batched ppr: looped single-seed runs ...")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for j in range(k):
@@ -440,7 +464,9 @@ def bench_ppr(scale, k, tol, seed, fp64_seeds, hub_cols=0):
     # L1 against the FP64 personalised solution for the first seeds
     l1_b, l1_l = [], []
     if fp64_seeds > 0:
+        log(f"looped: {t_looped:.3f} s; FP64 reference of {fp64_seeds} seeds ...")
         X64, _ = fp64_pagerank(off, col, deg, seeds=seeds[:fp64_seeds], tol=1e-13)
+        log("FP64 reference done")
         l1_b = [float((X[:, j].double() - X64[:, j]).abs().sum()) for j in range(fp64_seeds)]
         l1_l = [float((looped[j].double() - X64[:, j]).abs().sum()) for j in range(fp64_seeds)]
     # algorithmic bytes of the batched solve (every array once per iteration,
@@ -458,7 +484,7 @@ def bench_ppr(scale, k, tol, seed, fp64_seeds, hub_cols=0):
             "speedup": round(t_looped / t_batched, 2), "batched_iterations": bst["iterations"],
             "batched_column_iterations_total": col_its,
             "looped_iterations_total": its, "k_chunk": bp.info()["k_chunk"],
-            "hub_cols": bp.info()["hub_cols"],
+            "hub_cols": bp.info()["hub_cols"], "plan_config_source": source,
             "l1_vs_fp64_batched": l1_b, "l1_vs_fp64_looped": l1_l,
             "fp64_seeds_checked": fp64_seeds}
 
@@ -543,10 +569,12 @@ def bench_symgs(n, rmat_scale, seed, reps=20):
     return out
 
 
-def bench_multigraph(n, lo, hi, tol, seed):
+def bench_multigraph(n, lo, hi, tol, seed, tune_cache=None):
     """n RMAT graphs of scales uniform in [lo, hi]: one multi-graph plan (one
     pull + one finalize launch per iteration for all graphs, replayed from a
-    hipGraph) vs one single-graph plan per graph run back to back."""
+    hipGraph) vs one single-graph plan per graph run back to back.  The
+    iterations per captured chunk come from ``tune_cache`` when it holds a
+    tuned value for this set of graphs."""
     import numpy as np
     import torch
 
@@ -563,15 +591,23 @@ def bench_multigraph(n, lo, hi, tol, seed):
     init = [float(np.float32(1.0) / np.float32(int(g[2].shape[0]))) for g in gs]
     scores = [torch.empty(int(g[2].shape[0]), dtype=torch.float32, device=dev) for g in gs]
     mg = MultiGraphPageRank(gs)
+    run_kw, source = {}, "library defaults"
+    if tune_cache:
+        from gardenia_ocl_amd import autotune
+
+        tuned = autotune.multigraph_config(tune_cache, gs)
+        if tuned:
+            run_kw, source = dict(tuned), f"autotuned ({os.path.relpath(tune_cache, ROOT)})"
     out = {"workload": f"{n} RMAT graphs, scales {lo}-{hi} (seeded), L1 tol {tol}",
-           "scales": [int(x) for x in scales], "total_edges": E}
+           "scales": [int(x) for x in scales], "total_edges": E,
+           "chunk": run_kw.get("chunk", 8), "plan_config_source": source}
     for use_graph in (True, False):
         for rep in range(2):  # first run captures / warms up
             for s, v in zip(scores, init):
                 s.fill_(v)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            its, errs, ms = mg.run(scores, tol=tol, max_iter=100, use_graph=use_graph)
+            its, errs, ms = mg.run(scores, tol=tol, max_iter=100, use_graph=use_graph, **run_kw)
             torch.cuda.synchronize()
             wall = time.perf_counter() - t0
         key = "hipgraph" if use_graph else "stream"
@@ -695,7 +731,14 @@ def main():
     ap.add_argument("--nt", type=int, default=-1, help="non-temporal level 0/1/2 (-1 auto)")
     ap.add_argument("--persistent", choices=["auto", "on", "off"], default="auto")
     ap.add_argument("--sort-tiles", choices=["auto", "on", "off"], default="auto")
-    ap.add_argument("--block", type=int, default=0, help="threads per row tile (256/1024, 0=auto)")
+    ap.add_argument("--block", type=int, default=0, help="threads per row tile (256/512/1024, 0=auto)")
+    ap.add_argument("--hub-threshold", type=int, default=0,
+                    help="rows longer than this are whole-row exact hubs (0=auto: the tile size)")
+    ap.add_argument("--tune-cache", default=TUNED,
+                    help="autotuner cache (python -m gardenia_ocl_amd.autotune --cache FILE): "
+                         "the headline, batched-PPR and multi-graph plans take its tuned "
+                         "configuration for their graphs when it holds one (default: the "
+                         "package's MI355X cache; '' for library defaults)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the tol-1e-6 run, SpMV (ER 2^22 x 32), batched-PPR (RMAT-22, "
                          "k=256), multi-graph and adjoint figures")
@@ -731,8 +774,8 @@ def main():
         print(json.dumps({"small": bench_small(16, 1e-4, 100, args.seed)[0]}), flush=True)
         return
     if args.only == "ppr":
-        print(json.dumps({"ppr": bench_ppr(args.ppr_scale, args.ppr_k, args.ppr_tol, args.seed, 0,
-                                           args.ppr_hub_cols)}), flush=True)
+        print(json.dumps({"ppr": bench_ppr(args.ppr_scale, args.ppr_k, args.ppr_tol, args.seed, 8,
+                                           args.ppr_hub_cols, args.tune_cache)}), flush=True)
         return
     if args.only == "spmv":
         print(json.dumps({"spmv": bench_spmv(args.spmv_log2, 32, 50, args.seed)[0]}), flush=True)
@@ -764,12 +807,22 @@ def main():
     deg = torch.from_numpy(out_deg).to(dev)
     scores = torch.empty(V, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
-    relabel = {"auto": None, "on": True, "off": False}[args.relabel]
-    plan = PageRankPlan(off, col, deg, stream=stream, relabel=relabel, items_per_thread=args.ipt,
-                        nontemporal=None if args.nt < 0 else args.nt,
-                        persistent={"auto": None, "on": True, "off": False}[args.persistent],
-                        sort_tiles={"auto": None, "on": True, "off": False}[args.sort_tiles],
-                        block=args.block)
+    tri = {"auto": None, "on": True, "off": False}
+    plan_kw = dict(relabel=tri[args.relabel], items_per_thread=args.ipt,
+                   nontemporal=None if args.nt < 0 else args.nt,
+                   persistent=tri[args.persistent], sort_tiles=tri[args.sort_tiles],
+                   block=args.block, hub_threshold=args.hub_threshold)
+    config_source = "library defaults and command-line knobs"
+    if args.tune_cache:
+        from gardenia_ocl_amd import autotune
+
+        tuned = autotune.pagerank_config(args.tune_cache, off, col, deg)
+        if tuned:
+            plan_kw = tuned
+            config_source = f"autotuned ({os.path.relpath(args.tune_cache, ROOT)})"
+        else:
+            log(f"no tuned configuration for this graph in {args.tune_cache}: defaults")
+    plan = PageRankPlan(off, col, deg, stream=stream, **plan_kw)
     info = plan.info()
     log(f"plan: {info}")
     init = float(np.float32(1.0) / np.float32(V))
@@ -838,7 +891,8 @@ def main():
         extras["spmv"], checks["spmv"] = bench_spmv(args.spmv_log2, 32, 50, args.seed)
         torch.cuda.empty_cache()
         log("batched ppr ...")
-        extras["ppr"] = bench_ppr(args.ppr_scale, args.ppr_k, args.ppr_tol, args.seed, 8)
+        extras["ppr"] = bench_ppr(args.ppr_scale, args.ppr_k, args.ppr_tol, args.seed, 8,
+                                  args.ppr_hub_cols, args.tune_cache)
         torch.cuda.empty_cache()
         log("device CSR build ...")
         extras["csr_build"] = bench_csr_build(args.scale, args.seed, host_build_s, E)
@@ -847,7 +901,7 @@ def main():
         extras["symgs"] = bench_symgs(128, 20, args.seed)
         torch.cuda.empty_cache()
         log("multi-graph ...")
-        extras["multigraph"] = bench_multigraph(64, 12, 18, 1e-4, args.seed)
+        extras["multigraph"] = bench_multigraph(64, 12, 18, 1e-4, args.seed, args.tune_cache)
         torch.cuda.empty_cache()
         log("adjoint ...")
         extras["adjoint"] = bench_adjoint(20, args.spmv_log2, args.seed)
@@ -876,7 +930,8 @@ def main():
                    "relabel": bool(info["relabel"]), "items_per_thread": info["items_per_thread"],
                    "nontemporal": info["nontemporal"], "persistent": bool(info["persistent"]),
                    "pull_grid": info["pull_grid"], "sort_tiles": bool(info["sort_tiles"]),
-                   "block": info["block"],
+                   "block": info["block"], "hub_threshold": info["hub_threshold"],
+                   "plan_config_source": config_source,
                    "plan_ms": round(info["plan_ms"], 1)},
         "roofline": roof, "cpu_baseline": cpu, "production": production,
     }

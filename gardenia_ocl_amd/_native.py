@@ -74,7 +74,8 @@ class PrStats(ctypes.Structure):
 class PrPlanConfig(ctypes.Structure):
     _fields_ = [("relabel", ctypes.c_int), ("items_per_thread", ctypes.c_int),
                 ("nontemporal", ctypes.c_int), ("persistent", ctypes.c_int),
-                ("sort_tiles", ctypes.c_int), ("block", ctypes.c_int)]
+                ("sort_tiles", ctypes.c_int), ("block", ctypes.c_int),
+                ("hub_threshold", ctypes.c_int)]
 
 
 class PrPlanInfo(ctypes.Structure):
@@ -91,6 +92,7 @@ class PrPlanInfo(ctypes.Structure):
         ("plan_ms", ctypes.c_float),
         ("sort_tiles", ctypes.c_int32),
         ("block", ctypes.c_int32),
+        ("hub_threshold", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:

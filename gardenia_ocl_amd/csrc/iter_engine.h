@@ -207,9 +207,14 @@ struct IterEngine {
 
   template <class P, int IPT, int NT, bool SORTED>
   void launch_wide(const typename P::Args& a, const IterArgs& it, hipStream_t ts) {
-    if constexpr (wide_tiles<P>::value)
-      hipLaunchKernelGGL((pull_tile_kernel<P, IPT, NT, SORTED, 1024>), dim3(n_tiles), dim3(1024),
-                         0, ts, a, it);
+    if constexpr (wide_tiles<P>::value) {
+      if (block == 512)
+        hipLaunchKernelGGL((pull_tile_kernel<P, IPT, NT, SORTED, 512>), dim3(n_tiles), dim3(512),
+                           0, ts, a, it);
+      else
+        hipLaunchKernelGGL((pull_tile_kernel<P, IPT, NT, SORTED, 1024>), dim3(n_tiles),
+                           dim3(1024), 0, ts, a, it);
+    }
   }
 
   template <class P, int IPT, int NT, bool SORTED>
@@ -240,7 +245,7 @@ struct IterEngine {
       if (persistent)
         hipLaunchKernelGGL((pull_persistent_kernel<P, IPT, NT, SORTED>), dim3(pull_grid),
                            dim3(kBlock), 0, ts, a, it);
-      else if (wide_tiles<P>::value && block == 1024)
+      else if (wide_tiles<P>::value && block > kBlock)
         launch_wide<P, IPT, NT, SORTED>(a, it, ts);
       else
         hipLaunchKernelGGL((pull_tile_kernel<P, IPT, NT, SORTED>), dim3(n_tiles), dim3(kBlock), 0,

@@ -119,6 +119,7 @@ void gd_pr_default_plan_config(gd_pr_plan_config* c) {
   c->persistent = -1;
   c->sort_tiles = -1;
   c->block = 0;
+  c->hub_threshold = 0;
 }
 
 int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t* d_col,
@@ -139,7 +140,11 @@ int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t
   GD_CHECK_ARG(c.relabel >= -1 && c.relabel <= 1, "relabel must be -1, 0 or 1");
   GD_CHECK_ARG(c.nontemporal >= -1 && c.nontemporal <= 2, "nontemporal must be -1, 0, 1 or 2");
   GD_CHECK_ARG(c.persistent >= -1 && c.persistent <= 1, "persistent must be -1, 0 or 1");
-  GD_CHECK_ARG(c.block == 0 || c.block == 256 || c.block == 1024, "block must be 0, 256 or 1024");
+  GD_CHECK_ARG(c.block == 0 || c.block == 256 || c.block == 512 || c.block == 1024,
+               "block must be 0, 256, 512 or 1024");
+  const int ipt = c.items_per_thread ? c.items_per_thread : 16;
+  GD_CHECK_ARG(c.hub_threshold == 0 || (c.hub_threshold >= 64 && c.hub_threshold <= kBlock * ipt),
+               "hub_threshold must be 0 or in [64, 256*items_per_thread]");
   if (gd_device_count() <= 0) {
     set_error("no HIP device visible");
     return GD_ERR_NO_DEVICE;
@@ -152,17 +157,17 @@ int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t
   // Defaults from RMAT-24 measurements on MI355X: 16 edges per thread, no
   // cache hints, one workgroup per tile (the pipelined persistent form needs
   // 86 VGPRs, which cuts occupancy to 5 workgroups per CU and runs slower).
-  p->eng.ipt = c.items_per_thread ? c.items_per_thread : 16;
+  p->eng.ipt = ipt;
   p->eng.nt = c.nontemporal < 0 ? 0 : c.nontemporal;
   p->eng.persistent = c.persistent == 1;
   p->eng.sort_tiles = c.sort_tiles != 0;
-  p->eng.block = (c.block == 1024 && !p->eng.persistent) ? 1024 : kBlock;
-  // Rows longer than 256*ipt edges are whole-row hubs summed in the
-  // reference's order and rounding (exact_hub_kernel); shorter rows by their
-  // owner thread in a row tile: every score is the reference's bit for bit
-  // (its sequential FP32 sum drifts by 3e-4 relative on the 238,668-edge
-  // RMAT-24 row, so matching its rounding matters).
-  p->eng.exact_len = kBlock * p->eng.ipt;
+  p->eng.block = (c.block > kBlock && !p->eng.persistent) ? c.block : kBlock;
+  // Rows longer than hub_threshold (256*ipt by default) edges are whole-row
+  // hubs summed in the reference's order and rounding (exact_hub_kernel);
+  // shorter rows by their owner thread in a row tile: every score is the
+  // reference's bit for bit (its sequential FP32 sum drifts by 3e-4 relative
+  // on the 238,668-edge RMAT-24 row, so matching its rounding matters).
+  p->eng.exact_len = c.hub_threshold ? c.hub_threshold : kBlock * ipt;
   int rc = GD_OK;
   if (m > 0) {
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -204,6 +209,7 @@ int gd_pr_plan_info(const gd_pr_plan* p, gd_pr_plan_info_t* info) {
   info->plan_ms = p->plan_ms;
   info->sort_tiles = p->eng.sort_tiles ? 1 : 0;
   info->block = p->eng.block;
+  info->hub_threshold = p->eng.exact_len;
   return GD_OK;
 }
 

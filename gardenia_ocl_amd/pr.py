@@ -42,12 +42,17 @@ def _as_i32(a) -> np.ndarray:
 
 def PRSolver(m: int, nnz: int, in_row_offsets, in_column_indices, out_row_offsets,
              out_column_indices, degree, scores: np.ndarray, *, damping=K_DAMP, tol=EPSILON,
-             max_iter=MAX_ITER, seed_vertex=-1, verbose=False, variant="hip_pull"):
+             max_iter=MAX_ITER, seed_vertex=-1, verbose=False, variant="hip_pull",
+             config=None):
     """PRSolver(m, nnz, in_off, in_col, out_off, out_col, degree, scores) on the GPU.
 
     ``scores`` (float32, length m) is updated in place, as in the reference.
     Returns (stats dict, per-iteration L1 errors).  Raises GardeniaError on any
     failure (the reference exit()s instead, include/cutil_subset.h:4-12).
+    ``config``: PageRankPlan keyword arguments (e.g. a tuned configuration,
+    ``autotune.pagerank_config``); the solve then runs on a plan built with
+    them (same scores and iterations: no knob changes the arithmetic);
+    None: the library's one-shot gd_pr_solve with its defaults.
     """
     if not (isinstance(scores, np.ndarray) and scores.dtype == np.float32
             and scores.flags.c_contiguous):
@@ -64,9 +69,24 @@ def PRSolver(m: int, nnz: int, in_row_offsets, in_column_indices, out_row_offset
     errors = np.zeros(max(1, max_iter), np.float64)
     if verbose:
         print("Launching HIP PR solver (gfx950 pull, fused contrib/L1) ...")
-    N.call("gd_pr_solve", m, nnz, _np_ptr(in_off), _np_ptr(in_col), _np_ptr(out_off),
-           _np_ptr(out_col), _np_ptr(deg), _np_ptr(scores), ctypes.byref(opts),
-           ctypes.byref(stats), _np_ptr(errors))
+    if config is None:
+        N.call("gd_pr_solve", m, nnz, _np_ptr(in_off), _np_ptr(in_col), _np_ptr(out_off),
+               _np_ptr(out_col), _np_ptr(deg), _np_ptr(scores), ctypes.byref(opts),
+               ctypes.byref(stats), _np_ptr(errors))
+    else:
+        import torch
+
+        dev = torch.device("cuda", torch.cuda.current_device())
+        t = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731
+        plan = PageRankPlan(t(in_off), t(in_col[:nnz]), t(deg), **config)
+        x = t(scores)
+        st, errs = plan.run(x, damping=damping, tol=tol, max_iter=max_iter,
+                            seed_vertex=seed_vertex)
+        scores[:] = x.cpu().numpy()
+        plan.close()
+        for name, _ in N.PrStats._fields_:
+            setattr(stats, name, st[name])
+        errors[: len(errs)] = errs
     errs = errors[: stats.iterations].copy()
     if verbose:
         for i, e in enumerate(errs):
@@ -133,11 +153,13 @@ class PageRankPlan:
 
     def __init__(self, in_row_offsets, in_column_indices, out_degree, stream=None, *,
                  relabel=None, items_per_thread=0, nontemporal=None, persistent=None,
-                 sort_tiles=None, block=0):
+                 sort_tiles=None, block=0, hub_threshold=0):
         """relabel / nontemporal / persistent / sort_tiles: None = library default,
         True / False force it (sort_tiles: gather each tile's edges in source
-        order; same results).  block: threads per row-tile workgroup (256 or
-        1024; 0 = library default); same results."""
+        order; same results).  block: threads per row-tile workgroup (256, 512
+        or 1024; 0 = library default).  hub_threshold: rows with more in-edges
+        are whole-row hubs of their own workgroup (64 .. 256*items_per_thread;
+        0 = library default).  Every setting gives the same results."""
         import torch
 
         for t in (in_row_offsets, in_column_indices, out_degree):
@@ -161,6 +183,7 @@ class PageRankPlan:
         if sort_tiles is not None:
             cfg.sort_tiles = 1 if sort_tiles else 0
         cfg.block = int(block)
+        cfg.hub_threshold = int(hub_threshold)
         h = ctypes.c_void_p()
         N.call("gd_pr_plan_create_ex", self.m, self.nnz, in_row_offsets.data_ptr(),
                in_column_indices.data_ptr(), out_degree.data_ptr(),

@@ -82,10 +82,16 @@ typedef struct gd_pr_plan_config {
                            fewer L2 requests; costs 2 bytes per edge of plan memory and of
                            index stream; 0: CSR order; -1: auto (1) */
   int block;            /* threads per row-tile workgroup: 256 (tiles of 256*items_per_thread
-                           edges) or 1024 (four times larger tiles: with sort_tiles the
-                           gathers of a tile are sorted over 4x more edges, so more lanes of
-                           a gather instruction share a cache line; one workgroup per CU);
-                           0: auto (256); the persistent grid always uses 256 */
+                           edges), 512 or 1024 (two or four times larger tiles: with
+                           sort_tiles the gathers of a tile are sorted over more edges, so
+                           more lanes of a gather instruction share a cache line, at fewer
+                           workgroups per CU); 0: auto (256); the persistent grid always
+                           uses 256 */
+  int hub_threshold;    /* rows with more in-edges than this are whole-row hubs summed by a
+                           workgroup of their own (exact_hub_kernel), the rest by their
+                           owner thread inside a row tile; 64 .. 256*items_per_thread;
+                           0: auto (256*items_per_thread, the longest row a 256-thread
+                           tile holds) */
 } gd_pr_plan_config;
 
 void gd_pr_default_plan_config(gd_pr_plan_config *cfg);
@@ -106,6 +112,7 @@ typedef struct gd_pr_plan_info_t {
   float plan_ms;          /* plan construction time (validation, relabel, schedule) */
   int32_t sort_tiles;     /* 1 if tiles are gathered in source order */
   int32_t block;          /* threads per row-tile workgroup */
+  int32_t hub_threshold;  /* rows longer than this are whole-row hubs */
 } gd_pr_plan_info_t;
 
 /* Plan over a device-resident in-CSR (rows = destinations, columns = sources)

@@ -72,6 +72,30 @@ def test_argument_errors_before_device():
     assert lib.gd_symgs_plan_destroy(None) == N.GD_OK
 
 
+def test_pr_plan_config_rejected_before_device():
+    """Out-of-range plan knobs are argument errors, checked before any device
+    work (an empty graph, so no device array is needed)."""
+    from gardenia_ocl_amd import _native as N
+
+    lib = N.lib()
+    h = ctypes.c_void_p()
+    for field, value, text in (("hub_threshold", 63, b"hub_threshold"),
+                               ("hub_threshold", 4097, b"hub_threshold"),
+                               ("block", 300, b"block"), ("items_per_thread", 4, b"items_per_thread")):
+        cfg = N.PrPlanConfig()
+        lib.gd_pr_default_plan_config(ctypes.byref(cfg))
+        assert cfg.hub_threshold == 0 and cfg.block == 0
+        setattr(cfg, field, value)
+        assert lib.gd_pr_plan_create_ex(0, 0, None, None, None, None, ctypes.byref(cfg),
+                                        ctypes.byref(h)) == N.GD_ERR_INVALID, field
+        assert text in lib.gd_last_error()
+    cfg = N.PrPlanConfig()
+    lib.gd_pr_default_plan_config(ctypes.byref(cfg))
+    cfg.items_per_thread, cfg.hub_threshold = 8, 2049  # above 256*8
+    assert lib.gd_pr_plan_create_ex(0, 0, None, None, None, None, ctypes.byref(cfg),
+                                    ctypes.byref(h)) == N.GD_ERR_INVALID
+
+
 def test_default_options_are_reference_constants():
     from gardenia_ocl_amd import _native as N
 

@@ -23,7 +23,7 @@ def _key(ds):
 
 
 def _gpu_pr(device, in_csr, outdeg, tol, max_iter, seed_vertex=-1, init=None, chunk=0,
-            relabel=None, ipt=0, persistent=None, block=0, sort_tiles=None):
+            relabel=None, ipt=0, persistent=None, block=0, sort_tiles=None, hub_threshold=0):
     import torch
 
     m = in_csr.m
@@ -34,7 +34,8 @@ def _gpu_pr(device, in_csr, outdeg, tol, max_iter, seed_vertex=-1, init=None, ch
         init = np.full(m, np.float32(1.0) / np.float32(m), np.float32)
     scores = torch.from_numpy(np.array(init, np.float32)).to(device)
     plan = PageRankPlan(off, col, deg, relabel=relabel, items_per_thread=ipt,
-                        persistent=persistent, block=block, sort_tiles=sort_tiles)
+                        persistent=persistent, block=block, sort_tiles=sort_tiles,
+                        hub_threshold=hub_threshold)
     stats, errs = plan.run(scores, tol=tol, max_iter=max_iter, seed_vertex=seed_vertex,
                            chunk=chunk)
     out = scores.cpu().numpy()
@@ -416,16 +417,70 @@ def test_pr_persistent_tiny_and_empty_tail(device):
         np.testing.assert_array_equal(s, np.full(m, base, np.float32))
 
 
+@pytest.mark.parametrize("hub_threshold,ipt", [(64, 16), (1024, 16), (2048, 8)])
+def test_pr_hub_threshold_bit_exact(device, hub_threshold, ipt):
+    """Any cut between row-tile rows and whole-row exact hubs gives the
+    oracle's scores and iteration count (rows above the cut are summed by the
+    workgroup's binade scan, below it by their owner thread)."""
+    out_csr, in_csr = graph.rmat(16, 16, seed=5)
+    s, stats, errs, info = _gpu_pr(device, in_csr, out_csr.degree, 1e-6, 100, ipt=ipt,
+                                   hub_threshold=hub_threshold)
+    assert info["hub_threshold"] == hub_threshold
+    deg = np.diff(in_csr.row_offsets)
+    assert info["tiles"] > 0 and (deg > hub_threshold).sum() > 0
+    ref, it, ref_errs = oracle.pr(in_csr.m, in_csr.row_offsets, in_csr.column_indices,
+                                  out_csr.degree, tol=1e-6, max_iter=100)
+    np.testing.assert_array_equal(s, ref)
+    assert_same_iterations(stats["iterations"], errs, it, ref_errs, 1e-6)
+
+
+def test_pr_personalized_exact_hubs_power_of_two_sums(device):
+    """One-hot personalised starts on a graph where every row above 64
+    in-edges is an exact hub: a hub row gathering the seed holds the sum
+    1/outdeg(seed) -- an exact power of two for outdeg 1, 2, 4, ... -- across
+    long runs of zero terms.  Scores and iteration counts are the oracle's
+    scalar personalised loop's, bit for bit."""
+    import torch
+
+    out_csr, in_csr = graph.rmat(16, 16, seed=7)
+    m = in_csr.m
+    od = out_csr.degree
+    deg = np.diff(in_csr.row_offsets)
+    hubs = np.nonzero(deg > 64)[0]
+    picks = []
+    for want in (1, 2, 4, 8, 3):  # powers of two, and one that is not
+        for h in hubs:
+            srcs = in_csr.column_indices[in_csr.row_offsets[h]:in_csr.row_offsets[h + 1]]
+            cand = srcs[od[srcs] == want]
+            if cand.size:
+                picks.append(int(cand[0]))
+                break
+    assert len(picks) >= 4
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(device)  # noqa: E731
+    plan = PageRankPlan(t(in_csr.row_offsets), t(in_csr.column_indices), t(od), hub_threshold=64)
+    assert plan.info()["hub_threshold"] == 64
+    for seed in picks:
+        s = torch.zeros(m, dtype=torch.float32, device=device)
+        s[seed] = 1.0
+        st, errs = plan.run(s, tol=1e-6, max_iter=100, seed_vertex=seed)
+        ref, it, ref_errs = oracle.ppr(m, in_csr.row_offsets, in_csr.column_indices, od, seed,
+                                       tol=1e-6, max_iter=100)
+        np.testing.assert_array_equal(s.cpu().numpy(), ref)
+        assert st["iterations"] == it
+    plan.close()
+
+
+@pytest.mark.parametrize("block", [512, 1024])
 @pytest.mark.parametrize("scale,relabel,ipt,sort", [(16, True, 16, True), (18, True, 16, True),
                                                     (18, False, 8, False), (20, True, 16, True)])
-def test_pr_wide_tiles_bit_exact(device, scale, relabel, ipt, sort):
-    """Row tiles on 1024-thread workgroups (tiles of 1024*ipt edges and 4096
-    rows, sorted by source over the whole tile): every score and L1 value is
-    the oracle's, as with 256-thread tiles."""
+def test_pr_wide_tiles_bit_exact(device, block, scale, relabel, ipt, sort):
+    """Row tiles on 512- and 1024-thread workgroups (tiles of block*ipt edges
+    and 4*block rows, sorted by source over the whole tile): every score and L1
+    value is the oracle's, as with 256-thread tiles."""
     out_csr, in_csr = graph.rmat(scale, 16, seed=3)
     s, stats, errs, info = _gpu_pr(device, in_csr, out_csr.degree, 1e-5, 100, relabel=relabel,
-                                   ipt=ipt, block=1024, sort_tiles=sort)
-    assert info["block"] == 1024 and info["sort_tiles"] == int(sort)
+                                   ipt=ipt, block=block, sort_tiles=sort)
+    assert info["block"] == block and info["sort_tiles"] == int(sort)
     ref, it, ref_errs = oracle.pr(in_csr.m, in_csr.row_offsets, in_csr.column_indices,
                                   out_csr.degree, tol=1e-5, max_iter=100)
     np.testing.assert_array_equal(s, ref)
