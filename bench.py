@@ -708,11 +708,14 @@ def bench_adjoint(scale, spmv_log2, seed):
 
 
 def traffic_from_profile(scale):
+    """The pull phase's counters measured with rocprofv3 (HBM bytes from
+    FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy, L2 requests and hits from
+    TCC_REQ / TCC_HIT) for this workload, or {} when none is committed."""
     path = os.path.join(ROOT, "profiles", f"pr_pull_rmat{scale}_traffic.json")
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        return json.load(f)
 
 
 def main():
@@ -851,9 +854,13 @@ def main():
     avg_pull_ms = pull_ms / max(1, pull_n)
     bytes_iter = 4 * E + 24 * V + 4
     achieved = bytes_iter / (avg_pull_ms * 1e-3) / 1e9
+    prof = traffic_from_profile(args.scale)
     roof = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(args.scale),
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": prof.get("hbm_bytes_per_launch"),
+        "traffic_kernels": prof.get("phase_kernels"),
+        "l2_requests_per_edge": prof.get("l2_requests_per_edge"),
+        "l2_hit_rate": prof.get("l2_hit_rate"),
         "kernel": (f"{'pull_persistent_kernel' if info['persistent'] else 'pull_tile_kernel'}"
                    f"<PrPolicy, {info['items_per_thread']}, {info['nontemporal']}, "
                    f"{'true' if info['sort_tiles'] else 'false'}"

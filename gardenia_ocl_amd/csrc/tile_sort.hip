@@ -87,9 +87,12 @@ int build_tile_sort(const Schedule& s, int chunk, const int32_t* d_col, int32_t*
   if (rc) return rc;
   GD_HIP_TRY(hipMemcpyAsync(d_segs.ptr, segs.data(), sizeof(int2) * segs.size(),
                             hipMemcpyHostToDevice, stream));
-  if (chunk > 4096)  // 128 KB of keys: one workgroup per CU
+  if (chunk > 8192)  // 128 KB of keys: one workgroup per CU
     hipLaunchKernelGGL(segment_sort_kernel<kSortMax>, dim3((unsigned)segs.size()),
                        dim3(kSortBlock), 0, stream, d_segs.as<int2>(), d_col, d_col_sorted, d_pos);
+  else if (chunk > 4096)  // 64 KB of keys
+    hipLaunchKernelGGL(segment_sort_kernel<8192>, dim3((unsigned)segs.size()), dim3(kSortBlock), 0,
+                       stream, d_segs.as<int2>(), d_col, d_col_sorted, d_pos);
   else
     hipLaunchKernelGGL(segment_sort_kernel<4096>, dim3((unsigned)segs.size()), dim3(kSortBlock), 0,
                        stream, d_segs.as<int2>(), d_col, d_col_sorted, d_pos);
