@@ -50,11 +50,13 @@ typedef struct gd_pr_stats {
   float solve_ms;       /* HIP-event time of the whole iteration loop */
   float pull_ms;        /* summed pull-kernel time (profile != 0), else 0 */
   int pull_launches;    /* pull kernels timed */
-  int near_tie;         /* 1 if one of the last two L1 changes lies within
-                           1e-7 + 1e-6*tol of tol: the measured difference between
-                           this library's and the reference's L1 values (rows of
-                           65-4096 in-edges are tree-summed), where a stop decision
-                           could fall one iteration apart */
+  int near_tie;         /* 1 if one of the last two L1 changes lies within the band
+                           of tol where this library's and the reference's L1 values
+                           can differ, so a stop decision could fall one iteration
+                           apart: 1e-12 + 1e-12*tol for PageRank (every row summed in
+                           the reference's order: the L1 values differ only in their
+                           FP64 summation order), 1e-7 + 1e-6*tol for the adjoint
+                           (tree-summed rows) */
 } gd_pr_stats;
 
 void gd_pr_default_options(gd_pr_options *opts);
