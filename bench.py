@@ -363,6 +363,65 @@ def bench_verify(scale, seed, scores_h):
     return rec, out_csr
 
 
+def bench_push(out_csr, iters, scores_pull):
+    """Push PageRank (the reference's scatter variant) on the same RMAT
+    out-CSR, the same fixed iterations from 1/V: FP32 atomic adds into HBM.
+    Scores against the pull path's (FP32 summation-order noise)."""
+    import numpy as np
+    import torch
+
+    from gardenia_ocl_amd import pagerank_push
+
+    dev = torch.device("cuda:0")
+    off = torch.from_numpy(out_csr.row_offsets).to(dev)
+    col = torch.from_numpy(out_csr.column_indices).to(dev)
+    m, E = out_csr.m, out_csr.nnz
+    init = float(np.float32(1.0) / np.float32(m))
+    s = torch.full((m,), init, dtype=torch.float32, device=dev)
+    pagerank_push(off, col, s, tol=0.0, max_iter=2)  # warm-up
+    s.fill_(init)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, _ = pagerank_push(off, col, s, tol=0.0, max_iter=iters)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    got = s.cpu().numpy()
+    ref = np.asarray(scores_pull, np.float32)
+    del off, col, s
+    return {"workload": f"push PageRank over the RMAT out-CSR ({E} edges), {iters} iterations",
+            "ms_per_iteration": round(st["solve_ms"] / iters, 4),
+            "gteps": round(E * iters / (st["solve_ms"] * 1e-3) / 1e9, 2),
+            "wall_ms": round(wall * 1e3, 2),
+            "max_rel_vs_pull": float(np.max(np.abs(got - ref) / np.maximum(ref, 1e-30))),
+            "l1_vs_pull": float(np.abs(got.astype(np.float64) - ref).sum())}
+
+
+def bench_oneshot(in_csr, out_deg, iters):
+    """The drop-in path a reference caller takes: PRSolver on host arrays
+    (gd_pr_solve: copies the in-CSR to HBM, builds a plan, iterates, copies
+    the scores back), timed end to end on the headline graph; the iteration
+    loop alone is the library's HIP-event solve time."""
+    import numpy as np
+
+    from gardenia_ocl_amd import PRSolver
+
+    m, E = in_csr.m, in_csr.nnz
+    init = np.float32(1.0) / np.float32(m)
+    s = np.full(m, init, np.float32)
+    PRSolver(m, E, in_csr.row_offsets, in_csr.column_indices, None, None, out_deg, s,
+             tol=0.0, max_iter=2)  # warm-up
+    s[:] = init
+    t0 = time.perf_counter()
+    st, _ = PRSolver(m, E, in_csr.row_offsets, in_csr.column_indices, None, None, out_deg, s,
+                     tol=0.0, max_iter=iters)
+    wall = time.perf_counter() - t0
+    return {"workload": f"PRSolver on host arrays, {iters} iterations, V={m} E={E}",
+            "wall_ms": round(wall * 1e3, 2), "solve_ms": round(st["solve_ms"], 3),
+            "copy_and_plan_ms": round(wall * 1e3 - st["solve_ms"], 2),
+            "gteps_end_to_end": round(E * iters / wall / 1e9, 2),
+            "gteps_iterations_only": round(E * iters / (st["solve_ms"] * 1e-3) / 1e9, 2)}
+
+
 def bench_small(scale, tol, max_iter, seed, reps=7):
     """A whole solve on a small graph (RMAT-16, tol 1e-4, max 100: the
     reference's CPU-runnable case) with the plan reused, scores reset between
@@ -889,8 +948,14 @@ def main():
         checks["tol_run"] = (tol_scores, tol_errs, tol_x64)
         del plan, off, col, deg, scores
         torch.cuda.empty_cache()
+        log("one-shot PRSolver on host arrays ...")
+        extras["oneshot"] = bench_oneshot(in_csr, out_deg, args.iters)
+        torch.cuda.empty_cache()
         log("device verifier ...")
         extras["verify"], verify_csr = bench_verify(args.scale, args.seed, scores_h)
+        torch.cuda.empty_cache()
+        log("push PageRank ...")
+        extras["push"] = bench_push(verify_csr, args.iters, scores_h)
         torch.cuda.empty_cache()
         log("small graph ...")
         extras["small"], checks["small"] = bench_small(16, 1e-4, 100, args.seed)

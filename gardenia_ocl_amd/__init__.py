@@ -9,14 +9,15 @@ tensors.
 """
 from . import graph
 from ._native import GardeniaError, LIB_PATH
-from .pr import EPSILON, K_DAMP, MAX_ITER, PageRankPlan, PRSolver, PRVerifier, verify_residual
+from .pr import (EPSILON, K_DAMP, MAX_ITER, PageRankPlan, PRSolver, PRVerifier, pagerank_push,
+                 verify_residual)
 from .spmv import SpmvSolver, SpmvPlan
 from .symgs import SymGSPlan, SymGSSolver
 from .ppr import PersonalizedPageRankPlan
 from .multigraph import MultiGraphPageRank
 
 __all__ = [
-    "graph", "GardeniaError", "PersonalizedPageRankPlan", "MultiGraphPageRank", "LIB_PATH", "PRSolver", "PRVerifier", "verify_residual", "PageRankPlan", "SpmvSolver", "SpmvPlan",
+    "graph", "GardeniaError", "PersonalizedPageRankPlan", "MultiGraphPageRank", "LIB_PATH", "PRSolver", "PRVerifier", "pagerank_push", "verify_residual", "PageRankPlan", "SpmvSolver", "SpmvPlan",
     "SymGSPlan", "SymGSSolver",
     "EPSILON", "K_DAMP", "MAX_ITER",
 ]

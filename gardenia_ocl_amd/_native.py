@@ -189,6 +189,8 @@ SIGNATURES = {
     "gd_device_csr_copy": (_i, [ctypes.POINTER(DeviceCsr), _P, _P, _P, _P, _P]),
     "gd_device_csr_free": (_i, [ctypes.POINTER(DeviceCsr)]),
     "gd_pr_verify": (_i, [_i, _i64, _P, _P, _P, _P, ctypes.c_float, _P, _F64P]),
+    "gd_pr_push_solve": (_i, [_i, _i64, _P, _P, _P, ctypes.POINTER(PrOptions), _P,
+                              ctypes.POINTER(PrStats), _P]),
     "gd_symgs_plan_create": (_i, [_i, _i64, _P, _P, _P, _P, _P, _i, _P, ctypes.POINTER(_P)]),
     "gd_symgs_plan_run": (_i, [_P, _P, _P, _i, _P]),
     "gd_symgs_plan_info": (_i, [_P, ctypes.POINTER(SymgsPlanInfo)]),

@@ -99,6 +99,32 @@ def PRSolver(m: int, nnz: int, in_row_offsets, in_column_indices, out_row_offset
     return stats.as_dict(), errs
 
 
+def pagerank_push(out_row_offsets, out_column_indices, scores, *, damping=K_DAMP, tol=EPSILON,
+                  max_iter=MAX_ITER, stream=None):
+    """Push-style PageRank over the out-CSR (torch int32 CUDA tensors; scores
+    float32, in/out): the reference's scatter variant (src/pr/scatter.cu:11-41),
+    FP32 atomic adds in the hardware's order, a cross-check of the pull path.
+    Continues while the L1 change is > tol, at least once.  Returns (stats
+    dict, per-iteration L1 changes)."""
+    import torch
+
+    for t, dt in ((out_row_offsets, torch.int32), (out_column_indices, torch.int32),
+                  (scores, torch.float32)):
+        if not (t.is_cuda and t.dtype == dt and t.is_contiguous()):
+            raise TypeError("pagerank_push inputs must be contiguous CUDA tensors")
+    m = int(scores.shape[0])
+    if int(out_row_offsets.shape[0]) != m + 1:
+        raise ValueError("out_row_offsets must have m+1 entries")
+    st = stream if stream is not None else torch.cuda.current_stream()
+    opts = make_options(damping, tol, max_iter)
+    stats = N.PrStats()
+    errors = np.zeros(max(1, max_iter), np.float64)
+    N.call("gd_pr_push_solve", m, int(out_column_indices.shape[0]), out_row_offsets.data_ptr(),
+           out_column_indices.data_ptr(), scores.data_ptr(), ctypes.byref(opts),
+           ctypes.c_void_p(st.cuda_stream), ctypes.byref(stats), _np_ptr(errors))
+    return stats.as_dict(), errors[: stats.iterations].copy()
+
+
 def verify_residual(out_row_offsets, out_column_indices, degree, scores, *, damping=K_DAMP,
                     stream=None) -> float:
     """The reference verifier's push residual on the GPU (torch CUDA tensors:

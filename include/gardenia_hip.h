@@ -165,6 +165,23 @@ int gd_device_csr_copy(const gd_device_csr *c, int32_t *d_row_offsets, int32_t *
                        float *d_weight, int32_t *d_degree, void *stream);
 int gd_device_csr_free(gd_device_csr *c);
 
+/* Push-style PageRank over the OUT-CSR: the reference's scatter variant
+ * (src/pr/scatter.cu:11-41 and its host loop :57-78), a cross-check of the pull
+ * path.  Every iteration sets next[v] = (1-d)/m, adds kDamp*score[u]/outdeg[u]
+ * (outdeg[u] = u's out-CSR row length) over every out-edge (u, v) with hardware
+ * FP32 atomic adds (addition order: the hardware's, as the reference's
+ * atomicAdd), sums |next - score| in FP64 in a fixed order, and sets
+ * score = next; it continues while that change is > tol (at least one
+ * iteration, the reference's do-while).  Scores agree with the pull path to
+ * FP32 summation-order noise.  d_scores (float[m], HBM) holds the initial
+ * scores and receives the result; opts->seed_vertex must be -1; errors (host,
+ * room for max(1, max_iter) values, may be NULL) receives each iteration's
+ * change.  The out-CSR is validated on the device first. */
+int gd_pr_push_solve(int m, int64_t nnz, const int32_t *d_out_row_offsets,
+                     const int32_t *d_out_column_indices, float *d_scores,
+                     const gd_pr_options *opts, void *stream, gd_pr_stats *stats,
+                     double *errors);
+
 /* The reference PRVerifier's residual (src/pr/verifier.cc:10-69, the push part
  * :50-68) on the device: one push iteration of the tested scores over the
  * OUT-CSR (d_degree = out-degree), residual = sum_v |base + d*sums_v -

@@ -38,6 +38,7 @@ _SIG = {
     "orc_pr": (_i, [_i, _P, _P, _P, _P, _f, _d, _i, _P]),
     "orc_ppr": (_i, [_i, _P, _P, _P, _i, _P, _f, _d, _i, _P]),
     "orc_pr_residual": (_d, [_i, _P, _P, _P, _P, _f]),
+    "orc_pr_push": (_i, [_i, _P, _P, _P, _f, _d, _i, _P]),
     "orc_pr_fp64": (_i, [_i, _P, _P, _P, _P, _d, _d, _i]),
     "orc_ppr_fp64": (_i, [_i, _P, _P, _P, _i, _P, _P, _d, _d, _i]),
     "orc_spmv": (None, [_i, _P, _P, _P, _P, _P]),
@@ -134,6 +135,17 @@ def ppr(m, in_off, in_col, degree, seed, *, damping=0.85, tol=1e-3, max_iter=30)
     errs = np.zeros(max(1, max_iter), np.float64)
     it = lib().orc_ppr(m, _p(in_off), _p(in_col), _p(degree), int(seed), _p(s), damping, tol,
                        max_iter, _p(errs))
+    return s, it, errs[:it].copy()
+
+
+def pr_push(m, out_off, out_col, scores=None, *, damping=0.85, tol=1e-3, max_iter=30):
+    """src/pr/scatter.cu:11-41 (push over the out-CSR, additions in out-CSR
+    order).  Returns (scores, iterations, errors)."""
+    out_off, out_col = _i32(out_off), _i32(out_col)
+    s = (np.full(m, np.float32(1.0) / np.float32(m), np.float32) if scores is None
+         else np.array(scores, np.float32, copy=True))
+    errs = np.zeros(max(1, max_iter), np.float64)
+    it = lib().orc_pr_push(m, _p(out_off), _p(out_col), _p(s), damping, tol, max_iter, _p(errs))
     return s, it, errs[:it].copy()
 
 

@@ -20,6 +20,8 @@
 //   orc_spmv          golden y from the reference's SpmvSolver, plus the
 //                     hand-computed nvGRAPH 5x5 example.
 //   orc_pr_residual   golden residual from the reference PRVerifier.
+//   orc_pr_push       the pull oracle's scores and iteration counts to FP32
+//                     summation-order noise, and the FP64 fixed point.
 //   orc_max_rel_err   golden values on crafted vectors.
 //   orc_pr_fp64       dense LAPACK solve of (I - d A^T D^-1) x = (1-d)/m.
 //   orc_ppr_fp64      linearity + dense solve on small graphs.
@@ -263,6 +265,38 @@ double orc_pr_residual(int m, const int* out_row_offsets, const int* out_column_
     error += std::fabs(new_score - scores_to_test[i]);
   }
   return error;
+}
+
+// src/pr/scatter.cu:11-41 and its host loop (:57-78): push PageRank over the
+// OUT-CSR.  next starts at base; every source adds kDamp*score/degree (its
+// out-CSR row length; float, (kDamp*score) first) to each out-neighbour --
+// here in out-CSR order, one of the orders the reference's atomicAdds can
+// take; then the L1 change, the swap and the reset of next to base.  The loop
+// continues while the change is > tol (:78; at least one iteration).  The L1
+// change is summed in FP64 (the reference: FP32 block sums and an FP32
+// atomicAdd, :36-40).  Returns the iterations.
+int orc_pr_push(int m, const int* out_row_offsets, const int* out_column_indices, float* scores,
+                float damp, double tol, int max_iter, double* errors) {
+  const float base_score = (1.0f - damp) / m;  // :57
+  std::vector<float> next(m, base_score);     // initialize, :11-14
+  int iter = 0;
+  do {
+    ++iter;
+    for (int src = 0; src < m; src++) {  // scatter, :16-27
+      const int b = out_row_offsets[src], e = out_row_offsets[src + 1];
+      const float value = damp * scores[src] / (float)(e - b);
+      for (int k = b; k < e; k++) next[out_column_indices[k]] += value;
+    }
+    double diff = 0;
+    for (int v = 0; v < m; v++) {  // reduce, :30-41
+      diff += std::fabs(next[v] - scores[v]);
+      scores[v] = next[v];
+      next[v] = base_score;
+    }
+    if (errors) errors[iter - 1] = diff;
+    if (!(diff > tol)) break;
+  } while (iter < max_iter);
+  return iter;
 }
 
 // FP64 power iteration of the same operator (uniform teleport, dropped

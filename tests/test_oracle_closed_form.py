@@ -211,3 +211,53 @@ def test_spmv_bwd_fp64_dense():
     np.testing.assert_allclose(dx, Af.T @ g.astype(np.float64), rtol=1e-12, atol=1e-12)
     r, c = np.nonzero(A)
     np.testing.assert_allclose(dA, g[r].astype(np.float64) * x[c], rtol=1e-15)
+
+
+def test_pr_push_hand_computed_iteration():
+    """One push iteration on 0->1, 0->2, 1->2, 2->0, 3->0 from 1/4 each, in FP32
+    and the reference's order (next = base, then each source's kDamp*score/deg
+    added to its out-neighbours in out-CSR order)."""
+    f = np.float32
+    off = np.array([0, 2, 3, 4, 5], np.int32)
+    col = np.array([1, 2, 2, 0, 0], np.int32)
+    d, q = f(0.85), f(0.25)
+    base = (f(1.0) - d) / f(4)
+    v0, v1, v2, v3 = d * q / f(2), d * q / f(1), d * q / f(1), d * q / f(1)
+    expect = np.array([(base + v2) + v3, base + v0, (base + v0) + v1, base], np.float32)
+    s, it, errs = oracle.pr_push(4, off, col, np.full(4, q, np.float32), tol=0.0, max_iter=1)
+    assert it == 1
+    np.testing.assert_array_equal(s, expect)
+    assert errs[0] == float(np.abs(expect.astype(np.float64) - 0.25).sum())
+
+
+@pytest.mark.parametrize("name,directed", [("test_pr.mtx", True), ("chesapeake.mtx", False)])
+def test_pr_push_matches_pull_oracle_and_fixed_point(name, directed):
+    """Push and pull are the same iteration up to the FP32 summation order: the
+    scores agree to order noise, the stop iteration is the same (the push
+    loop continues while the change is > tol, the pull loop stops once it is
+    < tol), and both approach the FP64 fixed point.  A push over the in-CSR,
+    or with the in-degree, fails the fixed point."""
+    o, i = graph.read_pr_graph(dataset(name), is_directed=directed)
+    for tol in (1e-3, 1e-6):
+        sp, itp, ep = oracle.pr_push(o.m, o.row_offsets, o.column_indices, tol=tol, max_iter=200)
+        sq, itq, eq = oracle.pr(o.m, i.row_offsets, i.column_indices, o.degree, tol=tol,
+                                max_iter=200)
+        assert itp == itq
+        np.testing.assert_allclose(sp, sq, rtol=2e-6, atol=1e-9)
+    x, _ = oracle.pr_fp64(o.m, i.row_offsets, i.column_indices, o.degree, tol=1e-15)
+    sp, _, _ = oracle.pr_push(o.m, o.row_offsets, o.column_indices, tol=0.0, max_iter=300)
+    assert np.abs(sp - x).sum() < 1e-5
+    if o.m > 4:  # transposed operator: not the fixed point
+        wrong, _, _ = oracle.pr_push(o.m, i.row_offsets, i.column_indices, tol=0.0, max_iter=300)
+        assert directed is False or np.abs(wrong - x).sum() > 1e-3
+
+
+def test_pr_push_rmat_matches_pull_oracle():
+    o, i = graph.rmat(12, 16, seed=5)
+    sp, itp, ep = oracle.pr_push(o.m, o.row_offsets, o.column_indices, tol=1e-6, max_iter=200)
+    sq, itq, eq = oracle.pr(o.m, i.row_offsets, i.column_indices, o.degree, tol=1e-6,
+                            max_iter=200)
+    assert itp == itq
+    assert np.max(np.abs(sp - sq) / sq) < 1e-5
+    # per-iteration L1: FP32 order noise of ~1e-10 per vertex, summed over 4096
+    np.testing.assert_allclose(ep, eq, rtol=1e-5, atol=1e-6)
