@@ -230,7 +230,7 @@ def tune_ppr(in_row_offsets, in_column_indices, out_degree, k, *, iters=3, repea
         return float(np.median(times))
 
     space = dict(space or PPR_SPACE)
-    space["k_chunk"] = [c for c in space["k_chunk"] if k % c == 0]
+    space["k_chunk"] = [c for c in space["k_chunk"] if c and k % c == 0] or [0]  # 0: auto
     best, _, timings = descend(space, measure, log=log)
     db[key] = {"best": best, "timings": timings, "tuned_at": time.time(),
                "unit": "ms per iteration"}
