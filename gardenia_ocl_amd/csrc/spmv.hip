@@ -77,16 +77,17 @@ __global__ __launch_bounds__(kBlock) void spmv_pull_kernel(
 //
 // For matrices whose x does not fit in one XCD's L2 (ER 2^22: 16.8 MB of x vs
 // 4 MB) every gather of the tile engine misses L2.  The sliced mode splits the
-// columns into kSlices ranges, one per XCD (workgroups are dispatched to the 8
-// XCDs round-robin, so block b runs slice b % 8 and each L2 holds one 2 MB
-// slice of x), and runs two passes:
-//   1. products p = x[Aj] * Ax over a slice-major copy of the matrix (fully
-//      coalesced; x gathers hit L2);
-//   2. y[r] += products of row r, slice 0's then slice 1's ..., i.e. in the
-//      original column order for rows with sorted columns -- bit-identical to
-//      the reference's sequential sum.
+// columns into kSlices ranges of x (2 MB each for ER 2^22) and keeps a
+// slice-major copy of the matrix: slice 0's entries of every row (rows in
+// order, each row's entries in column order), then slice 1's, ...  One SpMV
+// runs the row engine once per slice, slice 0 first: pass q sums each row's
+// slice-q products by the row's owner thread onto y[r] (LDS-staged tiles of
+// that slice's entries; the x gathers of a pass stay in one 2 MB range, which
+// every XCD's L2 holds).  For rows with sorted columns the passes add each
+// row's products in column order -- the reference's loop
+// (src/spmv/omp_base.cc:21-26) -- so y is its result bit for bit, with y
+// itself carrying the running sum between passes.
 constexpr int kSlices = 8;
-constexpr int kSliceIPT = 16;
 
 // cnt[s * (rows+1) + r] = entries of row r in column slice s.
 __global__ void slice_count_kernel(int rows, const int32_t* __restrict__ Ap,
@@ -130,64 +131,6 @@ __global__ void slice_scatter_kernel(int rows, const int32_t* __restrict__ Ap,
   }
 }
 
-// Pass 1: block b takes slice b % kSlices, chunk b / kSlices of that slice.
-__global__ __launch_bounds__(kBlock) void slice_products_kernel(
-    const int32_t* __restrict__ Aj_s, const float* __restrict__ Ax_s,
-    const int64_t* __restrict__ sbase, const float* __restrict__ x, float* __restrict__ p) {
-  const int q = blockIdx.x % kSlices;
-  const int64_t c = blockIdx.x / kSlices;
-  const int64_t lo = sbase[q] + c * (kBlock * kSliceIPT);
-  const int64_t hi = min(sbase[q + 1], lo + (int64_t)kBlock * kSliceIPT);
-  if (lo >= hi) return;  // this slice has fewer chunks than the longest one
-  int j[kSliceIPT];
-  float a[kSliceIPT];
-#pragma unroll
-  for (int i = 0; i < kSliceIPT; i++) {
-    const int64_t k = min(lo + threadIdx.x + (int64_t)i * kBlock, hi - 1);
-    j[i] = __builtin_nontemporal_load(Aj_s + k);
-    a[i] = __builtin_nontemporal_load(Ax_s + k);
-  }
-  float v[kSliceIPT];
-#pragma unroll
-  for (int i = 0; i < kSliceIPT; i++) v[i] = x[j[i]] * a[i];
-#pragma unroll
-  for (int i = 0; i < kSliceIPT; i++) {
-    const int64_t k = lo + threadIdx.x + (int64_t)i * kBlock;
-    if (k < hi) p[k] = v[i];
-  }
-}
-
-// Pass 2: y[r] = y[r] + products of row r in slice order (thread per row).
-__global__ __launch_bounds__(kBlock) void slice_rowsum_kernel(int rows,
-                                                              const int32_t* __restrict__ soff,
-                                                              const int64_t* __restrict__ sbase,
-                                                              const float* __restrict__ p,
-                                                              float* __restrict__ y) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= rows) return;
-  int b[kSlices], e[kSlices];
-#pragma unroll
-  for (int q = 0; q < kSlices; q++) {
-    b[q] = soff[(int64_t)q * (rows + 1) + r];
-    e[q] = soff[(int64_t)q * (rows + 1) + r + 1];
-  }
-  float s = y[r];
-#pragma unroll
-  for (int q = 0; q < kSlices; q++) {
-    const float* pq = p + sbase[q];
-    int k = b[q];
-    for (; k + 4 <= e[q]; k += 4) {
-      const float a0 = pq[k], a1 = pq[k + 1], a2 = pq[k + 2], a3 = pq[k + 3];
-      s += a0;
-      s += a1;
-      s += a2;
-      s += a3;
-    }
-    for (; k < e[q]; k++) s += pq[k];
-  }
-  y[r] = s;
-}
-
 }  // namespace gd
 
 using namespace gd;
@@ -204,10 +147,13 @@ struct gd_spmv_plan {
   DeviceBuffer tiles, long_tiles;
   DeviceBuffer dummy_col;  // 1 element, used as Aj when nnz == 0
   int e_max = 0;
-  // column-sliced mode
+  // column-sliced mode: per slice, its long rows then its row tiles in
+  // slice_tiles[slice_first[q] ...]
   bool sliced = false;
-  int width = 0, slice_blocks = 0;
-  DeviceBuffer soff, sbase, Aj_s, Ax_s, prod;
+  int width = 0;
+  DeviceBuffer soff, sbase, Aj_s, Ax_s, slice_tiles;
+  std::vector<int64_t> slice_base;
+  std::vector<int> slice_first, slice_long, slice_rows;
 };
 
 namespace {
@@ -219,7 +165,6 @@ int build_sliced(gd_spmv_plan* p) {
   if (!rc) rc = p->sbase.alloc(8 * (kSlices + 1));
   if (!rc) rc = p->Aj_s.alloc(4 * (size_t)p->nnz);
   if (!rc) rc = p->Ax_s.alloc(4 * (size_t)p->nnz);
-  if (!rc) rc = p->prod.alloc(4 * (size_t)p->nnz);
   if (rc) return rc;
   DeviceBuffer cnt, bad, tmp;
   rc = cnt.alloc(4 * (size_t)kSlices * (rows + 1));
@@ -253,22 +198,48 @@ int build_sliced(gd_spmv_plan* p) {
                               hipMemcpyDeviceToHost, p->stream));
   GD_HIP_TRY(hipStreamSynchronize(p->stream));
   std::vector<int64_t> base(kSlices + 1, 0);
-  int64_t chunks = 0;
-  for (int q = 0; q < kSlices; q++) {
-    base[q + 1] = base[q] + tot[q];
-    chunks = std::max<int64_t>(chunks, (tot[q] + kBlock * kSliceIPT - 1) / (kBlock * kSliceIPT));
-  }
+  for (int q = 0; q < kSlices; q++) base[q + 1] = base[q] + tot[q];
   if (base[kSlices] != p->nnz) {
     set_error("column-sliced SpMV: slice counts do not add up to nnz");
     return GD_ERR_INVALID;
   }
-  p->slice_blocks = (int)(chunks * kSlices);
   GD_HIP_TRY(hipMemcpyAsync(p->sbase.ptr, base.data(), 8 * (kSlices + 1), hipMemcpyHostToDevice,
                             p->stream));
   hipLaunchKernelGGL(slice_scatter_kernel, dim3(g), dim3(256), 0, p->stream, rows, p->Ap, p->Aj,
                      p->Ax, p->width, p->soff.as<int32_t>(), p->sbase.as<int64_t>(),
                      p->Aj_s.as<int32_t>(), p->Ax_s.as<float>(), 0);
   GD_HIP_TRY(hipGetLastError());
+  // one row-tile schedule per slice, over that slice's row offsets
+  std::vector<int32_t> h_soff((size_t)kSlices * (rows + 1));
+  GD_HIP_TRY(hipMemcpyAsync(h_soff.data(), p->soff.ptr, 4 * h_soff.size(), hipMemcpyDeviceToHost,
+                            p->stream));
+  GD_HIP_TRY(hipStreamSynchronize(p->stream));
+  std::vector<Tile> all;
+  p->slice_base.assign(base.begin(), base.end());
+  p->slice_first.assign(kSlices, 0);
+  p->slice_long.assign(kSlices, 0);
+  p->slice_rows.assign(kSlices, 0);
+  for (int q = 0; q < kSlices; q++) {
+    p->slice_first[q] = (int)all.size();
+    if (tot[q] == 0) continue;  // no entries: the pass would leave y unchanged
+    Schedule sc;
+    rc = build_schedule(rows, tot[q], h_soff.data() + (size_t)q * (rows + 1), kSpTileNnz,
+                        kTileRows, &sc, kSpTileNnz);
+    if (rc) return rc;
+    std::vector<Tile> row_tiles, longs;
+    for (const Tile& t : sc.tiles) (t.y < 0 ? longs : row_tiles).push_back(t);
+    std::stable_sort(longs.begin(), longs.end(),  // the longest rows start first
+                     [](const Tile& a, const Tile& b) { return a.w - a.z > b.w - b.z; });
+    p->slice_long[q] = (int)longs.size();
+    p->slice_rows[q] = (int)row_tiles.size();
+    all.insert(all.end(), longs.begin(), longs.end());
+    all.insert(all.end(), row_tiles.begin(), row_tiles.end());
+  }
+  rc = p->slice_tiles.alloc(sizeof(Tile) * std::max<size_t>(all.size(), 1));
+  if (rc) return rc;
+  if (!all.empty())
+    GD_HIP_TRY(hipMemcpyAsync(p->slice_tiles.ptr, all.data(), sizeof(Tile) * all.size(),
+                              hipMemcpyHostToDevice, p->stream));
   GD_HIP_TRY(hipStreamSynchronize(p->stream));
   return GD_OK;
 }
@@ -369,12 +340,22 @@ int gd_spmv_plan_run(gd_spmv_plan* p, const float* d_x, float* d_y) {
   if (p->rows == 0) return GD_OK;
   GD_CHECK_ARG(d_y != nullptr && (p->nnz == 0 || d_x != nullptr), "NULL x / y");
   if (p->sliced) {
-    hipLaunchKernelGGL(slice_products_kernel, dim3(p->slice_blocks), dim3(kBlock), 0, p->stream,
-                       p->Aj_s.as<int32_t>(), p->Ax_s.as<float>(), p->sbase.as<int64_t>(), d_x,
-                       p->prod.as<float>());
-    hipLaunchKernelGGL(slice_rowsum_kernel, dim3((p->rows + kBlock - 1) / kBlock), dim3(kBlock),
-                       0, p->stream, p->rows, p->soff.as<int32_t>(), p->sbase.as<int64_t>(),
-                       p->prod.as<float>(), d_y);
+    const Tile* tl = p->slice_tiles.as<Tile>();
+    for (int q = 0; q < kSlices; q++) {  // slice 0 first: each row's column order
+      const int nl = p->slice_long[q], nr = p->slice_rows[q];
+      if (nl + nr == 0) continue;
+      const Tile* lt = tl + p->slice_first[q];
+      const int32_t* off = p->soff.as<int32_t>() + (size_t)q * (p->rows + 1);
+      const int32_t* aj = p->Aj_s.as<int32_t>() + p->slice_base[q];
+      const float* ax = p->Ax_s.as<float>() + p->slice_base[q];
+      const int em = (int)std::max<int64_t>(p->slice_base[q + 1] - p->slice_base[q] - 1, 0);
+      if (nl)
+        hipLaunchKernelGGL(spmv_pull_kernel<true>, dim3(nl + nr), dim3(kBlock), 0, p->stream,
+                           lt + nl, lt, nl, off, aj, em, ax, d_x, d_y);
+      else
+        hipLaunchKernelGGL(spmv_pull_kernel<false>, dim3(nr), dim3(kBlock), 0, p->stream, lt, lt,
+                           0, off, aj, em, ax, d_x, d_y);
+    }
     GD_HIP_TRY(hipGetLastError());
     return GD_OK;
   }

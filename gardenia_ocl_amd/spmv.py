@@ -48,9 +48,9 @@ class SpmvPlan:
     MODES = {"auto": 0, "tiles": 1, "sliced": 2}
 
     def __init__(self, Ap, Aj, Ax, num_cols=None, stream=None, mode="tiles"):
-        """mode: "tiles" (the tile engine), "sliced" (XCD column slices, sorted
-        rows only), "auto" (sliced when x exceeds one XCD's L2 and rows are
-        short)."""
+        """mode: "tiles" (the tile engine), "sliced" (one tile-engine pass per
+        column slice of x sized for an XCD's L2, sorted rows only), "auto"
+        (sliced when x exceeds one XCD's L2 and rows are short)."""
         import torch
 
         for t, dt in ((Ap, torch.int32), (Aj, torch.int32), (Ax, torch.float32)):

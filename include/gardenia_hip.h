@@ -309,13 +309,13 @@ int gd_spmv_plan_destroy(gd_spmv_plan *plan);
 /* SpMV execution modes.
  *   GD_SPMV_TILES   the tile engine over the caller's CSR (any matrix, any
  *                   row order; bit-identical to the reference).
- *   GD_SPMV_SLICED  columns split into 8 slices, one per XCD: pass 1 forms the
- *                   products over a slice-major copy of the matrix (each XCD's
- *                   L2 holds one slice of x), pass 2 adds each row's products
- *                   to y in column order.  Needs sorted columns per row;
- *                   bit-identical to the reference for every row.  The copy
- *                   holds the values: call gd_spmv_plan_refresh_values after
- *                   changing d_Ax.
+ *   GD_SPMV_SLICED  columns split into 8 slices over a slice-major copy of the
+ *                   matrix; one tile-engine pass per slice, slice 0 first,
+ *                   adds each row's slice entries onto y in column order (the
+ *                   pass's x gathers stay in one slice, which every XCD's L2
+ *                   holds).  Needs sorted columns per row; bit-identical to
+ *                   the reference for every row.  The copy holds the values:
+ *                   call gd_spmv_plan_refresh_values after changing d_Ax.
  *   GD_SPMV_AUTO    SLICED when x spans more than one XCD's L2 (cols*4 > 4 MB),
  *                   every row has <= 64 nonzeros and the columns are sorted,
  *                   else TILES. */
