@@ -166,8 +166,9 @@ def test_spmv_sliced_er_bit_exact(golden, device):
 
 
 def test_spmv_sliced_long_rows_are_sequential(golden, device):
-    """Pass 2 sums every row in column order, so even rows longer than 64 are
-    bit-identical to the reference (RMAT-12 in-CSR, max row > 1000)."""
+    """Each slice pass adds a row's entries of that slice onto y in column
+    order, so rows longer than 64 are bit-identical to the reference too
+    (RMAT-12 in-CSR, max row > 1000)."""
     arrays, meta = golden
     _, ic = graph.rmat(12, 16, seed=1)
     assert ic.degree.max() > 64
@@ -211,7 +212,7 @@ def test_spmv_sliced_refresh_values_and_fallback(device):
 
 
 def test_spmv_sliced_wide_matrix_many_slices(device):
-    """2^16 rows over 2^24 columns (64 MB of x: 32 column slices of 2 MB),
+    """2^16 rows over 2^24 columns (64 MB of x in 8 column slices of 8 MB),
     rows of 0..64 sorted columns spread over the slices, some empty: every row
     is the oracle's sequential sum, y carried from slice to slice."""
     rng = np.random.default_rng(31)
@@ -230,6 +231,58 @@ def test_spmv_sliced_wide_matrix_many_slices(device):
     x = graph.uniform_pm1(cols, 33)
     y0 = graph.uniform_pm1(rows, 34)
     y, plan, _ = _gpu_spmv_mode(device, csr, Ax, x, y0, "auto", reps=2)
+    assert plan.mode == "sliced"
+    ref = y0
+    for _ in range(2):
+        ref = oracle.spmv(rows, csr.row_offsets, csr.column_indices, Ax, x, ref)
+    np.testing.assert_array_equal(y, ref)
+
+
+def _csr_from_rows(rows, cols, row_cols):
+    """CSR with the given sorted column lists (missing rows are empty)."""
+    lens = np.zeros(rows, np.int64)
+    for r, c in row_cols.items():
+        lens[r] = len(c)
+    off = np.zeros(rows + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    col = np.empty(int(off[-1]), np.int32)
+    for r, c in row_cols.items():
+        col[off[r]:off[r + 1]] = np.sort(np.asarray(c, np.int64))
+    assert col.size == 0 or col.max() < cols
+    return graph.Csr(rows, off.astype(np.int32), col, lens.astype(np.int32))
+
+
+@pytest.mark.parametrize("case", ["empty_slices", "long_in_every_slice"])
+def test_spmv_sliced_schedule_edge_cases(device, case):
+    """Forced sliced mode, 2^13 rows x 2^16 columns (8 slices of 8192
+    columns).  empty_slices: entries only in slices 0 and 5, one row of 5000
+    entries inside slice 0 and one of 6000 inside slice 5 (longer than a
+    2048-entry tile: the per-slice long-row chain), most rows empty.
+    long_in_every_slice: one row of 3000 entries in each of the 8 slices next
+    to short rows in every slice.  Every row is the oracle's sequential sum,
+    two runs in a row (y carried through every pass of both)."""
+    rng = np.random.default_rng(41)
+    rows, cols, width = 1 << 13, 1 << 16, 1 << 13
+    rc = {}
+    if case == "empty_slices":
+        rc[7] = rng.choice(width, 5000, replace=False)
+        rc[4000] = 5 * width + rng.choice(width, 6000, replace=False)
+        for r in rng.choice(rows, 300, replace=False):
+            if r in (7, 4000):
+                continue
+            q = (0, 5)[int(r) % 2]
+            rc[int(r)] = q * width + rng.choice(width, int(rng.integers(1, 40)), replace=False)
+    else:
+        rc[123] = np.concatenate([q * width + rng.choice(width, 3000, replace=False)
+                                  for q in range(8)])
+        for r in range(0, rows, 3):
+            if r != 123:
+                rc[r] = rng.choice(cols, int(rng.integers(1, 50)), replace=False)
+    csr = _csr_from_rows(rows, cols, rc)
+    Ax = graph.uniform_pm1(csr.column_indices.shape[0], 42)
+    x = graph.uniform_pm1(cols, 43)
+    y0 = graph.uniform_pm1(rows, 44)
+    y, plan, _ = _gpu_spmv_mode(device, csr, Ax, x, y0, "sliced", reps=2)
     assert plan.mode == "sliced"
     ref = y0
     for _ in range(2):
