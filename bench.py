@@ -812,7 +812,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (scales rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
-    ap.add_argument("--only", choices=["symgs", "csr_build", "spmv", "small", "ppr"], default=None,
+    ap.add_argument("--only", choices=["symgs", "csr_build", "spmv", "small", "ppr", "adjoint"],
+                    default=None,
                     help="run just this secondary figure and print its JSON (for profiling)")
     ap.add_argument("--tol-run", type=float, default=1e-6,
                     help="L1 tolerance of the extra run to convergence (with its FP64 L1)")
@@ -841,6 +842,9 @@ def main():
         return
     if args.only == "spmv":
         print(json.dumps({"spmv": bench_spmv(args.spmv_log2, 32, 50, args.seed)[0]}), flush=True)
+        return
+    if args.only == "adjoint":
+        print(json.dumps({"adjoint": bench_adjoint(20, args.spmv_log2, args.seed)}), flush=True)
         return
     if args.only == "csr_build":
         t0 = time.perf_counter()

@@ -29,6 +29,7 @@
 #include "gd_common.h"
 #include "iter_engine.h"
 #include "pull_engine.h"
+#include "spmv_internal.h"
 #include "transpose.h"
 
 namespace gd {
@@ -413,7 +414,32 @@ struct gd_spmv_bwd_plan {
   DeviceBuffer tiles;
   int n_tiles = 0;
   gd_spmv_plan* tplan = nullptr;  // SpMV over the CSC: dx = A^T g
+  // tplan in the column-sliced mode: its slice-major value copy is filled from
+  // A's values by vperm (two passes through buckets) instead of `at`
+  bool sliced = false;
+  BucketedPermutation vperm;  // A's values (CSR order) -> the CSC plan's slice-major copy
+  ~gd_spmv_bwd_plan() { gd_spmv_plan_destroy(tplan); }
 };
+
+namespace gd {
+namespace {
+
+// keys[k] = s * nb + (col[k] >> cshift) for the CSR entry k of row r, where
+// s = min(r / width, 7) is the column slice of the CSC plan holding the entry:
+// a bucket is one slice's entries in a range of 2^cshift columns of A.
+__global__ void bucket_keys_kernel(int rows, const int32_t* __restrict__ off,
+                                   const int32_t* __restrict__ col, int width, int cshift, int nb,
+                                   int32_t* __restrict__ keys) {
+  const int lane = threadIdx.x & 63;
+  for (int r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; r < rows;
+       r += gridDim.x * (blockDim.x / 64)) {
+    const int base = min(r / width, 7) * nb;
+    for (int e = off[r] + lane; e < off[r + 1]; e += 64) keys[e] = base + (col[e] >> cshift);
+  }
+}
+
+}  // namespace
+}  // namespace gd
 
 extern "C" {
 
@@ -453,9 +479,48 @@ int gd_spmv_bwd_plan_create(int rows, int cols, int64_t nnz, const int32_t* d_Ap
   }
   rc = build_transpose(rows, cols, nnz, d_Ap, d_Aj, p->stream, &p->tr);
   if (!rc) rc = p->at.alloc(4 * (size_t)std::max<int64_t>(nnz, 1));
-  if (!rc && cols > 0)
-    rc = gd_spmv_plan_create(cols, rows, nnz, p->tr.off.as<int32_t>(), p->tr.row.as<int32_t>(),
-                             p->at.as<float>(), stream, &p->tplan);
+  if (rc) return fail(rc);
+  GD_HIP_TRY(hipMemsetAsync(p->at.ptr, 0, 4 * (size_t)std::max<int64_t>(nnz, 1), p->stream));
+  if (cols > 0) {
+    // g larger than one XCD's L2: the column-sliced mode (the CSC's rows are
+    // sorted; a matrix with repeated entries in a row is not, and keeps the
+    // tile mode)
+    if (nnz > 0 && (int64_t)rows * 4 > (4 << 20)) {
+      rc = gd_spmv_plan_create_ex(cols, rows, nnz, p->tr.off.as<int32_t>(),
+                                  p->tr.row.as<int32_t>(), p->at.as<float>(), stream,
+                                  GD_SPMV_SLICED, &p->tplan);
+      if (rc == GD_ERR_INVALID) rc = GD_OK;  // unsorted: p->tplan stays NULL
+      if (rc) return fail(rc);
+    }
+    if (p->tplan) {
+      // The values go into the CSC plan's slice-major copy through buckets of
+      // about 2^18 entries (a slice's entries in a range of A's columns): the
+      // slice-major order reads the buckets one after another.
+      p->sliced = true;
+      DeviceBuffer perm_s, keys;
+      rc = perm_s.alloc(4 * (size_t)nnz);
+      if (!rc) rc = keys.alloc(4 * (size_t)nnz);
+      if (!rc) rc = spmv_slice_order(p->tplan, p->tr.perm.as<int32_t>(), perm_s.as<int32_t>());
+      const int64_t nb_want = std::max<int64_t>(1, nnz / (8ll << 18));
+      int cshift = 0;
+      while (((int64_t)cols >> cshift) > nb_want) cshift++;
+      const int nb = (int)(((int64_t)cols + (1ll << cshift) - 1) >> cshift);
+      int key_bits = 1;
+      while ((1ll << key_bits) < 8ll * nb) key_bits++;
+      if (!rc) {
+        const int width = std::max(1, (rows + 7) / 8);  // the CSC plan's slice width
+        hipLaunchKernelGGL(bucket_keys_kernel, dim3(grid_n(rows / 4 + 1)), dim3(256), 0,
+                           p->stream, rows, d_Ap, d_Aj, width, cshift, nb, keys.as<int32_t>());
+        if (hipGetLastError() != hipSuccess) rc = GD_ERR_DEVICE;
+      }
+      if (!rc)
+        rc = build_bucketed_permutation(nnz, keys.as<int32_t>(), key_bits, perm_s.as<int32_t>(),
+                                        p->stream, &p->vperm);
+    } else {
+      rc = gd_spmv_plan_create(cols, rows, nnz, p->tr.off.as<int32_t>(), p->tr.row.as<int32_t>(),
+                               p->at.as<float>(), stream, &p->tplan);
+    }
+  }
   if (rc) return fail(rc);
   *out = owner.release();
   return GD_OK;
@@ -467,7 +532,9 @@ int gd_spmv_bwd_run(gd_spmv_bwd_plan* p, const float* d_Ax, const float* d_x, co
   GD_CHECK_ARG(p->rows == 0 || d_g, "NULL g");
   if (d_dx && p->cols > 0) {
     GD_CHECK_ARG(p->nnz == 0 || d_Ax, "NULL Ax");
-    int rc = transpose_values(p->tr, p->nnz, d_Ax, p->at.as<float>(), p->stream);
+    int rc = p->sliced ? run_bucketed_permutation(p->vperm, d_Ax, spmv_sliced_values(p->tplan),
+                                                  p->stream)
+                       : transpose_values(p->tr, p->nnz, d_Ax, p->at.as<float>(), p->stream);
     if (rc) return rc;
     hipLaunchKernelGGL(fill_kernel, dim3(grid_n(p->cols)), dim3(256), 0, p->stream, p->cols,
                        d_dx, 0.0f);
@@ -485,9 +552,7 @@ int gd_spmv_bwd_run(gd_spmv_bwd_plan* p, const float* d_Ax, const float* d_x, co
 }
 
 int gd_spmv_bwd_plan_destroy(gd_spmv_bwd_plan* p) {
-  if (!p) return GD_OK;
-  gd_spmv_plan_destroy(p->tplan);
-  delete p;
+  delete p;  // the destructor frees the CSC plan
   return GD_OK;
 }
 

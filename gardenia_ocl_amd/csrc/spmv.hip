@@ -19,6 +19,7 @@
 #include "gardenia_hip.h"
 #include "gd_common.h"
 #include "pull_engine.h"
+#include "spmv_internal.h"
 #include <rocprim/device/device_scan.hpp>
 
 namespace gd {
@@ -108,12 +109,15 @@ __global__ void slice_count_kernel(int rows, const int32_t* __restrict__ Ap,
   }
 }
 
-// Scatter each row's entries into the slice-major arrays (row order kept).
+// Scatter each row's entries into the slice-major arrays (row order kept):
+// the column indices unless values_only, and a 4-byte payload per entry (the
+// values, or any other per-entry array).
+template <class T>
 __global__ void slice_scatter_kernel(int rows, const int32_t* __restrict__ Ap,
-                                     const int32_t* __restrict__ Aj, const float* __restrict__ Ax,
+                                     const int32_t* __restrict__ Aj, const T* __restrict__ Ax,
                                      int width, const int32_t* __restrict__ soff,
                                      const int64_t* __restrict__ sbase, int32_t* __restrict__ Aj_s,
-                                     float* __restrict__ Ax_s, int values_only) {
+                                     T* __restrict__ Ax_s, int values_only) {
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
     int64_t pos[kSlices];
 #pragma unroll
@@ -205,8 +209,8 @@ int build_sliced(gd_spmv_plan* p) {
   }
   GD_HIP_TRY(hipMemcpyAsync(p->sbase.ptr, base.data(), 8 * (kSlices + 1), hipMemcpyHostToDevice,
                             p->stream));
-  hipLaunchKernelGGL(slice_scatter_kernel, dim3(g), dim3(256), 0, p->stream, rows, p->Ap, p->Aj,
-                     p->Ax, p->width, p->soff.as<int32_t>(), p->sbase.as<int64_t>(),
+  hipLaunchKernelGGL(slice_scatter_kernel<float>, dim3(g), dim3(256), 0, p->stream, rows, p->Ap,
+                     p->Aj, p->Ax, p->width, p->soff.as<int32_t>(), p->sbase.as<int64_t>(),
                      p->Aj_s.as<int32_t>(), p->Ax_s.as<float>(), 0);
   GD_HIP_TRY(hipGetLastError());
   // one row-tile schedule per slice, over that slice's row offsets
@@ -375,12 +379,32 @@ int gd_spmv_plan_refresh_values(gd_spmv_plan* p) {
   GD_CHECK_ARG(p != nullptr, "NULL plan");
   if (!p->sliced) return GD_OK;  // the tile engine reads the caller's values directly
   const int g = std::max(1, std::min((p->rows + 255) / 256, 16384));
-  hipLaunchKernelGGL(slice_scatter_kernel, dim3(g), dim3(256), 0, p->stream, p->rows, p->Ap,
+  hipLaunchKernelGGL(slice_scatter_kernel<float>, dim3(g), dim3(256), 0, p->stream, p->rows, p->Ap,
                      p->Aj, p->Ax, p->width, p->soff.as<int32_t>(), p->sbase.as<int64_t>(),
                      p->Aj_s.as<int32_t>(), p->Ax_s.as<float>(), 1);
   GD_HIP_TRY(hipGetLastError());
   return GD_OK;
 }
+
+}  // extern "C"
+
+float* gd::spmv_sliced_values(gd_spmv_plan* p) {
+  return (p && p->sliced) ? p->Ax_s.as<float>() : nullptr;
+}
+
+int gd::spmv_slice_order(gd_spmv_plan* p, const int32_t* d_in, int32_t* d_out) {
+  GD_CHECK_ARG(p != nullptr && p->sliced, "not a sliced SpMV plan");
+  if (p->nnz == 0) return GD_OK;
+  GD_CHECK_ARG(d_in && d_out, "NULL payload");
+  const int g = std::max(1, std::min((p->rows + 255) / 256, 16384));
+  hipLaunchKernelGGL(slice_scatter_kernel<int32_t>, dim3(g), dim3(256), 0, p->stream, p->rows,
+                     p->Ap, p->Aj, d_in, p->width, p->soff.as<int32_t>(), p->sbase.as<int64_t>(),
+                     nullptr, d_out, 1);
+  GD_HIP_TRY(hipGetLastError());
+  return GD_OK;
+}
+
+extern "C" {
 
 int gd_spmv_plan_mode(const gd_spmv_plan* p) {
   return p == nullptr ? -1 : (p->sliced ? GD_SPMV_SLICED : GD_SPMV_TILES);

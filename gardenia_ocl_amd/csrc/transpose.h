@@ -22,6 +22,33 @@ int build_transpose(int rows, int cols, int64_t nnz, const int32_t* d_off, const
 int transpose_values(const Transpose& t, int64_t nnz, const float* csr_vals, float* csc_vals,
                      hipStream_t stream);
 
+// dst[i] = src[perm[i]] for i < nnz  (async on stream)
+int permute_values(int64_t nnz, const int32_t* perm, const float* src, float* dst,
+                   hipStream_t stream);
+
+// dst[i] = src[perm[i]] in two passes through a bucket buffer: pass 1
+// scatters src in its own order into buckets (source entry k to bucket
+// keys[k], in source order within a bucket), pass 2 gathers dst from the
+// bucket buffer.  Where the destination order reads the buckets one after
+// another, the gathers of pass 2 stay within about one bucket per XCD (hits
+// in its L2), while a direct random gather over a large src pulls a cache
+// line from memory per element.
+struct BucketedPermutation {
+  int64_t n = 0;
+  DeviceBuffer pos;         // [n] bucket-buffer position of source entry k (build time only)
+  DeviceBuffer lidx;        // [n] uint16: per 2048-entry chunk, its entries in position order
+  DeviceBuffer pos_sorted;  // [n] the bucket-buffer position of each of them
+  DeviceBuffer perm;        // [n] bucket-buffer position read for destination entry i
+  DeviceBuffer buf;         // [n] the bucket buffer (values)
+};
+
+// keys[k] < 2^key_bits.  Synchronises `stream`.
+int build_bucketed_permutation(int64_t n, const int32_t* d_keys, int key_bits,
+                               const int32_t* d_perm, hipStream_t stream, BucketedPermutation* b);
+// dst[i] = src[perm[i]]  (async on stream)
+int run_bucketed_permutation(const BucketedPermutation& b, const float* src, float* dst,
+                             hipStream_t stream);
+
 }  // namespace gd
 
 #endif  // GD_TRANSPOSE_H_

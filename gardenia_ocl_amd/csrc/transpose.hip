@@ -102,18 +102,190 @@ int build_transpose(int rows, int cols, int64_t nnz, const int32_t* d_off, const
   return GD_OK;
 }
 
-__global__ void permute_values_kernel(int64_t nnz, const int32_t* __restrict__ perm,
-                                      const float* __restrict__ src, float* __restrict__ dst) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
-       i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = src[perm[i]];
+// Chunks of kPermChunk positions, kPermU gathers in flight per thread.
+// Workgroups are dispatched to the 8 XCDs round-robin; workgroup b takes
+// chunk xcd_chunk(b), so XCD x walks the x-th eighth of the chunks in order
+// and the positions in flight on one XCD stay in one narrow window (of the
+// destination for a gather, of the source for a scatter).  The grid is a
+// multiple of 8 (G = 8 * ceil(chunks / 8)); chunks past the end are skipped.
+constexpr int kXcds = 8;
+constexpr int kPermU = 8;
+constexpr int kPermChunk = 256 * kPermU;
+
+__device__ __forceinline__ int64_t xcd_chunk() {
+  const int b = blockIdx.x, per_xcd = gridDim.x / kXcds;
+  return (int64_t)(b % kXcds) * per_xcd + b / kXcds;
+}
+
+static int xcd_grid(int64_t n) {
+  const int64_t chunks = (n + kPermChunk - 1) / kPermChunk;
+  return (int)(kXcds * ((chunks + kXcds - 1) / kXcds));
+}
+
+// dst[i] = src[perm[i]]
+template <class T>
+__global__ __launch_bounds__(256) void permute_kernel(int64_t n, const int32_t* __restrict__ perm,
+                                                      const T* __restrict__ src,
+                                                      T* __restrict__ dst) {
+  const int64_t c = xcd_chunk() * kPermChunk;
+  if (c >= n) return;
+  const int64_t last = n - 1;
+  int p[kPermU];
+  T v[kPermU];
+#pragma unroll
+  for (int q = 0; q < kPermU; q++) p[q] = perm[min(c + q * 256 + threadIdx.x, last)];
+#pragma unroll
+  for (int q = 0; q < kPermU; q++) v[q] = src[p[q]];
+#pragma unroll
+  for (int q = 0; q < kPermU; q++)
+    if (c + q * 256 + threadIdx.x <= last) dst[c + q * 256 + threadIdx.x] = v[q];
+}
+
+// Chunk-local scatter: the chunk's source values are staged in LDS, then
+// written in the order of their destinations (lidx[t]: the chunk-local source
+// index of the chunk's t-th destination in increasing order, pos_sorted[t]:
+// that destination), so consecutive lanes store to consecutive addresses
+// wherever a chunk sends a run of entries to one bucket.
+__global__ __launch_bounds__(256) void scatter_sorted_kernel(int64_t n,
+                                                             const uint16_t* __restrict__ lidx,
+                                                             const int32_t* __restrict__ pos_sorted,
+                                                             const float* __restrict__ src,
+                                                             float* __restrict__ dst) {
+  __shared__ float s_v[kPermChunk];
+  const int64_t c = xcd_chunk() * kPermChunk;
+  if (c >= n) return;
+#pragma unroll
+  for (int q = 0; q < kPermU; q++) {
+    const int64_t k = c + q * 256 + threadIdx.x;
+    if (k < n) s_v[q * 256 + threadIdx.x] = src[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kPermU; q++) {
+    const int64_t t = c + q * 256 + threadIdx.x;
+    if (t < n) dst[pos_sorted[t]] = s_v[lidx[t]];
+  }
+}
+
+// keys2[k] = (chunk of k) << key_bits | keys[k]
+__global__ void chunk_keys_kernel(int64_t n, const int32_t* __restrict__ keys, int key_bits,
+                                  int32_t* __restrict__ keys2) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    keys2[k] = (int32_t)(((k / kPermChunk) << key_bits) | keys[k]);
+}
+
+// lidx[t] = order[t] - chunk start, pos_sorted[t] = pos[order[t]]
+__global__ void chunk_order_kernel(int64_t n, const int32_t* __restrict__ order,
+                                   const int32_t* __restrict__ pos, uint16_t* __restrict__ lidx,
+                                   int32_t* __restrict__ pos_sorted) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t k = order[t];
+    lidx[t] = (uint16_t)(k % kPermChunk);
+    pos_sorted[t] = pos[k];
+  }
+}
+
+// pos[order[t]] = t
+__global__ void invert_kernel(int64_t n, const int32_t* __restrict__ order,
+                              int32_t* __restrict__ pos) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    pos[order[t]] = (int32_t)t;
 }
 
 int transpose_values(const Transpose& t, int64_t nnz, const float* csr_vals, float* csc_vals,
                      hipStream_t stream) {
   if (nnz == 0) return GD_OK;
-  hipLaunchKernelGGL(permute_values_kernel, dim3(grid64(nnz)), dim3(256), 0, stream, nnz,
-                     t.perm.as<int32_t>(), csr_vals, csc_vals);
+  return permute_values(nnz, t.perm.as<int32_t>(), csr_vals, csc_vals, stream);
+}
+
+int permute_values(int64_t nnz, const int32_t* perm, const float* src, float* dst,
+                   hipStream_t stream) {
+  if (nnz == 0) return GD_OK;
+  hipLaunchKernelGGL(permute_kernel<float>, dim3(xcd_grid(nnz)), dim3(256), 0, stream, nnz, perm,
+                     src, dst);
+  GD_HIP_TRY(hipGetLastError());
+  return GD_OK;
+}
+
+int build_bucketed_permutation(int64_t n, const int32_t* d_keys, int key_bits,
+                               const int32_t* d_perm, hipStream_t stream, BucketedPermutation* b) {
+  b->n = n;
+  int rc = b->pos.alloc(4 * (size_t)std::max<int64_t>(n, 1));
+  if (!rc) rc = b->perm.alloc(4 * (size_t)std::max<int64_t>(n, 1));
+  if (!rc) rc = b->buf.alloc(4 * (size_t)std::max<int64_t>(n, 1));
+  if (rc || n == 0) return rc;
+  DeviceBuffer ids, order, keys_out, tmp;
+  rc = ids.alloc(4 * (size_t)n);
+  if (!rc) rc = order.alloc(4 * (size_t)n);
+  if (!rc) rc = keys_out.alloc(4 * (size_t)n);
+  if (rc) return rc;
+  hipLaunchKernelGGL(iota64_kernel, dim3(grid64(n)), dim3(256), 0, stream, ids.as<int32_t>(), n);
+  GD_HIP_TRY(hipGetLastError());
+  // stable: the entries of one bucket keep their source order
+  size_t bytes = 0;
+  GD_HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, d_keys, keys_out.as<int32_t>(),
+                                       ids.as<int32_t>(), order.as<int32_t>(), (size_t)n, 0,
+                                       key_bits, stream));
+  rc = tmp.alloc(bytes);
+  if (rc) return rc;
+  GD_HIP_TRY(rocprim::radix_sort_pairs(tmp.ptr, bytes, d_keys, keys_out.as<int32_t>(),
+                                       ids.as<int32_t>(), order.as<int32_t>(), (size_t)n, 0,
+                                       key_bits, stream));
+  hipLaunchKernelGGL(invert_kernel, dim3(grid64(n)), dim3(256), 0, stream, n,
+                     order.as<int32_t>(), b->pos.as<int32_t>());
+  hipLaunchKernelGGL(permute_kernel<int32_t>, dim3(xcd_grid(n)), dim3(256), 0, stream, n, d_perm,
+                     b->pos.as<int32_t>(), b->perm.as<int32_t>());
+  GD_HIP_TRY(hipGetLastError());
+  // each chunk's entries in the order of their bucket-buffer positions: a
+  // stable sort by (chunk, bucket) (the positions increase with the source
+  // index inside a bucket); chunk < 2^(31 - key_bits) for n < 2^31
+  int chunk_bits = 1;
+  while ((1ll << chunk_bits) < (n + kPermChunk - 1) / kPermChunk) chunk_bits++;
+  if (chunk_bits + key_bits > 31) {
+    set_error("bucketed permutation: too many chunks x buckets");
+    return GD_ERR_INVALID;
+  }
+  rc = b->lidx.alloc(2 * (size_t)n);
+  if (!rc) rc = b->pos_sorted.alloc(4 * (size_t)n);
+  if (rc) return rc;
+  hipLaunchKernelGGL(chunk_keys_kernel, dim3(grid64(n)), dim3(256), 0, stream, n, d_keys,
+                     key_bits, keys_out.as<int32_t>());
+  GD_HIP_TRY(hipGetLastError());
+  DeviceBuffer keys_sorted;
+  rc = keys_sorted.alloc(4 * (size_t)n);
+  if (rc) return rc;
+  size_t bytes2 = 0;
+  GD_HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes2, keys_out.as<int32_t>(),
+                                       keys_sorted.as<int32_t>(), ids.as<int32_t>(),
+                                       order.as<int32_t>(), (size_t)n, 0,
+                                       chunk_bits + key_bits, stream));
+  if (bytes2 > bytes) {
+    rc = tmp.alloc(bytes2);  // frees the smaller one first
+    if (rc) return rc;
+  }
+  GD_HIP_TRY(rocprim::radix_sort_pairs(tmp.ptr, bytes2, keys_out.as<int32_t>(),
+                                       keys_sorted.as<int32_t>(), ids.as<int32_t>(),
+                                       order.as<int32_t>(), (size_t)n, 0,
+                                       chunk_bits + key_bits, stream));
+  hipLaunchKernelGGL(chunk_order_kernel, dim3(grid64(n)), dim3(256), 0, stream, n,
+                     order.as<int32_t>(), b->pos.as<int32_t>(), b->lidx.as<uint16_t>(),
+                     b->pos_sorted.as<int32_t>());
+  GD_HIP_TRY(hipGetLastError());
+  GD_HIP_TRY(hipStreamSynchronize(stream));  // the temporaries are freed on return
+  b->pos.release();  // only the chunk-sorted form is used from here on
+  return GD_OK;
+}
+
+int run_bucketed_permutation(const BucketedPermutation& b, const float* src, float* dst,
+                             hipStream_t stream) {
+  if (b.n == 0) return GD_OK;
+  hipLaunchKernelGGL(scatter_sorted_kernel, dim3(xcd_grid(b.n)), dim3(256), 0, stream, b.n,
+                     b.lidx.as<uint16_t>(), b.pos_sorted.as<int32_t>(), src, b.buf.as<float>());
+  hipLaunchKernelGGL(permute_kernel<float>, dim3(xcd_grid(b.n)), dim3(256), 0, stream, b.n,
+                     b.perm.as<int32_t>(), b.buf.as<float>(), dst);
   GD_HIP_TRY(hipGetLastError());
   return GD_OK;
 }

@@ -216,3 +216,61 @@ def test_spmv_backward_er22(device):
     np.testing.assert_array_equal(dA.cpu().numpy(), gv[rows] * x[csr.column_indices])
     dx64, _ = oracle.spmv_bwd_fp64(m, m, csr.row_offsets, csr.column_indices, csr.weight, x, gv)
     assert np.abs(dx.cpu().numpy() - dx64).max() <= 1e-5 * max(1.0, np.abs(dx64).max())
+
+
+def _host_csc_dx(rows, cols, off, col, vals, g):
+    """dx = A^T g summed per column in increasing row order (a stable
+    transpose of the CSR, entries of one row in CSR order): the oracle's
+    sequential FP32 loop over that CSC."""
+    order = np.argsort(col, kind="stable")
+    row_of = np.repeat(np.arange(rows, dtype=np.int32), np.diff(off))
+    csc_off = np.zeros(cols + 1, np.int32)
+    csc_off[1:] = np.cumsum(np.bincount(col, minlength=cols))
+    return oracle.spmv(cols, csc_off, np.ascontiguousarray(row_of[order]),
+                       np.ascontiguousarray(vals[order]), g, np.zeros(cols, np.float32))
+
+
+@pytest.mark.parametrize("kind", ["square_er21", "rect", "dups"])
+def test_spmv_backward_dx_bit_exact(device, kind):
+    """dx = A^T g from the backward plan equals the sequential sum over the
+    stable CSC bit for bit.  square_er21 and rect (2^21 x 2^18, 16 per row)
+    have g of 8 MB, so dx runs in the column-sliced mode with the values
+    permuted straight into its slice-major copy; dups (repeated entries in
+    rows, so CSC rows are not strictly sorted) falls back to the tile mode.
+    dA = g[row] * x[col], one FP32 product, in every case."""
+    import torch
+
+    from gardenia_ocl_amd.autograd import SpmvBackwardPlan
+
+    rows = 1 << 21
+    if kind == "square_er21":
+        csr = graph.er_matrix(21, 32, seed=8)
+        cols, off, col, vals = rows, csr.row_offsets, csr.column_indices, csr.weight
+    else:
+        rng = np.random.default_rng(9)
+        cols = (1 << 18) if kind == "rect" else rows
+        n = rows * 16
+        src = rng.integers(0, rows, n).astype(np.int32)
+        dst = rng.integers(0, cols, n).astype(np.int32)
+        if kind == "dups":
+            dst[1::2] = dst[0::2]  # every other entry repeats its neighbour's column
+            src[1::2] = src[0::2]
+        m = max(rows, cols)
+        c = graph.csr_from_edges(m, src, dst, sorted=True, remove_selfloops=False,
+                                 remove_redundents=(kind != "dups"))
+        off = np.ascontiguousarray(c.row_offsets[:rows + 1])
+        col = np.ascontiguousarray(c.column_indices[:int(off[-1])])
+        assert col.max() < cols
+        vals = graph.uniform_pm1(col.size, 10)
+    x = graph.uniform_pm1(cols, 11)
+    gv = graph.uniform_pm1(rows, 12)
+    t = lambda v: torch.from_numpy(np.ascontiguousarray(v)).to(device)  # noqa: E731
+    bwd = SpmvBackwardPlan(t(off), t(col), num_cols=cols)
+    dx = torch.empty(cols, dtype=torch.float32, device=device)
+    dA = torch.empty(col.size, dtype=torch.float32, device=device)
+    for _ in range(2):  # the second run overwrites the first's results
+        bwd.run(t(vals), t(x), t(gv), dx, dA)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(dx.cpu().numpy(), _host_csc_dx(rows, cols, off, col, vals, gv))
+    row_of = np.repeat(np.arange(rows), np.diff(off))
+    np.testing.assert_array_equal(dA.cpu().numpy(), gv[row_of] * x[col])
