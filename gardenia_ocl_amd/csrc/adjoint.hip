@@ -273,15 +273,8 @@ int gd_wpr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* 
   auto fail = [](int rc) { return rc; };  // owner frees the plan
   int rc = check_columns(d_col, nnz, m, p->stream);
   if (rc) return fail(rc);
-  {
-    std::vector<int32_t> h_off(m + 1);
-    GD_HIP_TRY(hipMemcpyAsync(h_off.data(), d_off, 4 * ((size_t)m + 1), hipMemcpyDeviceToHost,
-                              p->stream));
-    GD_HIP_TRY(hipStreamSynchronize(p->stream));
-    Schedule tmp;
-    rc = build_schedule(m, nnz, h_off.data(), kBlock * 8, kTileRows, &tmp);
-    if (rc) return fail(rc);
-  }
+  rc = check_offsets(d_off, m, nnz, p->stream);  // before any device walk
+  if (rc) return fail(rc);
   rc = build_transpose(m, m, nnz, d_off, d_col, p->stream, &p->tr);
   if (!rc) rc = p->at.alloc(4 * (size_t)std::max<int64_t>(nnz, 1));
   if (!rc) rc = p->s.alloc(4 * (size_t)m);

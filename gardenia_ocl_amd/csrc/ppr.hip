@@ -470,15 +470,9 @@ int ppr_build(gd_ppr_plan* p, const int32_t* d_off, const int32_t* d_col,
   const int m = p->m;
   int rc = check_columns(d_col, p->nnz, m, p->stream);
   if (rc) return rc;
+  rc = check_offsets(d_off, m, p->nnz, p->stream);  // before any device walk
+  if (rc) return rc;
   std::vector<int32_t> h_off(m + 1);
-  GD_HIP_TRY(hipMemcpyAsync(h_off.data(), d_off, 4 * ((size_t)m + 1), hipMemcpyDeviceToHost,
-                            p->stream));
-  GD_HIP_TRY(hipStreamSynchronize(p->stream));
-  {
-    Schedule tmp;  // validates the offsets before any device walk
-    rc = build_schedule(m, p->nnz, h_off.data(), kSegEdges, m, &tmp);
-    if (rc) return rc;
-  }
   p->off = d_off;
   p->col = d_col;
   p->outdeg = d_outdeg;
@@ -488,10 +482,10 @@ int ppr_build(gd_ppr_plan* p, const int32_t* d_off, const int32_t* d_col,
     p->off = p->rl.off.as<int32_t>();
     p->col = p->rl.col.as<int32_t>();
     p->outdeg = p->rl.outdeg.as<int32_t>();
-    GD_HIP_TRY(hipMemcpyAsync(h_off.data(), p->off, 4 * ((size_t)m + 1), hipMemcpyDeviceToHost,
-                              p->stream));
-    GD_HIP_TRY(hipStreamSynchronize(p->stream));
   }
+  GD_HIP_TRY(hipMemcpyAsync(h_off.data(), p->off, 4 * ((size_t)m + 1), hipMemcpyDeviceToHost,
+                            p->stream));
+  GD_HIP_TRY(hipStreamSynchronize(p->stream));
   // wavefront tiles: <= kTileEdges edges / kTileRowsP rows; rows > kSegEdges are
   // hubs (ppr_hub_chain_kernel), longest first
   Schedule& s = p->sched;

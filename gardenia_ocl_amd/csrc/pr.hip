@@ -63,16 +63,9 @@ int build_plan(gd_pr_plan* p, const int32_t* d_off, const int32_t* d_col,
   const int m = p->m;
   int rc = check_columns(d_col, p->nnz, m, p->stream);
   if (rc) return rc;
-  {
-    // offsets must be valid before any device walk (relabel, schedule)
-    std::vector<int32_t> h_off(m + 1);
-    GD_HIP_TRY(hipMemcpyAsync(h_off.data(), d_off, 4 * ((size_t)m + 1), hipMemcpyDeviceToHost,
-                              p->stream));
-    GD_HIP_TRY(hipStreamSynchronize(p->stream));
-    Schedule tmp;
-    rc = build_schedule(m, p->nnz, h_off.data(), kBlock * p->eng.ipt, kTileRows, &tmp);
-    if (rc) return rc;
-  }
+  // offsets must be valid before any device walk (relabel, schedule)
+  rc = check_offsets(d_off, m, p->nnz, p->stream);
+  if (rc) return rc;
   const int32_t* off = d_off;
   const int32_t* col = d_col;
   p->outdeg = d_outdeg;

@@ -2,6 +2,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "gd_common.h"
 
@@ -30,6 +32,64 @@ void DeviceBuffer::release() {
   bytes = 0;
 }
 
+namespace {
+
+// The greedy tiling of rows [r0, r1): hub slots (and hub indices) numbered
+// from 0 within the range; bad_row = first row with a negative length.
+struct SchedulePart {
+  std::vector<Tile> tiles;
+  std::vector<int32_t> hub_row, hub_nseg;
+  int32_t n_slots = 0, max_row_len = 0, bad_row = -1;
+};
+
+void build_part(int32_t r0, int32_t r1, const int32_t* off, int tile_nnz, int tile_rows,
+                int exact_len, SchedulePart* o) {
+  int32_t cur = r0, cur_nnz = 0;
+  for (int32_t r = r0; r < r1; r++) {
+    const int32_t d = off[r + 1] - off[r];
+    if (d < 0) {
+      o->bad_row = r;
+      return;
+    }
+    if (d > o->max_row_len) o->max_row_len = d;
+    if (exact_len > 0 && d > exact_len) {  // whole-row hub tile
+      if (r > cur) o->tiles.push_back(Tile{cur, r, off[cur], off[r]});
+      const int32_t h = (int32_t)o->hub_row.size();
+      o->hub_row.push_back(r);
+      o->hub_nseg.push_back(1);
+      o->tiles.push_back(Tile{r, -(h + 1), off[r], off[r + 1]});
+      o->n_slots += 1;
+      cur = r + 1;
+      cur_nnz = 0;
+      continue;
+    }
+    if (d > tile_nnz) {
+      if (r > cur) o->tiles.push_back(Tile{cur, r, off[cur], off[r]});
+      const int32_t nseg = (d + tile_nnz - 1) / tile_nnz;
+      o->hub_row.push_back(r);
+      o->hub_nseg.push_back(nseg);
+      for (int32_t k = 0; k < nseg; k++) {
+        const int32_t e0 = off[r] + k * tile_nnz;
+        const int32_t e1 = (k + 1 == nseg) ? off[r + 1] : e0 + tile_nnz;
+        o->tiles.push_back(Tile{r, -(o->n_slots + k + 1), e0, e1});
+      }
+      o->n_slots += nseg;
+      cur = r + 1;
+      cur_nnz = 0;
+      continue;
+    }
+    if (r > cur && (cur_nnz + d > tile_nnz || r - cur >= tile_rows)) {
+      o->tiles.push_back(Tile{cur, r, off[cur], off[r]});
+      cur = r;
+      cur_nnz = 0;
+    }
+    cur_nnz += d;
+  }
+  if (r1 > cur) o->tiles.push_back(Tile{cur, r1, off[cur], off[r1]});
+}
+
+}  // namespace
+
 int build_schedule(int m, int64_t nnz, const int32_t* off, int tile_nnz, int tile_rows,
                    Schedule* s, int exact_len) {
   s->tiles.clear();
@@ -47,51 +107,51 @@ int build_schedule(int m, int64_t nnz, const int32_t* off, int tile_nnz, int til
     set_error("row_offsets[0] must be 0 and row_offsets[m] must equal nnz");
     return GD_ERR_INVALID;
   }
-  s->tiles.reserve((size_t)(nnz / tile_nnz + m / tile_rows + 16));
-  int32_t cur = 0, cur_nnz = 0;
-  for (int32_t r = 0; r < m; r++) {
-    const int32_t d = off[r + 1] - off[r];
-    if (d < 0) {
-      set_error("row_offsets must be non-decreasing (row " + std::to_string(r) + ")");
+  // Large graphs: kParts fixed row ranges, each tiled greedily on its own (a
+  // tile never spans two ranges), built by up to 16 threads and joined in
+  // range order.  The ranges depend on m only, so the schedule does not
+  // depend on the thread count.
+  constexpr int kParts = 64;
+  const int parts = m >= (1 << 20) ? kParts : 1;
+  std::vector<SchedulePart> part(parts);
+  auto range = [&](int p) { return (int32_t)((int64_t)m * p / parts); };
+  if (parts == 1) {
+    build_part(0, m, off, tile_nnz, tile_rows, exact_len, &part[0]);
+  } else {
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+      pool.emplace_back([&, t] {
+        for (int p = t; p < parts; p += nt)
+          build_part(range(p), range(p + 1), off, tile_nnz, tile_rows, exact_len, &part[p]);
+      });
+    for (std::thread& th : pool) th.join();
+  }
+  size_t n_tiles = 0;
+  for (const SchedulePart& pt : part) {
+    if (pt.bad_row >= 0) {
+      set_error("row_offsets must be non-decreasing (row " + std::to_string(pt.bad_row) + ")");
       return GD_ERR_INVALID;
     }
-    if (d > s->max_row_len) s->max_row_len = d;
-    if (exact_len > 0 && d > exact_len) {  // whole-row hub tile
-      if (r > cur) s->tiles.push_back(Tile{cur, r, off[cur], off[r]});
-      const int32_t h = (int32_t)s->hub_row.size();
-      s->hub_row.push_back(r);
-      s->hub_slot.push_back(h);
-      s->hub_nseg.push_back(1);
-      s->tiles.push_back(Tile{r, -(h + 1), off[r], off[r + 1]});
-      s->n_slots += 1;
-      cur = r + 1;
-      cur_nnz = 0;
-      continue;
-    }
-    if (d > tile_nnz) {
-      if (r > cur) s->tiles.push_back(Tile{cur, r, off[cur], off[r]});
-      const int32_t nseg = (d + tile_nnz - 1) / tile_nnz;
-      s->hub_row.push_back(r);
-      s->hub_slot.push_back(s->n_slots);
-      s->hub_nseg.push_back(nseg);
-      for (int32_t k = 0; k < nseg; k++) {
-        const int32_t e0 = off[r] + k * tile_nnz;
-        const int32_t e1 = (k + 1 == nseg) ? off[r + 1] : e0 + tile_nnz;
-        s->tiles.push_back(Tile{r, -(s->n_slots + k + 1), e0, e1});
-      }
-      s->n_slots += nseg;
-      cur = r + 1;
-      cur_nnz = 0;
-      continue;
-    }
-    if (r > cur && (cur_nnz + d > tile_nnz || r - cur >= tile_rows)) {
-      s->tiles.push_back(Tile{cur, r, off[cur], off[r]});
-      cur = r;
-      cur_nnz = 0;
-    }
-    cur_nnz += d;
+    n_tiles += pt.tiles.size();
   }
-  if (m > cur) s->tiles.push_back(Tile{cur, m, off[cur], off[m]});
+  s->tiles.reserve(n_tiles);
+  for (const SchedulePart& pt : part) {
+    const int32_t base = s->n_slots;  // slots (and hub indices) of the earlier ranges
+    for (Tile t : pt.tiles) {
+      if (t.y < 0) t.y -= base;
+      s->tiles.push_back(t);
+    }
+    int32_t slot = base;
+    for (size_t h = 0; h < pt.hub_row.size(); h++) {
+      s->hub_row.push_back(pt.hub_row[h]);
+      s->hub_slot.push_back(slot);
+      s->hub_nseg.push_back(pt.hub_nseg[h]);
+      slot += pt.hub_nseg[h];
+    }
+    s->n_slots += pt.n_slots;
+    s->max_row_len = std::max(s->max_row_len, pt.max_row_len);
+  }
   if (exact_len > 0 && !s->hub_row.empty()) {
     // Whole-row hubs of more than kLongHub edges first, longest first (they run
     // longest); the other hubs spread evenly among the row tiles, so their
