@@ -931,18 +931,22 @@ def main():
         "avg_launch_ms": round(avg_pull_ms, 5),
         "algorithmic_bytes_per_launch": bytes_iter,
     }
-    # the same steps in production mode: chunks replayed from captured hipGraphs,
-    # no per-launch events (the timed region above launches eagerly to time
-    # every pull launch)
+    # The headline: the same steps in production mode (the plan's own run:
+    # chunks of iterations replayed from captured hipGraphs, no per-launch
+    # events).  The timed region above launches eagerly with HIP events around
+    # every pull launch to time the pull phase for the roofline; it is
+    # reported beside the headline as "profiled".
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step(False)
     torch.cuda.synchronize()
     dt_prod = time.perf_counter() - t0
-    production = {"ms_per_step": round(dt_prod / args.steps * 1e3, 4),
-                  "gteps": round(E * args.iters * args.steps / dt_prod / 1e9, 3),
-                  "mode": "hipGraph replay of the iteration chunks, no events"}
+    log(f"timed {args.steps} production steps in {dt_prod:.3f} s")
+    gteps_prod = E * args.iters * args.steps / dt_prod / 1e9
+    profiled = {"ms_per_step": round(dt / args.steps * 1e3, 4), "gteps": round(gteps, 3),
+                "mode": "eager launches, HIP events around every pull launch (the roofline's "
+                        "kernel timing)"}
     extras = {}
     checks = {}  # GPU results the CPU leg compares with the oracle
     if not args.no_extras:
@@ -995,8 +999,9 @@ def main():
                                                    verify=(verify_csr, scores_h), checks=checks,
                                                    extras=extras)
     rec = {
-        "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "metric": METRIC, "value": round(gteps_prod, 3), "unit": "GTEPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt_prod / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": f"RMAT-{args.scale} pull PageRank, {args.iters} fixed iterations",
@@ -1009,7 +1014,8 @@ def main():
                    "block": info["block"], "hub_threshold": info["hub_threshold"],
                    "plan_config_source": config_source,
                    "plan_ms": round(info["plan_ms"], 1)},
-        "roofline": roof, "cpu_baseline": cpu, "production": production,
+        "mode": "hipGraph replay of the iteration chunks, no events (the plan's run)",
+        "roofline": roof, "cpu_baseline": cpu, "profiled": profiled,
     }
     if "ppr" in extras:
         rec["ppr_speedup_vs_looped"] = extras["ppr"]["speedup"]
