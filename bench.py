@@ -697,6 +697,12 @@ def bench_multigraph(n, lo, hi, tol, seed, tune_cache=None):
     out["speedup_vs_looped"] = round(out["looped_ms"] / out["hipgraph_ms"], 2)
     out["gteps_hipgraph"] = round(int(sum(int(g[1].shape[0]) * int(i) for g, i in zip(gs, its)))
                                   / (out["hipgraph_ms"] * 1e-3) / 1e9, 2)
+    # plain PR's 4E + 24V + 4 bytes per graph-iteration, summed over the run
+    byts = sum((4 * int(g[1].shape[0]) + 24 * int(g[2].shape[0]) + 4) * int(i)
+               for g, i in zip(gs, its))
+    out["algorithmic_bytes"] = byts
+    out["achieved_gbs"] = round(byts / (out["hipgraph_ms"] * 1e-3) / 1e9, 1)
+    out["frac_of_hbm_peak"] = round(out["achieved_gbs"] / HBM_PEAK_GBS, 4)
     for p in plans:
         p.close()
     mg.close()
@@ -735,6 +741,13 @@ def bench_adjoint(scale, spmv_log2, seed):
            "backward_iterations": b["iterations"], "backward_ms": round(b["solve_ms"], 3),
            "forward_gteps": round(nnz * f["iterations"] / (f["solve_ms"] * 1e-3) / 1e9, 2),
            "backward_gteps": round(nnz * b["iterations"] / (b["solve_ms"] * 1e-3) / 1e9, 2)}
+    # per iteration, forward and adjoint alike: plain PR's 4E + 24V + 4 bytes
+    # (the adjoint pulls over the reversed graph)
+    it_bytes = 4 * nnz + 24 * m + 4
+    for k, r in (("forward", f), ("backward", b)):
+        gbs = it_bytes * r["iterations"] / (r["solve_ms"] * 1e-3) / 1e9
+        out[f"{k}_achieved_gbs"] = round(gbs, 1)
+        out[f"{k}_frac_of_hbm_peak"] = round(gbs / HBM_PEAK_GBS, 4)
     wp.close()
     del off, col, wt, grad
     torch.cuda.empty_cache()
@@ -812,7 +825,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (scales rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
-    ap.add_argument("--only", choices=["symgs", "csr_build", "spmv", "small", "ppr", "adjoint"],
+    ap.add_argument("--only", choices=["symgs", "csr_build", "spmv", "small", "ppr", "adjoint",
+                                       "multigraph"],
                     default=None,
                     help="run just this secondary figure and print its JSON (for profiling)")
     ap.add_argument("--tol-run", type=float, default=1e-6,
@@ -845,6 +859,10 @@ def main():
         return
     if args.only == "adjoint":
         print(json.dumps({"adjoint": bench_adjoint(20, args.spmv_log2, args.seed)}), flush=True)
+        return
+    if args.only == "multigraph":
+        print(json.dumps({"multigraph": bench_multigraph(64, 12, 18, 1e-4, args.seed,
+                                                         args.tune_cache)}), flush=True)
         return
     if args.only == "csr_build":
         t0 = time.perf_counter()
