@@ -75,7 +75,7 @@ class PrPlanConfig(ctypes.Structure):
     _fields_ = [("relabel", ctypes.c_int), ("items_per_thread", ctypes.c_int),
                 ("nontemporal", ctypes.c_int), ("persistent", ctypes.c_int),
                 ("sort_tiles", ctypes.c_int), ("block", ctypes.c_int),
-                ("hub_threshold", ctypes.c_int)]
+                ("hub_threshold", ctypes.c_int), ("stage_hubs", ctypes.c_int)]
 
 
 class PrPlanInfo(ctypes.Structure):
@@ -93,6 +93,7 @@ class PrPlanInfo(ctypes.Structure):
         ("sort_tiles", ctypes.c_int32),
         ("block", ctypes.c_int32),
         ("hub_threshold", ctypes.c_int32),
+        ("staged_hubs", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -51,6 +51,7 @@ PR_SPACE = {
     "relabel": [True, False],
     "nontemporal": [0, 1, 2],
     "persistent": [False, True],
+    "stage_hubs": [-1, 0, 1],
 }
 PPR_SPACE = {"k_chunk": [256, 128, 64], "hub_cols": [64, 32, 16], "relabel": [True, False]}
 MULTI_SPACE = {"chunk": [8, 4, 16, 32, 2]}

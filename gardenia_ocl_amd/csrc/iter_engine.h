@@ -45,6 +45,13 @@ struct IterEngine {
   Schedule sched;
   int n_tiles = 0, n_hubs = 0, n_hub_tiles = 0, fin_blocks = 1;
   bool fused = false;  // the pull launches finish each iteration (no finalize launch)
+  // Exact hub rows gathered by hub_stage_kernel ahead of exact_hub_kernel
+  // (every hub of the plan, or none): -1 auto (when the longest hub's chain would
+  // outlast the row tiles: longest row * 500 > edges in row tiles), 0 off, 1 on.
+  // Not used by the one-launch small iterations.
+  int stage_hubs = -1;
+  int n_stage_blocks = 0, n_staged_hubs = 0;
+  DeviceBuffer stage, stage_blocks, hub_soff;
   DeviceBuffer tiles, hub_tiles, hub_row, hub_slot, hub_nseg, hub_part, err_part, fin_part, state,
       errors;
   int errors_cap = 0;
@@ -140,6 +147,35 @@ struct IterEngine {
       if (rc) return rc;
       GD_HIP_TRY(hipMemcpyAsync(hub_tiles.ptr, hubs.data(), sizeof(Tile) * n_hub_tiles,
                                 hipMemcpyHostToDevice, stream));
+      // staged: every hub, in hub order
+      int64_t hub_edges = 0;
+      for (const Tile& t : hubs) hub_edges += t.w - t.z;
+      const int64_t longest = hubs[0].w - hubs[0].z;
+      const bool want = stage_hubs == 1 ||
+                        (stage_hubs < 0 && longest * 500 > (int64_t)nnz - hub_edges);
+      std::vector<int32_t> soff(n_hub_tiles, -1);
+      std::vector<int4> blocks;
+      int64_t staged = 0;
+      for (int h = 0; want && h < n_hub_tiles; h++) {
+        const int ne = hubs[h].w - hubs[h].z;
+        soff[h] = (int32_t)staged;
+        n_staged_hubs++;
+        for (int o = 0; o < ne; o += kScanChunk)
+          blocks.push_back(make_int4(hubs[h].z + o, std::min(kScanChunk, ne - o),
+                                     (int)(staged + o), 0));
+        staged += ne;
+      }
+      n_stage_blocks = (int)blocks.size();
+      if (n_stage_blocks) {
+        rc = stage.alloc(4 * (size_t)staged);
+        if (!rc) rc = stage_blocks.alloc(sizeof(int4) * blocks.size());
+        if (!rc) rc = hub_soff.alloc(4 * (size_t)n_hub_tiles);
+        if (rc) return rc;
+        GD_HIP_TRY(hipMemcpyAsync(stage_blocks.ptr, blocks.data(), sizeof(int4) * blocks.size(),
+                                  hipMemcpyHostToDevice, stream));
+        GD_HIP_TRY(hipMemcpyAsync(hub_soff.ptr, soff.data(), 4 * (size_t)n_hub_tiles,
+                                  hipMemcpyHostToDevice, stream));
+      }
       GD_HIP_TRY(hipStreamSynchronize(stream));  // host vectors go out of scope
       GD_HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
       GD_HIP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
@@ -191,10 +227,12 @@ struct IterEngine {
 
   IterArgs iter_args() {
     const int parts = (n_tiles > 0 ? pull_grid : 0) + n_hub_tiles;
+    const bool staged = n_stage_blocks > 0 && !fused;
     return IterArgs{tiles.as<Tile>(), n_tiles, hub_tiles.as<Tile>(), n_hub_tiles,
                     n_tiles > 0 ? pull_grid : 0, off, col, e_max, pos.as<uint16_t>(),
                     hub_part.as<float>(), err_part.as<double>(), state.as<IterState>(),
-                    fused ? parts : 0, errors.as<double>()};
+                    fused ? parts : 0, errors.as<double>(),
+                    staged ? hub_soff.as<int32_t>() : nullptr, stage.as<float>()};
   }
 
   // exact hubs finish their rows in the pull launch: no hub epilogues here
@@ -230,6 +268,13 @@ struct IterEngine {
       return;
     }
     const bool fork = n_hub_tiles > 0 && n_tiles > 0;
+    const bool staged = P::kExactHubs && n_hub_tiles > 0 && it.hub_soff;
+    // staged hubs: their values are gathered first, by the whole GPU, before
+    // the fork (the row tiles would otherwise fill the CUs first and hold the
+    // hub rows back until they drain)
+    if (staged)
+      hipLaunchKernelGGL((hub_stage_kernel<P, NT>), dim3(n_stage_blocks), dim3(kBlock), 0, stream,
+                         a, it, stage_blocks.as<int4>(), stage.as<float>());
     hipStream_t ts = stream;
     if (fork) {
       (void)hipEventRecord(fork_ev, stream);
@@ -237,9 +282,15 @@ struct IterEngine {
       ts = side;
     }
     if (n_hub_tiles) {
-      if constexpr (P::kExactHubs)
-        hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED>), dim3(n_hub_tiles),
-                           dim3(kBlock), 0, stream, a, it);
+      if constexpr (P::kExactHubs) {
+        if (staged) {
+          hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED, true>), dim3(n_hub_tiles),
+                             dim3(kBlock), 0, stream, a, it);
+        } else {
+          hipLaunchKernelGGL((exact_hub_kernel<P, IPT, NT, SORTED, false>), dim3(n_hub_tiles),
+                             dim3(kBlock), 0, stream, a, it);
+        }
+      }
     }
     if (n_tiles > 0) {
       if (persistent)

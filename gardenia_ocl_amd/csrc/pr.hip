@@ -113,6 +113,7 @@ void gd_pr_default_plan_config(gd_pr_plan_config* c) {
   c->sort_tiles = -1;
   c->block = 0;
   c->hub_threshold = 0;
+  c->stage_hubs = -1;
 }
 
 int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t* d_col,
@@ -135,6 +136,7 @@ int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t
   GD_CHECK_ARG(c.persistent >= -1 && c.persistent <= 1, "persistent must be -1, 0 or 1");
   GD_CHECK_ARG(c.block == 0 || c.block == 256 || c.block == 512 || c.block == 1024,
                "block must be 0, 256, 512 or 1024");
+  GD_CHECK_ARG(c.stage_hubs >= -1 && c.stage_hubs <= 1, "stage_hubs must be -1, 0 or 1");
   const int ipt = c.items_per_thread ? c.items_per_thread : 16;
   GD_CHECK_ARG(c.hub_threshold == 0 || (c.hub_threshold >= 64 && c.hub_threshold <= kBlock * ipt),
                "hub_threshold must be 0 or in [64, 256*items_per_thread]");
@@ -161,6 +163,7 @@ int gd_pr_plan_create_ex(int m, int64_t nnz, const int32_t* d_off, const int32_t
   // reference's bit for bit (its sequential FP32 sum drifts by 3e-4 relative
   // on the 238,668-edge RMAT-24 row, so matching its rounding matters).
   p->eng.exact_len = c.hub_threshold ? c.hub_threshold : kBlock * ipt;
+  p->eng.stage_hubs = c.stage_hubs;
   int rc = GD_OK;
   if (m > 0) {
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -203,6 +206,7 @@ int gd_pr_plan_info(const gd_pr_plan* p, gd_pr_plan_info_t* info) {
   info->sort_tiles = p->eng.sort_tiles ? 1 : 0;
   info->block = p->eng.block;
   info->hub_threshold = p->eng.exact_len;
+  info->staged_hubs = p->eng.fused ? 0 : p->eng.n_staged_hubs;
   return GD_OK;
 }
 

@@ -615,8 +615,9 @@ __device__ __forceinline__ float exact_scan_chunk(const float* __restrict__ val,
 // chunk c+1's gathers and chunk c+2's index loads are in flight meanwhile.
 // The first kExactHead terms are one lane's chain (lane_seq_sum): there the
 // running sum crosses binades every few terms, and each crossing would cost
-// the scan a restart.
-template <bool NT, class G>
+// the scan a restart.  STAGED: the row's values were gathered beforehand into
+// a buffer (`g` reads it by edge, the column index is not loaded).
+template <bool NT, class G, bool STAGED = false>
 __device__ float exact_row_sum(const int32_t* __restrict__ col, int e_max, int e0, int ne,
                                const G& g, float s0, float* __restrict__ buf0,
                                float* __restrict__ buf1, ExactLds& x) {
@@ -629,7 +630,8 @@ __device__ float exact_row_sum(const int32_t* __restrict__ col, int e_max, int e
   };
   auto load_idx = [&](int c) {
 #pragma unroll
-    for (int j = 0; j < kScanPT; j++) idx[j] = ld_stream<NT>(col + min(e0 + elem(c, j), e_max));
+    for (int j = 0; j < kScanPT; j++)
+      idx[j] = STAGED ? 0 : ld_stream<NT>(col + min(e0 + elem(c, j), e_max));
   };
   auto gather = [&](int c) {
 #pragma unroll

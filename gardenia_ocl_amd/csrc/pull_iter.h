@@ -62,6 +62,17 @@ struct IterArgs {
   // over this many L1 partials, instead of a finalize launch
   int fuse_parts;
   double* __restrict__ errors;  // per-iteration L1 (fused_finish)
+  // staged plans: exact hub h's values gathered beforehand (hub_stage_kernel)
+  // at stage[hub_soff[h]...]; nullptr when the plan does not stage
+  const int32_t* __restrict__ hub_soff;
+  const float* __restrict__ stage;
+};
+
+// Values of a staged hub row: the row's first edge e0 is stage[0].
+struct StagedGather {
+  const float* __restrict__ s;
+  int e0;
+  __device__ __forceinline__ float value(int e, int /*u*/) const { return s[e - e0]; }
 };
 
 struct FinArgs {
@@ -225,7 +236,7 @@ struct HubLds {
   ExactLds xs;
 };
 
-template <class P, int IPT, int NT, bool SORTED>
+template <class P, int IPT, int NT, bool SORTED, bool STAGED = false>
 __device__ __forceinline__ void hub_body(const typename P::Args& a, const IterArgs& it, int h,
                                          HubLds& lds) {
   const int odd = it.st->iter & 1;
@@ -237,8 +248,14 @@ __device__ __forceinline__ void hub_body(const typename P::Args& a, const IterAr
   const auto e = P::template epilogue<NT>(a, odd);
   const auto pre = e.pre(t.x);
   const int ne = t.w - t.z;
-  const float sum = exact_row_sum<(NT > 0)>(it.col, it.e_max, t.z, ne, g, e.init(pre), buf,
-                                            buf + kScanChunk, xs);
+  float sum;
+  if constexpr (STAGED)
+    sum = exact_row_sum<(NT > 0), StagedGather, true>(
+        it.col, it.e_max, t.z, ne, StagedGather{it.stage + it.hub_soff[h], t.z}, e.init(pre), buf,
+        buf + kScanChunk, xs);
+  else
+    sum = exact_row_sum<(NT > 0)>(it.col, it.e_max, t.z, ne, g, e.init(pre), buf,
+                                  buf + kScanChunk, xs);
   double err = 0.0;
   if (threadIdx.x == 0) err = e.finish(t.x, sum, pre);
   const double tot = block_sum<kWaves>(err, s_err);
@@ -249,14 +266,42 @@ __device__ __forceinline__ void hub_body(const typename P::Args& a, const IterAr
   }
 }
 
-// Held to 3 waves per SIMD (<= 168 VGPRs, none spilled): the hub workgroups
-// share the CUs with the row tiles running beside them.
-template <class P, int IPT, int NT, bool SORTED>
+// Gathers the values of the long exact hub rows into `stage` ahead of
+// exact_hub_kernel (same stream): the hub kernel then scans values read
+// coalesced from L2 instead of waiting on each chunk's random gathers, so a
+// long row's workgroup is bound by its scan.  Workgroup b gathers block b:
+// x = first edge, y = edges (<= kScanChunk), z = offset in stage.
+template <class P, int NT>
+__global__ __launch_bounds__(kBlock) void hub_stage_kernel(const typename P::Args a,
+                                                           const IterArgs it,
+                                                           const int4* __restrict__ blocks,
+                                                           float* __restrict__ stage) {
+  if (it.st->done) return;
+  const int4 b = blocks[blockIdx.x];
+  const auto g = P::template gather<NT>(a, it.st->iter & 1);
+  int idx[kScanPT];
+  float v[kScanPT];
+#pragma unroll
+  for (int j = 0; j < kScanPT; j++)
+    idx[j] = ld_stream<(NT > 0)>(it.col + b.x + min((int)threadIdx.x + j * kBlock, b.y - 1));
+#pragma unroll
+  for (int j = 0; j < kScanPT; j++)
+    v[j] = g.value(b.x + min((int)threadIdx.x + j * kBlock, b.y - 1), idx[j]);
+#pragma unroll
+  for (int j = 0; j < kScanPT; j++)
+    if ((int)threadIdx.x + j * kBlock < b.y) stage[b.z + threadIdx.x + j * kBlock] = v[j];
+}
+
+// Held to 3 waves per SIMD (<= 168 VGPRs): the hub workgroups share the CUs
+// with the row tiles running beside them.  The gathering form spills three
+// registers once per 4096-edge chunk, outside the scan's loops; the staged
+// form spills none.
+template <class P, int IPT, int NT, bool SORTED, bool STAGED>
 __global__ __launch_bounds__(kBlock, 3) void exact_hub_kernel(const typename P::Args a,
                                                               const IterArgs it) {
   if (it.st->done) return;
   __shared__ HubLds lds;
-  hub_body<P, IPT, NT, SORTED>(a, it, blockIdx.x, lds);
+  hub_body<P, IPT, NT, SORTED, STAGED>(a, it, blockIdx.x, lds);
 }
 
 // Small iterations (it.fuse_parts > 0) in one launch: workgroups [0,

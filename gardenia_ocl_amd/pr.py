@@ -179,13 +179,16 @@ class PageRankPlan:
 
     def __init__(self, in_row_offsets, in_column_indices, out_degree, stream=None, *,
                  relabel=None, items_per_thread=0, nontemporal=None, persistent=None,
-                 sort_tiles=None, block=0, hub_threshold=0):
+                 sort_tiles=None, block=0, hub_threshold=0, stage_hubs=-1):
         """relabel / nontemporal / persistent / sort_tiles: None = library default,
         True / False force it (sort_tiles: gather each tile's edges in source
         order; same results).  block: threads per row-tile workgroup (256, 512
         or 1024; 0 = library default).  hub_threshold: rows with more in-edges
         are whole-row hubs of their own workgroup (64 .. 256*items_per_thread;
-        0 = library default).  Every setting gives the same results."""
+        0 = library default).  stage_hubs: 1 gathers the hub rows with a kernel
+        of their own before their workgroups sum them, 0 never, -1 library
+        default (when the longest hub would outlast the row tiles).  Every
+        setting gives the same results."""
         import torch
 
         for t in (in_row_offsets, in_column_indices, out_degree):
@@ -210,6 +213,7 @@ class PageRankPlan:
             cfg.sort_tiles = 1 if sort_tiles else 0
         cfg.block = int(block)
         cfg.hub_threshold = int(hub_threshold)
+        cfg.stage_hubs = int(stage_hubs)
         h = ctypes.c_void_p()
         N.call("gd_pr_plan_create_ex", self.m, self.nnz, in_row_offsets.data_ptr(),
                in_column_indices.data_ptr(), out_degree.data_ptr(),

@@ -94,16 +94,23 @@ typedef struct gd_pr_plan_config {
                            owner thread inside a row tile; 64 .. 256*items_per_thread;
                            0: auto (256*items_per_thread, the longest row a 256-thread
                            tile holds) */
+  int stage_hubs;       /* 1: the hub rows are gathered by a kernel of their own, spread
+                           over the GPU, before their
+                           workgroups sum them (each hub workgroup is then bound by its scan
+                           instead of its gathers); same results; 0: never; -1: auto (when
+                           the longest hub row * 500 exceeds the edges in row tiles: the
+                           longest row's chain would outlast the row tiles).  Iterations
+                           small enough to run as one launch never stage */
 } gd_pr_plan_config;
 
 void gd_pr_default_plan_config(gd_pr_plan_config *cfg);
 
 typedef struct gd_pr_plan_info_t {
   int64_t n_tiles;        /* workgroups per iteration: row tiles + whole-row hub tiles */
-  int64_t n_hubs;         /* rows summed as hubs: longer than one tile, 256 *
-                             items_per_thread edges (whole-row tiles in the
-                             reference's summation order: one lane's chain up to
-                             2^17 edges, the workgroup's exact scan beyond) */
+  int64_t n_hubs;         /* rows summed as hubs: longer than hub_threshold edges
+                             (whole-row tiles in the reference's summation order:
+                             one lane's chain over the first 4096 edges, the
+                             workgroup's exact scan beyond) */
   int64_t n_hub_segments; /* tiles spent on hub rows */
   int32_t max_row_len;
   int32_t relabel;        /* 1 if the plan relabelled the vertices */
@@ -115,6 +122,7 @@ typedef struct gd_pr_plan_info_t {
   int32_t sort_tiles;     /* 1 if tiles are gathered in source order */
   int32_t block;          /* threads per row-tile workgroup */
   int32_t hub_threshold;  /* rows longer than this are whole-row hubs */
+  int32_t staged_hubs;    /* hub rows gathered ahead by the staging kernel (0: none) */
 } gd_pr_plan_info_t;
 
 /* Plan over a device-resident in-CSR (rows = destinations, columns = sources)

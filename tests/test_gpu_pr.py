@@ -23,7 +23,8 @@ def _key(ds):
 
 
 def _gpu_pr(device, in_csr, outdeg, tol, max_iter, seed_vertex=-1, init=None, chunk=0,
-            relabel=None, ipt=0, persistent=None, block=0, sort_tiles=None, hub_threshold=0):
+            relabel=None, ipt=0, persistent=None, block=0, sort_tiles=None, hub_threshold=0,
+            stage_hubs=-1):
     import torch
 
     m = in_csr.m
@@ -35,7 +36,7 @@ def _gpu_pr(device, in_csr, outdeg, tol, max_iter, seed_vertex=-1, init=None, ch
     scores = torch.from_numpy(np.array(init, np.float32)).to(device)
     plan = PageRankPlan(off, col, deg, relabel=relabel, items_per_thread=ipt,
                         persistent=persistent, block=block, sort_tiles=sort_tiles,
-                        hub_threshold=hub_threshold)
+                        hub_threshold=hub_threshold, stage_hubs=stage_hubs)
     stats, errs = plan.run(scores, tol=tol, max_iter=max_iter, seed_vertex=seed_vertex,
                            chunk=chunk)
     out = scores.cpu().numpy()
@@ -432,6 +433,40 @@ def test_pr_hub_threshold_bit_exact(device, hub_threshold, ipt):
                                   out_csr.degree, tol=1e-6, max_iter=100)
     np.testing.assert_array_equal(s, ref)
     assert_same_iterations(stats["iterations"], errs, it, ref_errs, 1e-6)
+
+
+@pytest.mark.parametrize("hub_threshold,seed", [(0, -1), (1024, -1), (0, "hub")])
+def test_pr_staged_hubs_bit_exact(device, hub_threshold, seed):
+    """Hub rows gathered ahead by the staging kernel (stage_hubs=1) give the
+    oracle's scores and iteration count, as the gathering form does: RMAT-18
+    (over 512 workgroups per iteration, so the staged launch runs), the
+    default cut, a 1024-edge cut (hubs of one scan chunk staged too), and a
+    personalised start on a hub row's source."""
+    out_csr, in_csr = graph.rmat(18, 16, seed=9)
+    m = in_csr.m
+    sv, init = -1, None
+    if seed == "hub":
+        deg = np.diff(in_csr.row_offsets)
+        r = int(np.argmax(deg))
+        sv = int(in_csr.column_indices[in_csr.row_offsets[r]])
+        init = np.zeros(m, np.float32)
+        init[sv] = 1.0
+    s, stats, errs, info = _gpu_pr(device, in_csr, out_csr.degree, 1e-6, 100, seed_vertex=sv,
+                                   init=init, hub_threshold=hub_threshold, stage_hubs=1)
+    assert info["staged_hubs"] > 0 and info["staged_hubs"] == info["hubs"]
+    if sv < 0:
+        ref, it, ref_errs = oracle.pr(m, in_csr.row_offsets, in_csr.column_indices,
+                                      out_csr.degree, tol=1e-6, max_iter=100)
+    else:
+        ref, it, ref_errs = oracle.ppr(m, in_csr.row_offsets, in_csr.column_indices,
+                                       out_csr.degree, sv, tol=1e-6, max_iter=100)
+    np.testing.assert_array_equal(s, ref)
+    assert_same_iterations(stats["iterations"], errs, it, ref_errs, 1e-6)
+    s0, st0, _, info0 = _gpu_pr(device, in_csr, out_csr.degree, 1e-6, 100, seed_vertex=sv,
+                                init=init, hub_threshold=hub_threshold, stage_hubs=0)
+    assert info0["staged_hubs"] == 0
+    np.testing.assert_array_equal(s0, s)
+    assert st0["iterations"] == stats["iterations"]
 
 
 def test_pr_personalized_exact_hubs_power_of_two_sums(device):
