@@ -288,7 +288,14 @@ int gd_wpr_plan_create(int m, int64_t nnz, const int32_t* d_off, const int32_t* 
     e->nt = 0;
     e->persistent = false;
   }
-  p->fwd.exact_len = 2048;  // hub rows in edge order, as the plain PageRank plan
+  // The forward sums every row in edge order (bit-exact), so it takes the
+  // plain PageRank plan's tiles: 4096 edges, gathered in source order, hub
+  // rows above 4096 edges.  The adjoint tree-sums its longer rows: sorting or
+  // resizing its tiles changes that rounding, and at RMAT-20, tol 1e-7, took
+  // 145 iterations instead of 128 at the same time per iteration.
+  p->fwd.ipt = 16;
+  p->fwd.sort_tiles = true;
+  p->fwd.exact_len = 4096;  // hub rows in edge order, as the plain PageRank plan
   rc = p->fwd.build(m, nnz, d_off, d_col, p->stream);
   if (!rc) rc = p->fwd.setup_grid<WprPolicy>();
   if (!rc) rc = p->adj.build(m, nnz, p->tr.off.as<int32_t>(), p->tr.row.as<int32_t>(), p->stream);
