@@ -81,10 +81,11 @@ def test_pr_plan_config_rejected_before_device():
     h = ctypes.c_void_p()
     for field, value, text in (("hub_threshold", 63, b"hub_threshold"),
                                ("hub_threshold", 4097, b"hub_threshold"),
-                               ("block", 300, b"block"), ("items_per_thread", 4, b"items_per_thread")):
+                               ("block", 300, b"block"), ("items_per_thread", 4, b"items_per_thread"),
+                               ("stage_hubs", 2, b"stage_hubs"), ("stage_hubs", -2, b"stage_hubs")):
         cfg = N.PrPlanConfig()
         lib.gd_pr_default_plan_config(ctypes.byref(cfg))
-        assert cfg.hub_threshold == 0 and cfg.block == 0
+        assert cfg.hub_threshold == 0 and cfg.block == 0 and cfg.stage_hubs == -1
         setattr(cfg, field, value)
         assert lib.gd_pr_plan_create_ex(0, 0, None, None, None, None, ctypes.byref(cfg),
                                         ctypes.byref(h)) == N.GD_ERR_INVALID, field
