@@ -1,4 +1,4 @@
-// relabel.h -- plan-time vertex relabelling by decreasing out-degree.
+// relabel.h -- plan-time vertex relabelling by decreasing out-degree octave.
 #ifndef GD_RELABEL_H_
 #define GD_RELABEL_H_
 

@@ -65,7 +65,8 @@ typedef struct gd_pr_plan gd_pr_plan;
 
 /* Plan-time choices (the autotuner's knobs). */
 typedef struct gd_pr_plan_config {
-  int relabel;          /* 1: relabel vertices by decreasing out-degree inside the plan
+  int relabel;          /* 1: relabel vertices by decreasing out-degree octave (original
+                           order inside an octave) inside the plan
                            (hot sources become an L2-resident prefix of contrib; rows keep
                            their edge order; scores permuted in/out per run), 0: never,
                            -1: auto (when nnz >= 2^20) */
