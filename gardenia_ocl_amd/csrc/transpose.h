@@ -35,6 +35,7 @@ int permute_values(int64_t nnz, const int32_t* perm, const float* src, float* ds
 // line from memory per element.
 struct BucketedPermutation {
   int64_t n = 0;
+  bool direct = false;      // too large for the chunk sort: perm holds src positions, one gather
   DeviceBuffer pos;         // [n] bucket-buffer position of source entry k (build time only)
   DeviceBuffer lidx;        // [n] uint16: per 2048-entry chunk, its entries in position order
   DeviceBuffer pos_sorted;  // [n] the bucket-buffer position of each of them
@@ -42,7 +43,9 @@ struct BucketedPermutation {
   DeviceBuffer buf;         // [n] the bucket buffer (values)
 };
 
-// keys[k] < 2^key_bits.  Synchronises `stream`.
+// keys[k] < 2^key_bits.  Synchronises `stream`.  When (chunks x buckets) do
+// not fit a 31-bit sort key (about 2^29.5 entries), the permutation is kept
+// as one direct gather instead.
 int build_bucketed_permutation(int64_t n, const int32_t* d_keys, int key_bits,
                                const int32_t* d_perm, hipStream_t stream, BucketedPermutation* b);
 // dst[i] = src[perm[i]]  (async on stream)

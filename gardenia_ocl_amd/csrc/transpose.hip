@@ -213,6 +213,20 @@ int permute_values(int64_t nnz, const int32_t* perm, const float* src, float* ds
 int build_bucketed_permutation(int64_t n, const int32_t* d_keys, int key_bits,
                                const int32_t* d_perm, hipStream_t stream, BucketedPermutation* b) {
   b->n = n;
+  // the per-chunk store order sorts (chunk, bucket) keys of 31 bits: past that
+  // (about 2^29.5 entries) the permutation is one direct gather
+  int chunk_bits = 1;
+  while ((1ll << chunk_bits) < (n + kPermChunk - 1) / kPermChunk) chunk_bits++;
+  if (chunk_bits + key_bits > 31) {
+    b->direct = true;
+    int rc = b->perm.alloc(4 * (size_t)std::max<int64_t>(n, 1));
+    if (rc) return rc;
+    if (n > 0)
+      GD_HIP_TRY(hipMemcpyAsync(b->perm.ptr, d_perm, 4 * (size_t)n, hipMemcpyDeviceToDevice,
+                                stream));
+    GD_HIP_TRY(hipStreamSynchronize(stream));
+    return GD_OK;
+  }
   int rc = b->pos.alloc(4 * (size_t)std::max<int64_t>(n, 1));
   if (!rc) rc = b->perm.alloc(4 * (size_t)std::max<int64_t>(n, 1));
   if (!rc) rc = b->buf.alloc(4 * (size_t)std::max<int64_t>(n, 1));
@@ -241,13 +255,7 @@ int build_bucketed_permutation(int64_t n, const int32_t* d_keys, int key_bits,
   GD_HIP_TRY(hipGetLastError());
   // each chunk's entries in the order of their bucket-buffer positions: a
   // stable sort by (chunk, bucket) (the positions increase with the source
-  // index inside a bucket); chunk < 2^(31 - key_bits) for n < 2^31
-  int chunk_bits = 1;
-  while ((1ll << chunk_bits) < (n + kPermChunk - 1) / kPermChunk) chunk_bits++;
-  if (chunk_bits + key_bits > 31) {
-    set_error("bucketed permutation: too many chunks x buckets");
-    return GD_ERR_INVALID;
-  }
+  // index inside a bucket)
   rc = b->lidx.alloc(2 * (size_t)n);
   if (!rc) rc = b->pos_sorted.alloc(4 * (size_t)n);
   if (rc) return rc;
@@ -282,6 +290,7 @@ int build_bucketed_permutation(int64_t n, const int32_t* d_keys, int key_bits,
 int run_bucketed_permutation(const BucketedPermutation& b, const float* src, float* dst,
                              hipStream_t stream) {
   if (b.n == 0) return GD_OK;
+  if (b.direct) return permute_values(b.n, b.perm.as<int32_t>(), src, dst, stream);
   hipLaunchKernelGGL(scatter_sorted_kernel, dim3(xcd_grid(b.n)), dim3(256), 0, stream, b.n,
                      b.lidx.as<uint16_t>(), b.pos_sorted.as<int32_t>(), src, b.buf.as<float>());
   hipLaunchKernelGGL(permute_kernel<float>, dim3(xcd_grid(b.n)), dim3(256), 0, stream, b.n,
