@@ -435,13 +435,16 @@ def test_pr_hub_threshold_bit_exact(device, hub_threshold, ipt):
     assert_same_iterations(stats["iterations"], errs, it, ref_errs, 1e-6)
 
 
-@pytest.mark.parametrize("hub_threshold,seed", [(0, -1), (1024, -1), (0, "hub")])
-def test_pr_staged_hubs_bit_exact(device, hub_threshold, seed):
+@pytest.mark.parametrize("hub_threshold,seed,launch", [
+    (0, -1, {}), (1024, -1, {}), (0, "hub", {}), (0, -1, {"persistent": True}),
+    (0, -1, {"block": 512})])
+def test_pr_staged_hubs_bit_exact(device, hub_threshold, seed, launch):
     """Hub rows gathered ahead by the staging kernel (stage_hubs=1) give the
     oracle's scores and iteration count, as the gathering form does: RMAT-18
     (over 512 workgroups per iteration, so the staged launch runs), the
-    default cut, a 1024-edge cut (hubs of one scan chunk staged too), and a
-    personalised start on a hub row's source."""
+    default cut, a 1024-edge cut (hubs of one scan chunk staged too), a
+    personalised start on a hub row's source, and the persistent-grid and
+    512-thread row-tile launches beside the staged hubs."""
     out_csr, in_csr = graph.rmat(18, 16, seed=9)
     m = in_csr.m
     sv, init = -1, None
@@ -452,7 +455,7 @@ def test_pr_staged_hubs_bit_exact(device, hub_threshold, seed):
         init = np.zeros(m, np.float32)
         init[sv] = 1.0
     s, stats, errs, info = _gpu_pr(device, in_csr, out_csr.degree, 1e-6, 100, seed_vertex=sv,
-                                   init=init, hub_threshold=hub_threshold, stage_hubs=1)
+                                   init=init, hub_threshold=hub_threshold, stage_hubs=1, **launch)
     assert info["staged_hubs"] > 0 and info["staged_hubs"] == info["hubs"]
     if sv < 0:
         ref, it, ref_errs = oracle.pr(m, in_csr.row_offsets, in_csr.column_indices,
@@ -463,7 +466,7 @@ def test_pr_staged_hubs_bit_exact(device, hub_threshold, seed):
     np.testing.assert_array_equal(s, ref)
     assert_same_iterations(stats["iterations"], errs, it, ref_errs, 1e-6)
     s0, st0, _, info0 = _gpu_pr(device, in_csr, out_csr.degree, 1e-6, 100, seed_vertex=sv,
-                                init=init, hub_threshold=hub_threshold, stage_hubs=0)
+                                init=init, hub_threshold=hub_threshold, stage_hubs=0, **launch)
     assert info0["staged_hubs"] == 0
     np.testing.assert_array_equal(s0, s)
     assert st0["iterations"] == stats["iterations"]
