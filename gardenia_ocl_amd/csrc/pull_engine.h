@@ -82,6 +82,10 @@ __device__ __forceinline__ T block_sum(T v, T* s_tmp) {
 // s + v[0] + ... + v[n-1] in order with FP32 rounding, by the calling lane,
 // from 16-byte aligned LDS: 4*kQ values per step, the next step's LDS reads in
 // flight while the current step is added (the adds are one dependent chain).
+// kQ = 8 (reads 32 terms ahead) hides the LDS latency behind the chain: at
+// RMAT-24 the pull phase took 1.339 ms against 1.367 ms with kQ = 4, at
+// RMAT-16 32.8 against 35.5 µs per iteration; kQ = 12 was no faster and costs
+// the row tiles 24 more VGPRs.
 template <int kQ>  // float4 reads per step
 __device__ __forceinline__ float lane_seq_sum(float s, const float* __restrict__ v, int n) {
   const float4* __restrict__ v4 = reinterpret_cast<const float4*>(v);
@@ -123,7 +127,7 @@ __device__ __forceinline__ float lane_seq_sum(float s, const float* __restrict__
 
 // Sequential sum s + v[b] + ... + v[e-1] from LDS (`v` 16-byte aligned) by
 // the calling lane, in order with FP32 rounding.  More than kChainRow terms:
-// scalar adds up to a 16-byte boundary, then float4 reads 16 terms ahead of
+// scalar adds up to a 16-byte boundary, then float4 reads 32 terms ahead of
 // the dependent adds (lane_seq_sum); fewer: 4 reads in flight.
 constexpr int kChainRow = 32;
 
@@ -131,7 +135,7 @@ __device__ __forceinline__ float seq_sum(float s, const float* __restrict__ v, i
   int k = b;
   if (e - b > kChainRow) {
     for (; k & 3; k++) s += v[k];
-    return lane_seq_sum<4>(s, v + k, e - k);
+    return lane_seq_sum<8>(s, v + k, e - k);
   }
   for (; k + 4 <= e; k += 4) {
     const float a0 = v[k], a1 = v[k + 1], a2 = v[k + 2], a3 = v[k + 3];
@@ -658,7 +662,7 @@ __device__ float exact_row_sum(const int32_t* __restrict__ col, int e_max, int e
       // the first terms, where s crosses a binade every few steps: one lane's
       // pipelined chain, in order
       const int nh = min(nc, kExactHead);
-      if (tid == 0) x.s = lane_seq_sum<4>(s, cur, nh);
+      if (tid == 0) x.s = lane_seq_sum<8>(s, cur, nh);
       __syncthreads();
       s = x.s;
       start = nh;
