@@ -222,6 +222,12 @@ struct IterEngine {
     const int parts = (n_tiles > 0 ? pull_grid : 0) + n_hub_tiles;
     fused = !persistent && block == kBlock && (exact_len > 0 || n_hubs == 0) && parts > 0 &&
             parts < 2 * kBlock;
+    if (fused && n_stage_blocks) {  // one-launch iterations gather their hubs themselves
+      stage.release();
+      stage_blocks.release();
+      hub_soff.release();
+      n_stage_blocks = n_staged_hubs = 0;
+    }
     return GD_OK;
   }
 

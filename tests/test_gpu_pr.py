@@ -472,6 +472,20 @@ def test_pr_staged_hubs_bit_exact(device, hub_threshold, seed, launch):
     assert st0["iterations"] == stats["iterations"]
 
 
+def test_pr_staging_forced_on_small_graph_runs_unstaged(device):
+    """stage_hubs=1 on a graph small enough for the one-launch iterations:
+    the plan keeps no staging (the fused launch gathers its hubs itself) and
+    the scores and iteration count are the oracle's."""
+    out_csr, in_csr = graph.rmat(14, 16, seed=3)
+    s, stats, errs, info = _gpu_pr(device, in_csr, out_csr.degree, 1e-6, 100, stage_hubs=1,
+                                   hub_threshold=512)
+    assert info["hubs"] > 0 and info["staged_hubs"] == 0
+    ref, it, ref_errs = oracle.pr(in_csr.m, in_csr.row_offsets, in_csr.column_indices,
+                                  out_csr.degree, tol=1e-6, max_iter=100)
+    np.testing.assert_array_equal(s, ref)
+    assert_same_iterations(stats["iterations"], errs, it, ref_errs, 1e-6)
+
+
 def test_pr_personalized_exact_hubs_power_of_two_sums(device):
     """One-hot personalised starts on a graph where every row above 64
     in-edges is an exact hub: a hub row gathering the seed holds the sum
