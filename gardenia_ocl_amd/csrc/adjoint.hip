@@ -173,18 +173,49 @@ __global__ void pack_cq_kernel(int n, const float* __restrict__ x, const float* 
   }
 }
 
-// Per-edge map over a CSR tile schedule: f(e, row, col) for every edge.  Row
-// tiles find an edge's row by binary search in the tile's offsets (LDS).
+// Row tiles of at most this many edges label their edges' rows in LDS first.
+constexpr int kMapEdges = 4096;
+
+// Per-edge map over a CSR tile schedule: f(e, row, col) for every edge.  A row
+// tile's edge belongs to the unique row i with s_off[i] <= e < s_off[i + 1]
+// (s_off = the tile's offsets in LDS).  Up to kMapEdges edges, thread t finds
+// it for edges [t * per, + per) by one binary search and a forward walk and
+// stores it in a row table, so each edge then costs one LDS read; larger
+// tiles binary-search every edge.  A launch with gridDim.y > 1 splits each
+// one-row (hub) tile's edges over gridDim.y workgroups; row tiles need
+// gridDim.y == 1.
 template <class F>
 __global__ __launch_bounds__(kBlock) void edge_map_kernel(const Tile* __restrict__ tiles,
                                                           const int32_t* __restrict__ off,
                                                           const int32_t* __restrict__ col, F f) {
   __shared__ int s_off[kTileRows + 1];
+  __shared__ uint16_t s_row[kMapEdges];
+  static_assert(kTileRows <= 65536, "row table holds 16-bit tile rows");
   const Tile t = tiles[blockIdx.x];
   if (t.y >= 0) {
     const int nr = t.y - t.x;
     for (int i = threadIdx.x; i <= nr; i += kBlock) s_off[i] = off[t.x + i];
     __syncthreads();
+    const int n = t.w - t.z;
+    if (n <= kMapEdges) {
+      const int per = (n + kBlock - 1) / kBlock;
+      const int e0 = t.z + threadIdx.x * per, e1 = min(e0 + per, t.w);
+      if (e0 < e1) {
+        int lo = 0, hi = nr;  // s_off[0] = t.z <= e0 < t.w = s_off[nr]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_off[mid] <= e0) lo = mid; else hi = mid;
+        }
+        int next = s_off[lo + 1];
+        for (int e = e0; e < e1; e++) {
+          while (next <= e) next = s_off[++lo + 1];  // stops below nr: s_off[nr] > e
+          s_row[e - t.z] = (uint16_t)lo;
+        }
+      }
+      __syncthreads();
+      for (int e = t.z + threadIdx.x; e < t.w; e += kBlock) f(e, t.x + s_row[e - t.z], col[e]);
+      return;
+    }
     for (int e = t.z + threadIdx.x; e < t.w; e += kBlock) {
       int lo = 0, hi = nr;  // largest i with s_off[i] <= e
       while (hi - lo > 1) {
@@ -194,7 +225,8 @@ __global__ __launch_bounds__(kBlock) void edge_map_kernel(const Tile* __restrict
       f(e, t.x + lo, col[e]);
     }
   } else {
-    for (int e = t.z + threadIdx.x; e < t.w; e += kBlock) f(e, t.x, col[e]);
+    for (int e = t.z + blockIdx.y * kBlock + threadIdx.x; e < t.w; e += kBlock * gridDim.y)
+      f(e, t.x, col[e]);
   }
 }
 
@@ -214,6 +246,15 @@ struct SpmvDaOp {
   const float* __restrict__ g;
   float* __restrict__ dA;
   __device__ __forceinline__ void operator()(int k, int r, int c) const { dA[k] = g[r] * x[c]; }
+};
+
+// the same product with x[c] read from its slice-major copy: Xs[inv[k]] = x[c]
+struct SpmvDaSlicedOp {
+  const float* __restrict__ xs;
+  const int32_t* __restrict__ inv;
+  const float* __restrict__ g;
+  float* __restrict__ dA;
+  __device__ __forceinline__ void operator()(int k, int r, int) const { dA[k] = g[r] * xs[inv[k]]; }
 };
 
 static int grid_n(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 16384); }
@@ -386,9 +427,15 @@ int gd_wpr_backward(gd_wpr_plan* p, const float* d_x, const float* d_w, float* d
     if (p->fwd.n_tiles)
       hipLaunchKernelGGL(edge_map_kernel<WprGradOp>, dim3(p->fwd.n_tiles), dim3(kBlock), 0,
                          p->stream, p->fwd.tiles.as<Tile>(), p->off, p->col, op);
-    if (p->fwd.n_hub_tiles)
-      hipLaunchKernelGGL(edge_map_kernel<WprGradOp>, dim3(p->fwd.n_hub_tiles), dim3(kBlock), 0,
-                         p->stream, p->fwd.hub_tiles.as<Tile>(), p->off, p->col, op);
+    if (p->fwd.n_hub_tiles) {
+      // one workgroup per 8 edges per thread of the longest hub
+      const int split = std::min<int64_t>(
+          64, std::max<int64_t>(1, ((int64_t)p->fwd.sched.max_row_len + kBlock * 8 - 1) /
+                                       (kBlock * 8)));
+      hipLaunchKernelGGL(edge_map_kernel<WprGradOp>, dim3(p->fwd.n_hub_tiles, split),
+                         dim3(kBlock), 0, p->stream, p->fwd.hub_tiles.as<Tile>(), p->off, p->col,
+                         op);
+    }
     GD_HIP_TRY(hipGetLastError());
   }
   GD_HIP_TRY(hipEventRecord(t1, p->stream));
@@ -418,7 +465,16 @@ struct gd_spmv_bwd_plan {
   // A's values by vperm (two passes through buckets) instead of `at`
   bool sliced = false;
   BucketedPermutation vperm;  // A's values (CSR order) -> the CSC plan's slice-major copy
-  ~gd_spmv_bwd_plan() { gd_spmv_plan_destroy(tplan); }
+  // x larger than one XCD's L2 and A's rows sorted: x gathered in A's
+  // column-sliced order (fplan; slice s on XCD s, so the gathers hit L2), then
+  // dA over A's tiles reads x[c] from that copy (inv[k] = slice-major position
+  // of entry k: eight sequential streams, one per slice)
+  gd_spmv_plan* fplan = nullptr;
+  DeviceBuffer inv, prod;
+  ~gd_spmv_bwd_plan() {
+    gd_spmv_plan_destroy(tplan);
+    gd_spmv_plan_destroy(fplan);
+  }
 };
 
 namespace gd {
@@ -436,6 +492,20 @@ __global__ void bucket_keys_kernel(int rows, const int32_t* __restrict__ off,
     const int base = min(r / width, 7) * nb;
     for (int e = off[r] + lane; e < off[r + 1]; e += 64) keys[e] = base + (col[e] >> cshift);
   }
+}
+
+__global__ void iota_kernel(int64_t n, int32_t* __restrict__ a) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = (int32_t)i;
+}
+
+// inv[fwd[i]] = i
+__global__ void invert_perm_kernel(int64_t n, const int32_t* __restrict__ fwd,
+                                   int32_t* __restrict__ inv) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    inv[fwd[i]] = (int32_t)i;
 }
 
 }  // namespace
@@ -522,6 +592,31 @@ int gd_spmv_bwd_plan_create(int rows, int cols, int64_t nnz, const int32_t* d_Ap
     }
   }
   if (rc) return fail(rc);
+  if (rows > 0 && nnz > 0 && (int64_t)cols * 4 > (4 << 20)) {
+    // the sliced layout's value copy is never read here (`at` holds zeros)
+    rc = gd_spmv_plan_create_ex(rows, cols, nnz, d_Ap, d_Aj, p->at.as<float>(), stream,
+                                GD_SPMV_SLICED, &p->fplan);
+    if (rc == GD_ERR_INVALID) rc = GD_OK;  // unsorted rows: the edge map below
+    if (!rc && p->fplan) {
+      DeviceBuffer ids, kmap;
+      rc = ids.alloc(4 * (size_t)nnz);
+      if (!rc) rc = kmap.alloc(4 * (size_t)nnz);
+      if (!rc) rc = p->inv.alloc(4 * (size_t)nnz);
+      if (!rc) rc = p->prod.alloc(4 * (size_t)nnz);
+      if (!rc) {
+        hipLaunchKernelGGL(iota_kernel, dim3(grid_n(nnz)), dim3(256), 0, p->stream, nnz,
+                           ids.as<int32_t>());
+        rc = spmv_slice_order(p->fplan, ids.as<int32_t>(), kmap.as<int32_t>());
+      }
+      if (!rc) {
+        hipLaunchKernelGGL(invert_perm_kernel, dim3(grid_n(nnz)), dim3(256), 0, p->stream, nnz,
+                           kmap.as<int32_t>(), p->inv.as<int32_t>());
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess)
+          rc = GD_ERR_DEVICE;
+      }
+    }
+    if (rc) return fail(rc);
+  }
   *out = owner.release();
   return GD_OK;
 }
@@ -543,9 +638,17 @@ int gd_spmv_bwd_run(gd_spmv_bwd_plan* p, const float* d_Ax, const float* d_x, co
   }
   if (d_dA && p->nnz > 0) {
     GD_CHECK_ARG(d_x != nullptr, "NULL x");
-    const SpmvDaOp op{d_x, d_g, d_dA};
-    hipLaunchKernelGGL(edge_map_kernel<SpmvDaOp>, dim3(p->n_tiles), dim3(kBlock), 0, p->stream,
-                       p->tiles.as<Tile>(), p->Ap, p->Aj, op);
+    if (p->fplan) {
+      const int rc = spmv_sliced_gather(p->fplan, d_x, p->prod.as<float>());
+      if (rc) return rc;
+      const SpmvDaSlicedOp op{p->prod.as<float>(), p->inv.as<int32_t>(), d_g, d_dA};
+      hipLaunchKernelGGL(edge_map_kernel<SpmvDaSlicedOp>, dim3(p->n_tiles), dim3(kBlock), 0,
+                         p->stream, p->tiles.as<Tile>(), p->Ap, p->Aj, op);
+    } else {
+      const SpmvDaOp op{d_x, d_g, d_dA};
+      hipLaunchKernelGGL(edge_map_kernel<SpmvDaOp>, dim3(p->n_tiles), dim3(kBlock), 0, p->stream,
+                         p->tiles.as<Tile>(), p->Ap, p->Aj, op);
+    }
   }
   GD_HIP_TRY(hipGetLastError());
   return GD_OK;

@@ -135,6 +135,32 @@ __global__ void slice_scatter_kernel(int rows, const int32_t* __restrict__ Ap,
   }
 }
 
+// Xs[i] = x[Aj_s[i]] for every slice-major entry i.  Workgroup b takes slice
+// b % kSlices and its entries [2048 (b / kSlices), + 2048): workgroups are
+// dispatched to the 8 XCDs round-robin, so slice s runs on XCD s, whose L2
+// then holds that 1/8 of x; index loads and stores are coalesced, 8 gathers
+// in flight per thread.
+constexpr int kGatherPT = 8;
+
+__global__ __launch_bounds__(256) void slice_gather_kernel(const int64_t* __restrict__ sbase,
+                                                           const int32_t* __restrict__ Aj_s,
+                                                           const float* __restrict__ x,
+                                                           float* __restrict__ Xs) {
+  const int q = blockIdx.x % kSlices;
+  const int64_t b = sbase[q], e = sbase[q + 1];
+  const int64_t c = b + (int64_t)(blockIdx.x / kSlices) * (256 * kGatherPT);
+  if (c >= e) return;
+  int j[kGatherPT];
+  float v[kGatherPT];
+#pragma unroll
+  for (int t = 0; t < kGatherPT; t++) j[t] = Aj_s[min(c + t * 256 + threadIdx.x, e - 1)];
+#pragma unroll
+  for (int t = 0; t < kGatherPT; t++) v[t] = x[j[t]];
+#pragma unroll
+  for (int t = 0; t < kGatherPT; t++)
+    if (c + t * 256 + threadIdx.x < e) Xs[c + t * 256 + threadIdx.x] = v[t];
+}
+
 }  // namespace gd
 
 using namespace gd;
@@ -400,6 +426,20 @@ int gd::spmv_slice_order(gd_spmv_plan* p, const int32_t* d_in, int32_t* d_out) {
   hipLaunchKernelGGL(slice_scatter_kernel<int32_t>, dim3(g), dim3(256), 0, p->stream, p->rows,
                      p->Ap, p->Aj, d_in, p->width, p->soff.as<int32_t>(), p->sbase.as<int64_t>(),
                      nullptr, d_out, 1);
+  GD_HIP_TRY(hipGetLastError());
+  return GD_OK;
+}
+
+int gd::spmv_sliced_gather(gd_spmv_plan* p, const float* d_x, float* d_Xs) {
+  GD_CHECK_ARG(p != nullptr && p->sliced, "not a sliced SpMV plan");
+  if (p->nnz == 0) return GD_OK;
+  GD_CHECK_ARG(d_x && d_Xs, "NULL array");
+  int64_t most = 0;
+  for (int q = 0; q < kSlices; q++)
+    most = std::max<int64_t>(most, p->slice_base[q + 1] - p->slice_base[q]);
+  const int64_t chunks = (most + 256 * kGatherPT - 1) / (256 * kGatherPT);
+  hipLaunchKernelGGL(slice_gather_kernel, dim3((unsigned)(kSlices * chunks)), dim3(256), 0,
+                     p->stream, p->sbase.as<int64_t>(), p->Aj_s.as<int32_t>(), d_x, d_Xs);
   GD_HIP_TRY(hipGetLastError());
   return GD_OK;
 }

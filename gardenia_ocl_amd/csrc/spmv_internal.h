@@ -19,6 +19,10 @@ float* spmv_sliced_values(gd_spmv_plan* p);
 // sliced plan (any 4-byte payload; async on the plan's stream).
 int spmv_slice_order(gd_spmv_plan* p, const int32_t* d_in, int32_t* d_out);
 
+// For a sliced plan: d_Xs[i] = d_x[column of i] for every slice-major
+// position i, each slice's gathers on one XCD (async on the plan's stream).
+int spmv_sliced_gather(gd_spmv_plan* p, const float* d_x, float* d_Xs);
+
 }  // namespace gd
 
 #endif  // GD_SPMV_INTERNAL_H_
