@@ -468,9 +468,11 @@ struct gd_spmv_bwd_plan {
   // x larger than one XCD's L2 and A's rows sorted: x gathered in A's
   // column-sliced order (fplan; slice s on XCD s, so the gathers hit L2), then
   // dA over A's tiles reads x[c] from that copy (inv[k] = slice-major position
-  // of entry k: eight sequential streams, one per slice)
+  // of entry k: eight sequential streams, one per slice).  The copy lives in
+  // fplan's slice-major value array, which fplan itself never reads (it is
+  // never run as an SpMV).
   gd_spmv_plan* fplan = nullptr;
-  DeviceBuffer inv, prod;
+  DeviceBuffer inv;
   ~gd_spmv_bwd_plan() {
     gd_spmv_plan_destroy(tplan);
     gd_spmv_plan_destroy(fplan);
@@ -593,7 +595,7 @@ int gd_spmv_bwd_plan_create(int rows, int cols, int64_t nnz, const int32_t* d_Ap
   }
   if (rc) return fail(rc);
   if (rows > 0 && nnz > 0 && (int64_t)cols * 4 > (4 << 20)) {
-    // the sliced layout's value copy is never read here (`at` holds zeros)
+    // the sliced layout's value copy (of `at`, zeros) becomes the x copy
     rc = gd_spmv_plan_create_ex(rows, cols, nnz, d_Ap, d_Aj, p->at.as<float>(), stream,
                                 GD_SPMV_SLICED, &p->fplan);
     if (rc == GD_ERR_INVALID) rc = GD_OK;  // unsorted rows: the edge map below
@@ -602,7 +604,6 @@ int gd_spmv_bwd_plan_create(int rows, int cols, int64_t nnz, const int32_t* d_Ap
       rc = ids.alloc(4 * (size_t)nnz);
       if (!rc) rc = kmap.alloc(4 * (size_t)nnz);
       if (!rc) rc = p->inv.alloc(4 * (size_t)nnz);
-      if (!rc) rc = p->prod.alloc(4 * (size_t)nnz);
       if (!rc) {
         hipLaunchKernelGGL(iota_kernel, dim3(grid_n(nnz)), dim3(256), 0, p->stream, nnz,
                            ids.as<int32_t>());
@@ -639,9 +640,10 @@ int gd_spmv_bwd_run(gd_spmv_bwd_plan* p, const float* d_Ax, const float* d_x, co
   if (d_dA && p->nnz > 0) {
     GD_CHECK_ARG(d_x != nullptr, "NULL x");
     if (p->fplan) {
-      const int rc = spmv_sliced_gather(p->fplan, d_x, p->prod.as<float>());
+      float* xs = spmv_sliced_values(p->fplan);
+      const int rc = spmv_sliced_gather(p->fplan, d_x, xs);
       if (rc) return rc;
-      const SpmvDaSlicedOp op{p->prod.as<float>(), p->inv.as<int32_t>(), d_g, d_dA};
+      const SpmvDaSlicedOp op{xs, p->inv.as<int32_t>(), d_g, d_dA};
       hipLaunchKernelGGL(edge_map_kernel<SpmvDaSlicedOp>, dim3(p->n_tiles), dim3(kBlock), 0,
                          p->stream, p->tiles.as<Tile>(), p->Ap, p->Aj, op);
     } else {
