@@ -826,7 +826,7 @@ def main():
                     help="only copy 1 GiB device-to-device 5 times (scales rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     ap.add_argument("--only", choices=["symgs", "csr_build", "spmv", "small", "ppr", "adjoint",
-                                       "multigraph"],
+                                       "multigraph", "push"],
                     default=None,
                     help="run just this secondary figure and print its JSON (for profiling)")
     ap.add_argument("--tol-run", type=float, default=1e-6,
@@ -863,6 +863,19 @@ def main():
     if args.only == "multigraph":
         print(json.dumps({"multigraph": bench_multigraph(64, 12, 18, 1e-4, args.seed,
                                                          args.tune_cache)}), flush=True)
+        return
+    if args.only == "push":
+        out_csr, in_csr = graph.rmat(args.scale, 16, seed=args.seed)
+        off = torch.from_numpy(in_csr.row_offsets).to(dev)
+        col = torch.from_numpy(in_csr.column_indices).to(dev)
+        deg = torch.from_numpy(out_csr.degree).to(dev)
+        s = torch.full((in_csr.m,), float(np.float32(1.0) / np.float32(in_csr.m)),
+                       dtype=torch.float32, device=dev)
+        PageRankPlan(off, col, deg).run(s, tol=0.0, max_iter=args.iters)
+        scores_h = s.cpu().numpy()
+        del off, col, deg, s
+        torch.cuda.empty_cache()
+        print(json.dumps({"push": bench_push(out_csr, args.iters, scores_h)}), flush=True)
         return
     if args.only == "csr_build":
         t0 = time.perf_counter()

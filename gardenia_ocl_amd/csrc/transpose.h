@@ -8,6 +8,21 @@
 
 namespace gd {
 
+// Workgroups are dispatched to the 8 XCDs round-robin.  With a grid that is a
+// multiple of 8, workgroup b taking chunk xcd_chunk() makes XCD x walk the
+// x-th eighth of the chunks in order, so the data in flight on one XCD stays
+// in one narrow window.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int64_t xcd_chunk() {
+  const int b = blockIdx.x, per_xcd = gridDim.x / kXcds;
+  return (int64_t)(b % kXcds) * per_xcd + b / kXcds;
+}
+
+// row_of_edge[e] = the row r with off[r] <= e < off[r + 1], a wavefront per
+// row (async on stream).
+void edge_rows(int m, const int32_t* d_off, int32_t* d_row_of_edge, hipStream_t stream);
+
 struct Transpose {
   DeviceBuffer off;   // [cols+1] CSC column offsets
   DeviceBuffer row;   // [nnz] row index of each CSC entry (increasing within a column)

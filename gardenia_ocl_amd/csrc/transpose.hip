@@ -56,6 +56,11 @@ __global__ void fill_int_kernel(int32_t* __restrict__ a, int n, int v) {
 
 static int grid64(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 16384); }
 
+void edge_rows(int m, const int32_t* d_off, int32_t* d_row_of_edge, hipStream_t stream) {
+  hipLaunchKernelGGL(edge_rows_kernel, dim3(grid64(m / 4 + 1)), dim3(256), 0, stream, m, d_off,
+                     d_row_of_edge);
+}
+
 int build_transpose(int rows, int cols, int64_t nnz, const int32_t* d_off, const int32_t* d_col,
                     hipStream_t stream, Transpose* t) {
   int rc = t->off.alloc(4 * ((size_t)cols + 1));
@@ -103,19 +108,12 @@ int build_transpose(int rows, int cols, int64_t nnz, const int32_t* d_off, const
 }
 
 // Chunks of kPermChunk positions, kPermU gathers in flight per thread.
-// Workgroups are dispatched to the 8 XCDs round-robin; workgroup b takes
-// chunk xcd_chunk(b), so XCD x walks the x-th eighth of the chunks in order
-// and the positions in flight on one XCD stay in one narrow window (of the
-// destination for a gather, of the source for a scatter).  The grid is a
-// multiple of 8 (G = 8 * ceil(chunks / 8)); chunks past the end are skipped.
-constexpr int kXcds = 8;
+// Workgroup b takes chunk xcd_chunk(), so the positions in flight on one XCD
+// stay in one narrow window (of the destination for a gather, of the source
+// for a scatter).  The grid is a multiple of 8 (G = 8 * ceil(chunks / 8));
+// chunks past the end are skipped.
 constexpr int kPermU = 8;
 constexpr int kPermChunk = 256 * kPermU;
-
-__device__ __forceinline__ int64_t xcd_chunk() {
-  const int b = blockIdx.x, per_xcd = gridDim.x / kXcds;
-  return (int64_t)(b % kXcds) * per_xcd + b / kXcds;
-}
 
 static int xcd_grid(int64_t n) {
   const int64_t chunks = (n + kPermChunk - 1) / kPermChunk;

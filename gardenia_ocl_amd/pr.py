@@ -103,7 +103,9 @@ def pagerank_push(out_row_offsets, out_column_indices, scores, *, damping=K_DAMP
                   max_iter=MAX_ITER, stream=None):
     """Push-style PageRank over the out-CSR (torch int32 CUDA tensors; scores
     float32, in/out): the reference's scatter variant (src/pr/scatter.cu:11-41),
-    FP32 atomic adds in the hardware's order, a cross-check of the pull path.
+    propagation-blocked (each edge's value binned by destination window, summed
+    per window in LDS with FP32 atomic adds in the hardware's order), a
+    cross-check of the pull path.
     Continues while the L1 change is > tol, at least once.  Returns (stats
     dict, per-iteration L1 changes)."""
     import torch

@@ -177,12 +177,16 @@ int gd_device_csr_free(gd_device_csr *c);
 /* Push-style PageRank over the OUT-CSR: the reference's scatter variant
  * (src/pr/scatter.cu:11-41 and its host loop :57-78), a cross-check of the pull
  * path.  Every iteration sets next[v] = (1-d)/m, adds kDamp*score[u]/outdeg[u]
- * (outdeg[u] = u's out-CSR row length) over every out-edge (u, v) with hardware
- * FP32 atomic adds (addition order: the hardware's, as the reference's
- * atomicAdd), sums |next - score| in FP64 in a fixed order, and sets
- * score = next; it continues while that change is > tol (at least one
- * iteration, the reference's do-while).  Scores agree with the pull path to
- * FP32 summation-order noise.  d_scores (float[m], HBM) holds the initial
+ * (outdeg[u] = u's out-CSR row length) over every out-edge (u, v), sums
+ * |next - score| in FP64 in a fixed order, and sets score = next; it continues
+ * while that change is > tol (at least one iteration, the reference's
+ * do-while).  The additions are propagation-blocked: each edge's value is
+ * binned by destination window (32,768 vertices) and summed per window in LDS
+ * with FP32 atomic adds (addition order: the hardware's, as with the
+ * reference's atomicAdd), so scores agree with the pull path to FP32
+ * summation-order noise.  Each call first lays out the edges by window (a
+ * stable radix sort; not in stats->solve_ms) and holds device memory of about
+ * 22 bytes per edge while it does so, 14 during the iterations.  d_scores (float[m], HBM) holds the initial
  * scores and receives the result; opts->seed_vertex must be -1; errors (host,
  * room for max(1, max_iter) values, may be NULL) receives each iteration's
  * change.  The out-CSR is validated on the device first. */
