@@ -20,6 +20,8 @@
 // (out-CSR order inside a window).  Per iteration:
 //   1. bin: every edge writes value[u] into its slot.  Workgroups walk the
 //      out-CSR in XCD-contiguous chunks, value[u] is read in source order;
+//      a chunk's values are staged in LDS and stored in slot order, so the
+//      stores into one window are runs of consecutive slots;
 //   2. accumulate: a workgroup per piece of a window (at most kPieceEdges
 //      slots) adds the piece's values into the window in LDS (ds_add_f32)
 //      and stores the window's partial sums;
@@ -49,8 +51,9 @@ constexpr int kWin = 1 << kWinBits;   // destination window: 32,768 vertices, 12
 constexpr int kPieceEdges = 1 << 18;  // slots per accumulating workgroup at most
 constexpr int kAccBlock = 1024;
 constexpr int kAccU = 4;              // slots in flight per thread of the accumulation
-constexpr int kBinU = 8;              // edges per thread of the binning pass
+constexpr int kBinU = 8;              // edges per thread of the direct binning pass
 constexpr int kBinChunk = kPushBlock * kBinU;
+constexpr int kSortChunk = 4096;      // edges per workgroup of the chunk-sorted binning pass
 
 // keys[e] = destination window of edge e, ids[e] = e (once per call).
 __global__ void window_keys_kernel(int64_t nnz, const int32_t* __restrict__ col,
@@ -118,6 +121,58 @@ __global__ __launch_bounds__(kPushBlock) void push_bin_kernel(int64_t nnz,
 #pragma unroll
   for (int t = 0; t < kBinU; t++)
     if (c + t * kPushBlock + threadIdx.x < nnz) binval[at[t]] = x[t];
+}
+
+// 1'. bin, chunk-sorted: the chunk's values are staged in LDS in edge order,
+// then stored in slot order (lidx[t]: chunk-local edge of the chunk's t-th
+// slot, sslot[t]: that slot), so the stores into one window come out as runs
+// of consecutive slots.
+__global__ __launch_bounds__(kPushBlock) void push_bin_sorted_kernel(
+    int64_t nnz, const int32_t* __restrict__ src, const uint16_t* __restrict__ lidx,
+    const int32_t* __restrict__ sslot, const float* __restrict__ value,
+    float* __restrict__ binval) {
+  __shared__ float s_v[kSortChunk];
+  const int64_t c = xcd_chunk() * kSortChunk;
+  if (c >= nnz) return;
+  const int n = (int)min<int64_t>(kSortChunk, nnz - c);
+  for (int q0 = 0; q0 < kSortChunk; q0 += kPushBlock * kBinU) {
+    int u[kBinU];
+#pragma unroll
+    for (int t = 0; t < kBinU; t++) u[t] = src[c + min(q0 + t * kPushBlock + (int)threadIdx.x, n - 1)];
+#pragma unroll
+    for (int t = 0; t < kBinU; t++) {
+      const int k = q0 + t * kPushBlock + threadIdx.x;
+      if (k < n) s_v[k] = value[u[t]];
+    }
+  }
+  __syncthreads();
+  for (int q0 = 0; q0 < kSortChunk; q0 += kPushBlock * kBinU) {
+#pragma unroll
+    for (int t = 0; t < kBinU; t++) {
+      const int k = q0 + t * kPushBlock + threadIdx.x;
+      if (k < n) binval[sslot[c + k]] = s_v[lidx[c + k]];
+    }
+  }
+}
+
+// keys2[e] = (e's sort chunk) << wbits | e's window (once per call)
+__global__ void chunk_window_keys_kernel(int64_t nnz, const int32_t* __restrict__ col, int wbits,
+                                         int32_t* __restrict__ keys2) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
+       e += (int64_t)gridDim.x * blockDim.x)
+    keys2[e] = (int32_t)(((e / kSortChunk) << wbits) | (col[e] >> kWinBits));
+}
+
+// lidx[t] = order[t] - its chunk's start, sslot[t] = slot[order[t]]
+__global__ void chunk_slots_kernel(int64_t nnz, const int32_t* __restrict__ order,
+                                   const int32_t* __restrict__ slot, uint16_t* __restrict__ lidx,
+                                   int32_t* __restrict__ sslot) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nnz;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t e = order[t];
+    lidx[t] = (uint16_t)(e % kSortChunk);
+    sslot[t] = slot[e];
+  }
 }
 
 // 2. accumulate: piece p = (window, first slot, end slot) -> part[p][0, kWin).
@@ -195,10 +250,12 @@ __global__ __launch_bounds__(kPushBlock) void push_total_kernel(int n, const dou
   }
 }
 
-// The window layout of an out-CSR (once per call): slot, low, the pieces and
-// each window's first piece.  Synchronises `s`.
+// The window layout of an out-CSR (once per call): the slots (chunk-sorted:
+// lidx + sslot; or slot, for a graph whose (chunk, window) key exceeds 31
+// bits), low, the pieces and each window's first piece.  Synchronises `s`.
 struct WindowLayout {
-  DeviceBuffer slot, low, pieces, pfirst;
+  DeviceBuffer slot, lidx, sslot, low, pieces, pfirst;
+  bool sorted = false;
   int n_pieces = 0;
 };
 
@@ -238,7 +295,35 @@ int build_windows(int m, int64_t nnz, const int32_t* col, hipStream_t s, WindowL
     GD_HIP_TRY(hipGetLastError());
     GD_HIP_TRY(hipMemcpyAsync(start.data(), wstart.ptr, 4 * ((size_t)windows + 1),
                               hipMemcpyDeviceToHost, s));
+    int cbits = 1;
+    while ((1ll << cbits) < (nnz + kSortChunk - 1) / kSortChunk) cbits++;
+    if (cbits + bits <= 31) {
+      // each chunk's edges in slot order: a stable sort by (chunk, window)
+      // (slots increase with the edge inside a window)
+      rc = L->lidx.alloc(2 * (size_t)nnz);
+      if (!rc) rc = L->sslot.alloc(4 * (size_t)nnz);
+      if (rc) return rc;
+      hipLaunchKernelGGL(chunk_window_keys_kernel, dim3(g), dim3(256), 0, s, nnz, col, bits,
+                         keys.as<int32_t>());
+      GD_HIP_TRY(hipGetLastError());
+      size_t bytes2 = 0;
+      GD_HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes2, keys.as<int32_t>(),
+                                           keys_out.as<int32_t>(), ids.as<int32_t>(),
+                                           order.as<int32_t>(), (size_t)nnz, 0, cbits + bits, s));
+      if (bytes2 > bytes) {
+        rc = tmp.alloc(bytes2);
+        if (rc) return rc;
+      }
+      GD_HIP_TRY(rocprim::radix_sort_pairs(tmp.ptr, bytes2, keys.as<int32_t>(),
+                                           keys_out.as<int32_t>(), ids.as<int32_t>(),
+                                           order.as<int32_t>(), (size_t)nnz, 0, cbits + bits, s));
+      hipLaunchKernelGGL(chunk_slots_kernel, dim3(g), dim3(256), 0, s, nnz, order.as<int32_t>(),
+                         L->slot.as<int32_t>(), L->lidx.as<uint16_t>(), L->sslot.as<int32_t>());
+      GD_HIP_TRY(hipGetLastError());
+      L->sorted = true;
+    }
     GD_HIP_TRY(hipStreamSynchronize(s));
+    if (L->sorted) L->slot.release();
   }
   // pieces: each window's slots in ceil(n / kPieceEdges) near-equal parts
   std::vector<int4> pieces;
@@ -312,6 +397,8 @@ extern "C" int gd_pr_push_solve(int m, int64_t nnz, const int32_t* d_out_row_off
   const int ugrid = std::max(1, std::min(kPushParts, (m + kPushBlock - 1) / kPushBlock));
   const int64_t chunks = (nnz + kBinChunk - 1) / kBinChunk;
   const int bgrid = (int)(kXcds * ((chunks + kXcds - 1) / kXcds));  // a multiple of 8
+  const int64_t schunks = (nnz + kSortChunk - 1) / kSortChunk;
+  const int sgrid = (int)(kXcds * ((schunks + kXcds - 1) / kXcds));
   const float damp = o.damping;
   const float base = (1.0f - damp) / (float)m;  // scatter.cu:57
   EventPair ev;
@@ -327,9 +414,14 @@ extern "C" int gd_pr_push_solve(int m, int64_t nnz, const int32_t* d_out_row_off
   do {  // :66-78
     ++iter;
     if (nnz > 0) {
-      hipLaunchKernelGGL(push_bin_kernel, dim3(bgrid), dim3(kPushBlock), 0, s, nnz,
-                         src.as<int32_t>(), L.slot.as<int32_t>(), value.as<float>(),
-                         binval.as<float>());
+      if (L.sorted)
+        hipLaunchKernelGGL(push_bin_sorted_kernel, dim3(sgrid), dim3(kPushBlock), 0, s, nnz,
+                           src.as<int32_t>(), L.lidx.as<uint16_t>(), L.sslot.as<int32_t>(),
+                           value.as<float>(), binval.as<float>());
+      else
+        hipLaunchKernelGGL(push_bin_kernel, dim3(bgrid), dim3(kPushBlock), 0, s, nnz,
+                           src.as<int32_t>(), L.slot.as<int32_t>(), value.as<float>(),
+                           binval.as<float>());
       hipLaunchKernelGGL(push_window_kernel, dim3(L.n_pieces), dim3(kAccBlock), 0, s,
                          L.pieces.as<int4>(), L.low.as<uint16_t>(), binval.as<float>(),
                          part.as<float>());
