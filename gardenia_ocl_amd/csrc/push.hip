@@ -51,9 +51,8 @@ constexpr int kWin = 1 << kWinBits;   // destination window: 32,768 vertices, 12
 constexpr int kPieceEdges = 1 << 18;  // slots per accumulating workgroup at most
 constexpr int kAccBlock = 1024;
 constexpr int kAccU = 4;              // slots in flight per thread of the accumulation
-constexpr int kBinU = 8;              // edges per thread of the direct binning pass
-constexpr int kBinChunk = kPushBlock * kBinU;
-constexpr int kSortChunk = 4096;      // edges per workgroup of the chunk-sorted binning pass
+constexpr int kBinU = 8;              // edges per thread in flight in the binning pass
+constexpr int kSortChunk = 4096;      // edges per workgroup of the binning pass
 
 // keys[e] = destination window of edge e, ids[e] = e (once per call).
 __global__ void window_keys_kernel(int64_t nnz, const int32_t* __restrict__ col,
@@ -100,34 +99,12 @@ __global__ void push_prepare_kernel(int m, const int32_t* __restrict__ off,
     value[v] = damp * scores[v] / (float)(off[v + 1] - off[v]);
 }
 
-// 1. bin: binval[slot[e]] = value[src[e]], kBinU edges per thread in flight.
-__global__ __launch_bounds__(kPushBlock) void push_bin_kernel(int64_t nnz,
-                                                              const int32_t* __restrict__ src,
-                                                              const int32_t* __restrict__ slot,
-                                                              const float* __restrict__ value,
-                                                              float* __restrict__ binval) {
-  const int64_t c = xcd_chunk() * kBinChunk;
-  if (c >= nnz) return;
-  int u[kBinU], at[kBinU];
-#pragma unroll
-  for (int t = 0; t < kBinU; t++) {
-    const int64_t e = min(c + t * kPushBlock + threadIdx.x, nnz - 1);
-    u[t] = src[e];
-    at[t] = slot[e];
-  }
-  float x[kBinU];
-#pragma unroll
-  for (int t = 0; t < kBinU; t++) x[t] = value[u[t]];
-#pragma unroll
-  for (int t = 0; t < kBinU; t++)
-    if (c + t * kPushBlock + threadIdx.x < nnz) binval[at[t]] = x[t];
-}
-
-// 1'. bin, chunk-sorted: the chunk's values are staged in LDS in edge order,
-// then stored in slot order (lidx[t]: chunk-local edge of the chunk's t-th
-// slot, sslot[t]: that slot), so the stores into one window come out as runs
-// of consecutive slots.
-__global__ __launch_bounds__(kPushBlock) void push_bin_sorted_kernel(
+// 1. bin: binval[slot[e]] = value[src[e]] over a chunk of kSortChunk edges.
+// The chunk's values are staged in LDS in edge order, then stored in slot
+// order (lidx[t]: chunk-local edge of the chunk's t-th slot, sslot[t]: that
+// slot), so the stores into one window come out as runs of consecutive
+// slots.
+__global__ __launch_bounds__(kPushBlock) void push_bin_kernel(
     int64_t nnz, const int32_t* __restrict__ src, const uint16_t* __restrict__ lidx,
     const int32_t* __restrict__ sslot, const float* __restrict__ value,
     float* __restrict__ binval) {
@@ -138,7 +115,8 @@ __global__ __launch_bounds__(kPushBlock) void push_bin_sorted_kernel(
   for (int q0 = 0; q0 < kSortChunk; q0 += kPushBlock * kBinU) {
     int u[kBinU];
 #pragma unroll
-    for (int t = 0; t < kBinU; t++) u[t] = src[c + min(q0 + t * kPushBlock + (int)threadIdx.x, n - 1)];
+    for (int t = 0; t < kBinU; t++)
+      u[t] = src[c + min(q0 + t * kPushBlock + (int)threadIdx.x, n - 1)];
 #pragma unroll
     for (int t = 0; t < kBinU; t++) {
       const int k = q0 + t * kPushBlock + threadIdx.x;
@@ -157,10 +135,10 @@ __global__ __launch_bounds__(kPushBlock) void push_bin_sorted_kernel(
 
 // keys2[e] = (e's sort chunk) << wbits | e's window (once per call)
 __global__ void chunk_window_keys_kernel(int64_t nnz, const int32_t* __restrict__ col, int wbits,
-                                         int32_t* __restrict__ keys2) {
+                                         int64_t* __restrict__ keys2) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
        e += (int64_t)gridDim.x * blockDim.x)
-    keys2[e] = (int32_t)(((e / kSortChunk) << wbits) | (col[e] >> kWinBits));
+    keys2[e] = ((e / kSortChunk) << wbits) | (col[e] >> kWinBits);
 }
 
 // lidx[t] = order[t] - its chunk's start, sslot[t] = slot[order[t]]
@@ -250,12 +228,11 @@ __global__ __launch_bounds__(kPushBlock) void push_total_kernel(int n, const dou
   }
 }
 
-// The window layout of an out-CSR (once per call): the slots (chunk-sorted:
-// lidx + sslot; or slot, for a graph whose (chunk, window) key exceeds 31
-// bits), low, the pieces and each window's first piece.  Synchronises `s`.
+// The window layout of an out-CSR (once per call): each chunk's slots in
+// order (lidx + sslot), low, the pieces and each window's first piece.
+// Synchronises `s`.
 struct WindowLayout {
-  DeviceBuffer slot, lidx, sslot, low, pieces, pfirst;
-  bool sorted = false;
+  DeviceBuffer lidx, sslot, low, pieces, pfirst;
   int n_pieces = 0;
 };
 
@@ -263,13 +240,13 @@ int build_windows(int m, int64_t nnz, const int32_t* col, hipStream_t s, WindowL
   const int windows = (int)(((int64_t)m + kWin - 1) >> kWinBits);
   std::vector<int32_t> start(windows + 1, 0);
   if (nnz > 0) {
-    DeviceBuffer keys, keys_out, ids, order, wstart, tmp;
+    DeviceBuffer keys, keys_out, ids, order, wstart, slot, tmp;
     int rc = keys.alloc(4 * (size_t)nnz);
     if (!rc) rc = keys_out.alloc(4 * (size_t)nnz);
     if (!rc) rc = ids.alloc(4 * (size_t)nnz);
     if (!rc) rc = order.alloc(4 * (size_t)nnz);
     if (!rc) rc = wstart.alloc(4 * ((size_t)windows + 1));
-    if (!rc) rc = L->slot.alloc(4 * (size_t)nnz);
+    if (!rc) rc = slot.alloc(4 * (size_t)nnz);
     if (!rc) rc = L->low.alloc(2 * (size_t)nnz);
     if (rc) return rc;
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>(65536, (nnz + 255) / 256));
@@ -289,7 +266,7 @@ int build_windows(int m, int64_t nnz, const int32_t* col, hipStream_t s, WindowL
                                          keys_out.as<int32_t>(), ids.as<int32_t>(),
                                          order.as<int32_t>(), (size_t)nnz, 0, bits, s));
     hipLaunchKernelGGL(window_layout_kernel, dim3(g), dim3(256), 0, s, nnz, order.as<int32_t>(),
-                       col, L->slot.as<int32_t>(), L->low.as<uint16_t>());
+                       col, slot.as<int32_t>(), L->low.as<uint16_t>());
     hipLaunchKernelGGL(window_starts_kernel, dim3(g), dim3(256), 0, s, nnz,
                        keys_out.as<int32_t>(), windows, wstart.as<int32_t>());
     GD_HIP_TRY(hipGetLastError());
@@ -297,33 +274,34 @@ int build_windows(int m, int64_t nnz, const int32_t* col, hipStream_t s, WindowL
                               hipMemcpyDeviceToHost, s));
     int cbits = 1;
     while ((1ll << cbits) < (nnz + kSortChunk - 1) / kSortChunk) cbits++;
-    if (cbits + bits <= 31) {
-      // each chunk's edges in slot order: a stable sort by (chunk, window)
-      // (slots increase with the edge inside a window)
-      rc = L->lidx.alloc(2 * (size_t)nnz);
-      if (!rc) rc = L->sslot.alloc(4 * (size_t)nnz);
+    // each chunk's edges in slot order: a stable sort by (chunk, window)
+    // (slots increase with the edge inside a window)
+    keys.release();
+    keys_out.release();
+    DeviceBuffer keys2, keys2_out;
+    rc = keys2.alloc(8 * (size_t)nnz);
+    if (!rc) rc = keys2_out.alloc(8 * (size_t)nnz);
+    if (!rc) rc = L->lidx.alloc(2 * (size_t)nnz);
+    if (!rc) rc = L->sslot.alloc(4 * (size_t)nnz);
+    if (rc) return rc;
+    hipLaunchKernelGGL(chunk_window_keys_kernel, dim3(g), dim3(256), 0, s, nnz, col, bits,
+                       keys2.as<int64_t>());
+    GD_HIP_TRY(hipGetLastError());
+    size_t bytes2 = 0;
+    GD_HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes2, keys2.as<int64_t>(),
+                                         keys2_out.as<int64_t>(), ids.as<int32_t>(),
+                                         order.as<int32_t>(), (size_t)nnz, 0, cbits + bits, s));
+    if (bytes2 > bytes) {
+      rc = tmp.alloc(bytes2);
       if (rc) return rc;
-      hipLaunchKernelGGL(chunk_window_keys_kernel, dim3(g), dim3(256), 0, s, nnz, col, bits,
-                         keys.as<int32_t>());
-      GD_HIP_TRY(hipGetLastError());
-      size_t bytes2 = 0;
-      GD_HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes2, keys.as<int32_t>(),
-                                           keys_out.as<int32_t>(), ids.as<int32_t>(),
-                                           order.as<int32_t>(), (size_t)nnz, 0, cbits + bits, s));
-      if (bytes2 > bytes) {
-        rc = tmp.alloc(bytes2);
-        if (rc) return rc;
-      }
-      GD_HIP_TRY(rocprim::radix_sort_pairs(tmp.ptr, bytes2, keys.as<int32_t>(),
-                                           keys_out.as<int32_t>(), ids.as<int32_t>(),
-                                           order.as<int32_t>(), (size_t)nnz, 0, cbits + bits, s));
-      hipLaunchKernelGGL(chunk_slots_kernel, dim3(g), dim3(256), 0, s, nnz, order.as<int32_t>(),
-                         L->slot.as<int32_t>(), L->lidx.as<uint16_t>(), L->sslot.as<int32_t>());
-      GD_HIP_TRY(hipGetLastError());
-      L->sorted = true;
     }
+    GD_HIP_TRY(rocprim::radix_sort_pairs(tmp.ptr, bytes2, keys2.as<int64_t>(),
+                                         keys2_out.as<int64_t>(), ids.as<int32_t>(),
+                                         order.as<int32_t>(), (size_t)nnz, 0, cbits + bits, s));
+    hipLaunchKernelGGL(chunk_slots_kernel, dim3(g), dim3(256), 0, s, nnz, order.as<int32_t>(),
+                       slot.as<int32_t>(), L->lidx.as<uint16_t>(), L->sslot.as<int32_t>());
+    GD_HIP_TRY(hipGetLastError());
     GD_HIP_TRY(hipStreamSynchronize(s));
-    if (L->sorted) L->slot.release();
   }
   // pieces: each window's slots in ceil(n / kPieceEdges) near-equal parts
   std::vector<int4> pieces;
@@ -395,10 +373,8 @@ extern "C" int gd_pr_push_solve(int m, int64_t nnz, const int32_t* d_out_row_off
   if (rc) return rc;
   const int vgrid = std::max(1, std::min(16384, (m + kPushBlock - 1) / kPushBlock));
   const int ugrid = std::max(1, std::min(kPushParts, (m + kPushBlock - 1) / kPushBlock));
-  const int64_t chunks = (nnz + kBinChunk - 1) / kBinChunk;
-  const int bgrid = (int)(kXcds * ((chunks + kXcds - 1) / kXcds));  // a multiple of 8
   const int64_t schunks = (nnz + kSortChunk - 1) / kSortChunk;
-  const int sgrid = (int)(kXcds * ((schunks + kXcds - 1) / kXcds));
+  const int sgrid = (int)(kXcds * ((schunks + kXcds - 1) / kXcds));  // a multiple of 8
   const float damp = o.damping;
   const float base = (1.0f - damp) / (float)m;  // scatter.cu:57
   EventPair ev;
@@ -414,14 +390,9 @@ extern "C" int gd_pr_push_solve(int m, int64_t nnz, const int32_t* d_out_row_off
   do {  // :66-78
     ++iter;
     if (nnz > 0) {
-      if (L.sorted)
-        hipLaunchKernelGGL(push_bin_sorted_kernel, dim3(sgrid), dim3(kPushBlock), 0, s, nnz,
-                           src.as<int32_t>(), L.lidx.as<uint16_t>(), L.sslot.as<int32_t>(),
-                           value.as<float>(), binval.as<float>());
-      else
-        hipLaunchKernelGGL(push_bin_kernel, dim3(bgrid), dim3(kPushBlock), 0, s, nnz,
-                           src.as<int32_t>(), L.slot.as<int32_t>(), value.as<float>(),
-                           binval.as<float>());
+      hipLaunchKernelGGL(push_bin_kernel, dim3(sgrid), dim3(kPushBlock), 0, s, nnz,
+                         src.as<int32_t>(), L.lidx.as<uint16_t>(), L.sslot.as<int32_t>(),
+                         value.as<float>(), binval.as<float>());
       hipLaunchKernelGGL(push_window_kernel, dim3(L.n_pieces), dim3(kAccBlock), 0, s,
                          L.pieces.as<int4>(), L.low.as<uint16_t>(), binval.as<float>(),
                          part.as<float>());

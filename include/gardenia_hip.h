@@ -186,7 +186,7 @@ int gd_device_csr_free(gd_device_csr *c);
  * reference's atomicAdd), so scores agree with the pull path to FP32
  * summation-order noise.  Each call first lays out the edges by window (a
  * stable radix sort; not in stats->solve_ms) and holds device memory of about
- * 30 bytes per edge while it does so, 16 during the iterations.  d_scores (float[m], HBM) holds the initial
+ * 36 bytes per edge while it does so, 16 during the iterations.  d_scores (float[m], HBM) holds the initial
  * scores and receives the result; opts->seed_vertex must be -1; errors (host,
  * room for max(1, max_iter) values, may be NULL) receives each iteration's
  * change.  The out-CSR is validated on the device first. */
