@@ -93,3 +93,36 @@ def test_push_rejects_seed_vertex_before_device():
                                   ctypes.byref(st), None)
     assert rc == N.GD_ERR_INVALID
     assert b"seed_vertex" in N.lib().gd_last_error()
+
+
+@pytest.mark.gpu
+def test_push_window_edge_cases(device):
+    """The additions are summed per destination window of 32,768 vertices: an
+    empty window between full ones, a partial last window, a window whose
+    edges (more than 2^18) are split over several workgroups, and a graph
+    without edges; scores and per-iteration L1 against the oracle's push."""
+    rng = np.random.default_rng(5)
+    W = 1 << 15
+    m = 3 * W + 777
+    n = 600_000
+    src = rng.integers(0, m, n)
+    # no destination in window 1; about 400 K edges into window 0
+    dst = np.where(rng.random(n) < 0.67, rng.integers(0, W, n), rng.integers(2 * W, m, n))
+    o = graph.csr_from_edges(m, src, dst)
+    assert np.sum(o.column_indices < W) > (1 << 18)
+    assert not np.any((o.column_indices >= W) & (o.column_indices < 2 * W))
+    s, st, errs = _run(device, o, 0.0, 3)
+    ref, it, ref_errs = oracle.pr_push(o.m, o.row_offsets, o.column_indices, tol=0.0, max_iter=3)
+    assert st["iterations"] == it == 3
+    np.testing.assert_allclose(s, ref, rtol=2e-6, atol=1e-12)
+    np.testing.assert_allclose(errs, ref_errs, rtol=1e-5, atol=1e-9)
+    # window 1 receives nothing: base only
+    base = (np.float32(1.0) - np.float32(0.85)) / np.float32(m)  # scatter.cu:57
+    assert np.all(s[W:2 * W] == base) and np.all(ref[W:2 * W] == base)
+
+    empty = graph.Csr(m=5, row_offsets=np.zeros(6, np.int32),
+                      column_indices=np.zeros(0, np.int32), degree=np.zeros(5, np.int32))
+    s, st, errs = _run(device, empty, 0.0, 2)
+    ref, it, _ = oracle.pr_push(5, empty.row_offsets, empty.column_indices, tol=0.0, max_iter=2)
+    assert st["iterations"] == it == 2
+    np.testing.assert_array_equal(s, ref)
