@@ -17,7 +17,7 @@
 
 namespace gd {
 
-constexpr int kSortMax = 16384;  // longest sorted segment (one LDS bitonic sort)
+constexpr int kSortMax = 16384;  // longest sorted segment (one workgroup's LDS radix sort)
 
 // Sorts the edges of every work segment of `s` by source: row tiles, and hub
 // tiles in chunks of `chunk` edges (whole-row exact hubs) or as they are

@@ -1,6 +1,8 @@
 // tile_sort.hip -- per-segment sort of edges by source (see tile_sort.h).
 #include "tile_sort.h"
 
+#include <rocprim/block/block_radix_sort.hpp>
+
 #include <algorithm>
 #include <vector>
 
@@ -8,54 +10,49 @@ namespace gd {
 
 namespace {
 
-constexpr int kSortBlock = 256;
+constexpr int kSortIPT = 16;  // keys per thread
 
-// One workgroup per segment [seg.x, seg.x + seg.y): keys (source << 14 |
-// offset) sorted by an LDS bitonic network, padded to a power of two with
-// UINT64_MAX.  Sorting the 64-bit key orders by source, then by offset.
-// seg.y < 0: the -seg.y edges stay in edge order.
-// MAXN: the longest segment of the launch (LDS of MAXN keys).
-template <int MAXN>
-__global__ __launch_bounds__(kSortBlock) void segment_sort_kernel(const int2* __restrict__ segs,
-                                                                  const int32_t* __restrict__ col,
-                                                                  int32_t* __restrict__ col_sorted,
-                                                                  uint16_t* __restrict__ pos) {
-  __shared__ uint64_t key[MAXN];
+// One workgroup of BS threads per segment [seg.x, seg.x + seg.y) of at most
+// BS * kSortIPT edges: the sources, with their offsets in the segment as
+// values, sorted by a stable LDS radix sort (rocPRIM's block radix sort; ties
+// stay in edge order).  Thread t holds the segment's edges [t * kSortIPT,
+// + kSortIPT); missing ones are padded with key 0xffffffff, which no source
+// exceeds and which, the pads coming last in input order, stays after every
+// real key in the stable sort.  seg.y < 0: the -seg.y edges stay in edge
+// order.
+template <int BS>
+__global__ __launch_bounds__(BS) void segment_sort_kernel(const int2* __restrict__ segs,
+                                                          const int32_t* __restrict__ col,
+                                                          int32_t* __restrict__ col_sorted,
+                                                          uint16_t* __restrict__ pos) {
+  using Sort = rocprim::block_radix_sort<uint32_t, BS, kSortIPT, uint16_t>;
+  __shared__ typename Sort::storage_type storage;
   const int2 sg = segs[blockIdx.x];
   const int e0 = sg.x, n = sg.y;
   if (n < 0) {
-    for (int i = threadIdx.x; i < -n; i += kSortBlock) {
+    for (int i = threadIdx.x; i < -n; i += BS) {
       col_sorted[e0 + i] = col[e0 + i];
       pos[e0 + i] = (uint16_t)i;
     }
     return;
   }
-  int np = 1;
-  while (np < n) np <<= 1;
-  for (int i = threadIdx.x; i < np; i += kSortBlock)
-    key[i] = (i < n) ? (((uint64_t)(uint32_t)col[e0 + i] << 14) | (uint64_t)i) : ~0ull;
-  __syncthreads();
-  for (int k = 2; k <= np; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < np; i += kSortBlock) {
-        const int l = i ^ j;
-        if (l > i) {
-          const uint64_t a = key[i], b = key[l];
-          const bool up = (i & k) == 0;
-          if ((a > b) == up) {
-            key[i] = b;
-            key[l] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
+  uint32_t key[kSortIPT];
+  uint16_t off[kSortIPT];
+#pragma unroll
+  for (int i = 0; i < kSortIPT; i++) {
+    const int k = threadIdx.x * kSortIPT + i;
+    key[i] = k < n ? (uint32_t)col[e0 + k] : 0xffffffffu;
+    off[i] = (uint16_t)k;
   }
-  // every thread read its keys before the last barrier; col may alias col_sorted
-  for (int i = threadIdx.x; i < n; i += kSortBlock) {
-    const uint64_t v = key[i];
-    col_sorted[e0 + i] = (int32_t)(v >> 14);
-    pos[e0 + i] = (uint16_t)(v & 0x3fffu);
+  // every key is read before the sort's first barrier; col may alias col_sorted
+  Sort().sort(key, off, storage);
+#pragma unroll
+  for (int i = 0; i < kSortIPT; i++) {
+    const int k = threadIdx.x * kSortIPT + i;
+    if (k < n) {
+      col_sorted[e0 + k] = (int32_t)key[i];
+      pos[e0 + k] = off[i];
+    }
   }
 }
 
@@ -87,15 +84,17 @@ int build_tile_sort(const Schedule& s, int chunk, const int32_t* d_col, int32_t*
   if (rc) return rc;
   GD_HIP_TRY(hipMemcpyAsync(d_segs.ptr, segs.data(), sizeof(int2) * segs.size(),
                             hipMemcpyHostToDevice, stream));
-  if (chunk > 8192)  // 128 KB of keys: one workgroup per CU
-    hipLaunchKernelGGL(segment_sort_kernel<kSortMax>, dim3((unsigned)segs.size()),
-                       dim3(kSortBlock), 0, stream, d_segs.as<int2>(), d_col, d_col_sorted, d_pos);
-  else if (chunk > 4096)  // 64 KB of keys
-    hipLaunchKernelGGL(segment_sort_kernel<8192>, dim3((unsigned)segs.size()), dim3(kSortBlock), 0,
-                       stream, d_segs.as<int2>(), d_col, d_col_sorted, d_pos);
+  static_assert(1024 * kSortIPT == kSortMax, "the widest sort covers kSortMax edges");
+  const dim3 grid((unsigned)segs.size());
+  if (chunk > 8192)
+    hipLaunchKernelGGL(segment_sort_kernel<1024>, grid, dim3(1024), 0, stream, d_segs.as<int2>(),
+                       d_col, d_col_sorted, d_pos);
+  else if (chunk > 4096)
+    hipLaunchKernelGGL(segment_sort_kernel<512>, grid, dim3(512), 0, stream, d_segs.as<int2>(),
+                       d_col, d_col_sorted, d_pos);
   else
-    hipLaunchKernelGGL(segment_sort_kernel<4096>, dim3((unsigned)segs.size()), dim3(kSortBlock), 0,
-                       stream, d_segs.as<int2>(), d_col, d_col_sorted, d_pos);
+    hipLaunchKernelGGL(segment_sort_kernel<256>, grid, dim3(256), 0, stream, d_segs.as<int2>(),
+                       d_col, d_col_sorted, d_pos);
   GD_HIP_TRY(hipGetLastError());
   GD_HIP_TRY(hipStreamSynchronize(stream));  // d_segs is freed on return
   return GD_OK;
